@@ -1,0 +1,307 @@
+"""ctypes binding of libpartia_mi355x.so (include/partia_mi355x.h).
+
+The shared library is the product: there is no Python or CPU fallback.  If it is missing
+or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpartia_mi355x.so")
+
+PMX_OK, PMX_ERR_ARGUMENT, PMX_ERR_DIMENSION, PMX_ERR_STATE, PMX_ERR_HIP, PMX_ERR_NOMEM, PMX_ERR_CAPACITY = (
+    0, -1, -2, -3, -4, -5, -6)
+PMX_F32, PMX_F64 = 0, 1
+PMX_MEM_HOST, PMX_MEM_DEVICE = 0, 1
+PMX_GATHER, PMX_SCATTER, PMX_SYMMETRIC = 0, 1, 2
+PMX_PREC_F64, PMX_PREC_F32C = 0, 1
+PMX_BUILD_WRITEBACK = 1
+
+EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_build_lbvh",
+           "pmx_interpolate_points", "pmx_interpolate_lines", "pmx_point_samples_interpolation",
+           "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
+           "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
+           "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings")
+
+
+class PmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[pmx {code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class ArgumentError(PmxError, ValueError):
+    """≙ Julia ArgumentError."""
+
+
+class DimensionMismatch(PmxError, ValueError):
+    """≙ Julia DimensionMismatch."""
+
+
+def _raise(code, msg):
+    if code == PMX_ERR_ARGUMENT:
+        raise ArgumentError(code, msg)
+    if code == PMX_ERR_DIMENSION:
+        raise DimensionMismatch(code, msg)
+    raise PmxError(code, msg)
+
+
+class Particles(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("mem", C.c_int32), ("n", C.c_int64),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("m", C.c_void_p), ("h", C.c_void_p), ("rho", C.c_void_p),
+                ("ncol", C.c_int32), ("quant", C.POINTER(C.c_void_p)), ("kernel", C.c_int32)]
+
+
+class Catalog(C.Structure):
+    _fields_ = [("n_scalar", C.c_int32), ("scalar_slots", C.c_void_p), ("scalar_shepard", C.c_void_p),
+                ("n_grad", C.c_int32), ("grad_slots", C.c_void_p),
+                ("n_div", C.c_int32), ("div_slots", C.c_void_p),
+                ("n_curl", C.c_int32), ("curl_slots", C.c_void_p)]
+
+
+class Points(C.Structure):
+    _fields_ = [("mem", C.c_int32), ("n", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p)]
+
+
+class Lines(C.Structure):
+    _fields_ = [("mem", C.c_int32), ("n", C.c_int64), ("ox", C.c_void_p), ("oy", C.c_void_p),
+                ("oz", C.c_void_p), ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p)]
+
+
+class Output(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("mem", C.c_int32), ("cols", C.POINTER(C.c_void_p))]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -m distorteddustydiscanalysistoolkits_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.pmx_create.argtypes = [C.POINTER(vp), i32]
+        L.pmx_destroy.argtypes = [vp]
+        L.pmx_destroy.restype = None
+        L.pmx_last_error.argtypes = [vp]
+        L.pmx_last_error.restype = C.c_char_p
+        L.pmx_version.restype = C.c_char_p
+        L.pmx_build_lbvh.argtypes = [vp, C.POINTER(Particles), i32, vp, vp]
+        L.pmx_interpolate_points.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Points), C.POINTER(Output)]
+        L.pmx_interpolate_lines.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Lines), C.POINTER(Output)]
+        L.pmx_point_samples_interpolation.argtypes = [vp, C.POINTER(Particles), i32, i32, C.POINTER(Catalog),
+                                                      C.POINTER(Points), C.POINTER(Output)]
+        L.pmx_line_samples_interpolation.argtypes = [vp, C.POINTER(Particles), i32, i32, C.POINTER(Catalog),
+                                                     C.POINTER(Lines), C.POINTER(Output)]
+        L.pmx_export_tree.argtypes = [vp, vp, vp, vp, vp]
+        L.pmx_export_nodes.argtypes = [vp, vp, vp, vp]
+        L.pmx_export_particles.argtypes = [vp, C.POINTER(vp), i32]
+        L.pmx_nearest.argtypes = [vp, i32, C.POINTER(Points), vp]
+        L.pmx_point_neighbors.argtypes = [vp, i32, i32, C.POINTER(Points), vp, vp, vp, vp, i64]
+        L.pmx_line_neighbors.argtypes = [vp, i32, i32, C.POINTER(Lines), vp, vp, vp, vp, i64]
+        L.pmx_kernel_eval.argtypes = [vp, i32, i64, vp, vp, vp]
+        L.pmx_line_kernel_table.argtypes = [i32, vp, vp]
+        L.pmx_last_timings.argtypes = [vp, vp, i32]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    """Address of a numpy array or a torch tensor (device memory)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _mem_of(arrs):
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and any(isinstance(a, torch.Tensor) for a in arrs):
+        if not all(isinstance(a, torch.Tensor) and a.is_cuda for a in arrs):
+            raise ValueError("mixing host arrays and tensors, or CPU tensors, is not supported")
+        return PMX_MEM_DEVICE
+    return PMX_MEM_HOST
+
+
+def _dtype_code(a):
+    s = str(a.dtype)
+    if s in ("float64", "torch.float64"):
+        return PMX_F64
+    if s in ("float32", "torch.float32"):
+        return PMX_F32
+    raise TypeError(f"unsupported element type {a.dtype}")
+
+
+def line_kernel_table(kernel: int):
+    Q = np.empty(4096)
+    I = np.empty(4096)
+    rc = lib().pmx_line_kernel_table(int(kernel), Q.ctypes.data, I.ctypes.data)
+    if rc != PMX_OK:
+        _raise(rc, "pmx_line_kernel_table")
+    return Q, I
+
+
+def catalog_struct(scalar_slots=(), scalar_shepard=None, grad_slots=(), div_slots=(), curl_slots=()):
+    """pmx_catalog from 0-based slots; keeps the backing arrays alive on the struct."""
+    ss = np.ascontiguousarray(scalar_slots, np.int32).ravel()
+    sh = np.ascontiguousarray([1] * ss.size if scalar_shepard is None else scalar_shepard, np.uint8).ravel()
+    gs = np.ascontiguousarray(grad_slots, np.int32).ravel()
+    ds = np.ascontiguousarray(div_slots, np.int32).ravel()
+    cs = np.ascontiguousarray(curl_slots, np.int32).ravel()
+    cat = Catalog(ss.size, ss.ctypes.data, sh.ctypes.data, gs.size, gs.ctypes.data, ds.size // 3,
+                  ds.ctypes.data, cs.size // 3, cs.ctypes.data)
+    cat._keep = (ss, sh, gs, ds, cs)
+    return cat
+
+
+class Context:
+    """One pmx_ctx: a HIP stream plus the device-resident LBVH and particle columns."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.pmx_create(C.byref(h), int(device))
+        if rc != PMX_OK:
+            raise PmxError(rc, "pmx_create failed (no usable HIP device?)")
+        self._h = h
+        self.n = 0
+        self.ncol = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != PMX_OK:
+            _raise(rc, lib().pmx_last_error(self._h).decode())
+
+    # ---- build ----
+    def build(self, x, y, z, m, h, rho, quant=(), kernel=0, writeback=False, want_codes=False,
+              want_order=False):
+        cols = [x, y, z, m, h, rho] + list(quant)
+        mem = _mem_of(cols)
+        dt = _dtype_code(x)
+        n = int(x.shape[0])
+        for c in cols:
+            if _dtype_code(c) != dt:
+                raise TypeError("all particle columns must share one element type")
+        qp = (C.c_void_p * max(1, len(quant)))(*[_ptr(q) for q in quant])
+        p = Particles(dt, mem, n, _ptr(x), _ptr(y), _ptr(z), _ptr(m), _ptr(h), _ptr(rho), len(quant),
+                      C.cast(qp, C.POINTER(C.c_void_p)), int(kernel))
+        codes = np.empty(n, np.uint64) if want_codes else None
+        order = np.empty(n, np.int64) if want_order else None
+        rc = lib().pmx_build_lbvh(self._h, C.byref(p), PMX_BUILD_WRITEBACK if writeback else 0,
+                                  _ptr(codes), _ptr(order))
+        self._check(rc)
+        self.n, self.ncol = n, len(quant)
+        return codes, order
+
+    # ---- interpolation ----
+    def _points(self, qx, qy, qz):
+        mem = _mem_of([qx, qy, qz])
+        return Points(mem, int(qx.shape[0]), _ptr(qx), _ptr(qy), _ptr(qz))
+
+    def _lines(self, origin, direction):
+        arrs = list(origin) + list(direction)
+        mem = _mem_of(arrs)
+        return Lines(mem, int(origin[0].shape[0]), *[_ptr(a) for a in arrs])
+
+    def interpolate_points(self, strategy, precision, cat, qx, qy, qz, out_cols):
+        pts = self._points(qx, qy, qz)
+        mem = _mem_of(out_cols) if out_cols else PMX_MEM_HOST
+        dt = _dtype_code(out_cols[0]) if out_cols else PMX_F64
+        cp = (C.c_void_p * max(1, len(out_cols)))(*[_ptr(o) for o in out_cols])
+        out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
+        self._check(lib().pmx_interpolate_points(self._h, int(strategy), int(precision), C.byref(cat),
+                                                 C.byref(pts), C.byref(out)))
+
+    def interpolate_lines(self, strategy, precision, cat, origin, direction, out_cols):
+        ln = self._lines(origin, direction)
+        mem = _mem_of(out_cols) if out_cols else PMX_MEM_HOST
+        dt = _dtype_code(out_cols[0]) if out_cols else PMX_F64
+        cp = (C.c_void_p * max(1, len(out_cols)))(*[_ptr(o) for o in out_cols])
+        out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
+        self._check(lib().pmx_interpolate_lines(self._h, int(strategy), int(precision), C.byref(cat),
+                                                C.byref(ln), C.byref(out)))
+
+    # ---- inspection ----
+    def export_tree(self):
+        t = 2 * self.n - 1
+        arrs = [np.empty(t, np.int32) for _ in range(4)]
+        self._check(lib().pmx_export_tree(self._h, *[a.ctypes.data for a in arrs]))
+        return arrs  # left, right, parent, escape
+
+    def export_nodes(self):
+        m = max(self.n - 1, 0)
+        mn, mx, hm = np.empty((3, m)), np.empty((3, m)), np.empty(m)
+        self._check(lib().pmx_export_nodes(self._h, mn.ctypes.data, mx.ctypes.data, hm.ctypes.data))
+        return mn, mx, hm
+
+    def export_particles(self):
+        k = 6 + self.ncol
+        arrs = [np.empty(self.n) for _ in range(k)]
+        cp = (C.c_void_p * k)(*[a.ctypes.data for a in arrs])
+        self._check(lib().pmx_export_particles(self._h, cp, k))
+        return arrs
+
+    def nearest(self, qx, qy, qz, precision=PMX_PREC_F64):
+        pts = self._points(qx, qy, qz)
+        idx = np.empty(int(qx.shape[0]), np.int64)
+        self._check(lib().pmx_nearest(self._h, int(precision), C.byref(pts), idx.ctypes.data))
+        return idx
+
+    def point_neighbors(self, strategy, qx, qy, qz, ha=None, precision=PMX_PREC_F64):
+        pts = self._points(qx, qy, qz)
+        n = int(qx.shape[0])
+        counts = np.empty(n, np.int64)
+        hap = None if ha is None else np.ascontiguousarray(ha, np.float64)
+        self._check(lib().pmx_point_neighbors(self._h, int(strategy), int(precision), C.byref(pts), _ptr(hap),
+                                              counts.ctypes.data, None, None, 0))
+        total = int(counts.sum())
+        off = np.empty(n + 1, np.int64)
+        idx = np.empty(max(total, 1), np.int32)
+        self._check(lib().pmx_point_neighbors(self._h, int(strategy), int(precision), C.byref(pts), _ptr(hap),
+                                              counts.ctypes.data, off.ctypes.data, idx.ctypes.data, total))
+        return off, idx[:total]
+
+    def line_neighbors(self, strategy, origin, direction, ha=None, precision=PMX_PREC_F64):
+        ln = self._lines(origin, direction)
+        n = int(origin[0].shape[0])
+        counts = np.empty(n, np.int64)
+        hap = None if ha is None else np.ascontiguousarray(ha, np.float64)
+        self._check(lib().pmx_line_neighbors(self._h, int(strategy), int(precision), C.byref(ln), _ptr(hap),
+                                             counts.ctypes.data, None, None, 0))
+        total = int(counts.sum())
+        off = np.empty(n + 1, np.int64)
+        idx = np.empty(max(total, 1), np.int32)
+        self._check(lib().pmx_line_neighbors(self._h, int(strategy), int(precision), C.byref(ln), _ptr(hap),
+                                             counts.ctypes.data, off.ctypes.data, idx.ctypes.data, total))
+        return off, idx[:total]
+
+    def kernel_eval(self, kernel, q):
+        q = np.ascontiguousarray(q, np.float64)
+        f, df = np.empty_like(q), np.empty_like(q)
+        self._check(lib().pmx_kernel_eval(self._h, int(kernel), q.size, q.ctypes.data, f.ctypes.data,
+                                          df.ctypes.data))
+        return f, df
+
+    def timings(self):
+        t = np.zeros(8)
+        lib().pmx_last_timings(self._h, t.ctypes.data, 8)
+        return t

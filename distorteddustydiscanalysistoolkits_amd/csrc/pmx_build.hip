@@ -1,0 +1,764 @@
+// pmx_build.hip — LBVH build on gfx950: bounding box, 63-bit Morton codes, stable radix
+// sort, code de-duplication, Karras radix tree with escape links, atomic bottom-up refit,
+// and the gather of every particle column into Morton order.
+//
+// Reference: LinearBVH!(input, Val(3)) (Partia.jl/src/julia/KernelInterpolation/
+// interpolation_setup/InterpolationInput.jl:203-224) and the NeighborSearch files cited at
+// each kernel.  Codes, the sort permutation and the tree arrays are bit-identical to the
+// reference's: the normalisation and quantisation run in FP64 without contraction, the
+// sort is stable (the reference's sortperm breaks ties by index), and the de-dup scan is
+// the closed form of the reference's serial loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "pmx_internal.h"
+
+namespace pmx {
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------------
+// Bounding box of x, y, z (Base.minimum / Base.maximum in _normalize_vector,
+// MortonCode.jl:443-456) and a NaN count (the reference throws for NaN coordinates).
+// ---------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bbox_partial(const T* __restrict__ x, const T* __restrict__ y,
+                                                         const T* __restrict__ z, int64_t n,
+                                                         double* __restrict__ part, int* __restrict__ nan_count) {
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    int nans = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (v[k] != v[k]) { ++nans; continue; }
+            mn[k] = v[k] < mn[k] ? v[k] : mn[k];
+            mx[k] = v[k] > mx[k] ? v[k] : mx[k];
+        }
+    }
+    __shared__ double s[6][kBlock];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double a = s[k][threadIdx.x + w];
+                if (a < s[k][threadIdx.x]) s[k][threadIdx.x] = a;
+                double b = s[3 + k][threadIdx.x + w];
+                if (b > s[3 + k][threadIdx.x]) s[3 + k][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[threadIdx.x * gridDim.x + blockIdx.x] = s[threadIdx.x][0];
+    if (nans) atomicAdd(nan_count, nans);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bbox_final(const double* __restrict__ part, int nparts,
+                                                       double* __restrict__ box) {
+    __shared__ double s[6][kBlock];
+    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double a = part[k * nparts + i], b = part[(3 + k) * nparts + i];
+            mn[k] = a < mn[k] ? a : mn[k];
+            mx[k] = b > mx[k] ? b : mx[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double a = s[k][threadIdx.x + w];
+                if (a < s[k][threadIdx.x]) s[k][threadIdx.x] = a;
+                double b = s[3 + k][threadIdx.x + w];
+                if (b > s[3 + k][threadIdx.x]) s[3 + k][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) box[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// _expand_bits3D(::UInt64), MortonCode.jl:209-217.
+__device__ inline uint64_t expand_bits3(uint64_t x) {
+    x = (x | (x << 32)) & 0x1f00000000ffffULL;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+    x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+    x = (x | (x << 2)) & 0x1249249249249249ULL;
+    return x;
+}
+
+// FP64 quantisation of one coordinate: f = (v - vmin) * inv(vmax - vmin), 0.5 on a flat
+// axis; i = UInt64(floor((exp2(21) - 1) * f)) (MortonCode.jl:436-441, 473-498).
+__device__ inline uint64_t quantize(double v, double vmin, double vmax) {
+    const double scale = 2097151.0;
+    double dv = vmax - vmin;
+    double f;
+    if (dv == 0.0) {
+        f = 0.5;
+    } else {
+        double inv = 1.0 / dv;
+        f = (v - vmin) * inv;
+    }
+    return (uint64_t)floor(scale * f);
+}
+
+// _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez; value = index.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, const T* __restrict__ y,
+                                                   const T* __restrict__ z, int64_t n,
+                                                   const double* __restrict__ box,
+                                                   uint64_t* __restrict__ codes, int32_t* __restrict__ vals) {
+    const double x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        uint64_t ix = quantize((double)x[i], x0, x1);
+        uint64_t iy = quantize((double)y[i], y0, y1);
+        uint64_t iz = quantize((double)z[i], z0, z1);
+        codes[i] = (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
+        vals[i] = (int32_t)i;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Stable LSD radix sort of (u64 key, i32 value) pairs, 8-bit digits.  Per pass:
+//   k_radix_hist     per-tile digit counts, stored digit-major: hist[d * ntiles + tile]
+//   k_radix_scan     one workgroup per digit: exclusive scan over tiles, digit totals
+//   k_radix_scatter  tile loaded to registers, stable in-tile ranking with wavefront
+//                    ballots, keys staged in LDS in digit order, then written out so that
+//                    consecutive lanes store consecutive addresses of each digit run.
+// Stability: items are ranked in (round, wave, lane) order = input order within a tile,
+// and tiles are laid out in order inside each digit.
+// ---------------------------------------------------------------------------------
+constexpr int kRadixBits = 8;
+constexpr int kBins = 1 << kRadixBits;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;   // 4096
+constexpr int kWaves = kBlock / 64;
+
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                       int ntiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kBins];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        int64_t i = base + r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kBins - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Block-wide exclusive scan of one uint32 per thread; returns the block total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& excl) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) s_tmp[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+        uint32_t t = s_tmp[w];
+        if (w < wave) wbase += t;
+        total += t;
+    }
+    __syncthreads();
+    excl = wbase + inc - v;
+    return total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* __restrict__ hist, int ntiles,
+                                                       uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_tmp[kWaves];
+    uint32_t* row = hist + (int64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int base = 0; base < ntiles; base += kBlock * 4) {
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int i = base + threadIdx.x * 4 + j;
+            v[j] = i < ntiles ? row[i] : 0u;
+            sum += v[j];
+        }
+        uint32_t excl;
+        uint32_t total = block_exclusive_scan(sum, s_tmp, excl);
+        uint32_t run = carry + excl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int i = base + threadIdx.x * 4 + j;
+            if (i < ntiles) row[i] = run;
+            run += v[j];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __restrict__ keys_in,
+                                                          const int32_t* __restrict__ vals_in, int64_t n,
+                                                          int shift, int ntiles,
+                                                          const uint32_t* __restrict__ hist,
+                                                          const uint32_t* __restrict__ totals,
+                                                          uint64_t* __restrict__ keys_out,
+                                                          int32_t* __restrict__ vals_out) {
+    __shared__ uint64_t s_keys[kTile];
+    __shared__ int32_t s_vals[kTile];
+    __shared__ uint32_t s_gofs[kBins];     // global destination of this tile's digit run
+    __shared__ uint32_t s_lstart[kBins];   // start of the digit run inside the tile
+    __shared__ uint32_t s_run[kBins];      // items of the digit already placed (rounds so far)
+    __shared__ uint32_t s_wc[kWaves][kBins];
+    __shared__ uint32_t s_tmp[kWaves];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+
+    // global offsets: digit base (scan of totals) + this tile's prefix within the digit
+    uint32_t tot = totals[t], dbase;
+    block_exclusive_scan(tot, s_tmp, dbase);
+    s_gofs[t] = dbase + hist[(int64_t)t * ntiles + blockIdx.x];
+    s_run[t] = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_wc[w][t] = 0;
+
+    uint64_t k[kItems];
+    int32_t v[kItems];
+    int dg[kItems];
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        int64_t i = base + r * kBlock + t;
+        if (i < n) {
+            k[r] = keys_in[i];
+            v[r] = vals_in[i];
+            dg[r] = (int)((k[r] >> shift) & (kBins - 1));
+        } else {
+            k[r] = 0;
+            v[r] = 0;
+            dg[r] = -1;
+        }
+    }
+    // in-tile digit counts -> run starts
+    __shared__ uint32_t s_cnt[kBins];
+    s_cnt[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r)
+        if (dg[r] >= 0) atomicAdd(&s_cnt[dg[r]], 1u);
+    __syncthreads();
+    uint32_t lst;
+    block_exclusive_scan(s_cnt[t], s_tmp, lst);
+    s_lstart[t] = lst;
+    __syncthreads();
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const int d = dg[r];
+        uint64_t peers = __ballot(d >= 0);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            uint64_t bal = __ballot(d >= 0 && ((d >> b) & 1));
+            peers &= ((d >> b) & 1) ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        if (d >= 0 && rank == 0) s_wc[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (d >= 0) {
+            uint32_t pos = s_run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += s_wc[w][d];
+            const uint32_t slot = s_lstart[d] + pos;
+            s_keys[slot] = k[r];
+            s_vals[slot] = v[r];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) { add += s_wc[w][t]; s_wc[w][t] = 0; }
+        s_run[t] += add;
+        __syncthreads();
+    }
+    const int64_t count = (n - base) < kTile ? (n - base) : kTile;
+    for (int j = t; j < count; j += kBlock) {
+        uint64_t key = s_keys[j];
+        int d = (int)((key >> shift) & (kBins - 1));
+        uint32_t dst = s_gofs[d] + (uint32_t)(j - s_lstart[d]);
+        if (dst < (uint64_t)n) {   // always true for a consistent histogram; keeps stores in bounds
+            keys_out[dst] = key;
+            vals_out[dst] = s_vals[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// De-duplication (sort_by_morton!, MortonCode.jl:189-193): c'_i = max(c_i, c'_{i-1} + 1)
+// has the closed form c'_i = i + max_{j <= i}(c_j - j), computed as a max-scan.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_dedup_tilemax(const uint64_t* __restrict__ c, int64_t n,
+                                                          int64_t* __restrict__ tmax) {
+    __shared__ int64_t s[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    int64_t m = INT64_MIN;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        int64_t i = base + r * kBlock + threadIdx.x;
+        if (i < n) { int64_t d = (int64_t)c[i] - i; m = d > m ? d : m; }
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tmax[blockIdx.x] = s[0];
+}
+
+// exclusive max-scan of tile maxima (single workgroup)
+__global__ void __launch_bounds__(kBlock) k_dedup_scan(int64_t* __restrict__ tmax, int ntiles) {
+    __shared__ int64_t s[kBlock];
+    int64_t carry = INT64_MIN;
+    for (int base = 0; base < ntiles; base += kBlock) {
+        int i = base + threadIdx.x;
+        int64_t v = i < ntiles ? tmax[i] : INT64_MIN;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kBlock; o <<= 1) {
+            int64_t t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : INT64_MIN;
+            __syncthreads();
+            if (t > s[threadIdx.x]) s[threadIdx.x] = t;
+            __syncthreads();
+        }
+        int64_t excl = threadIdx.x == 0 ? INT64_MIN : s[threadIdx.x - 1];
+        excl = carry > excl ? carry : excl;
+        if (i < ntiles) tmax[i] = excl;
+        int64_t last = s[kBlock - 1];
+        __syncthreads();
+        carry = last > carry ? last : carry;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_dedup_apply(uint64_t* __restrict__ c, int64_t n,
+                                                        const int64_t* __restrict__ tpre) {
+    __shared__ int64_t s[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+    int64_t d[kItems];
+    int64_t m = INT64_MIN;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        int64_t i = base + r;
+        d[r] = i < n ? (int64_t)c[i] - i : INT64_MIN;
+        m = d[r] > m ? d[r] : m;
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        int64_t t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : INT64_MIN;
+        __syncthreads();
+        if (t > s[threadIdx.x]) s[threadIdx.x] = t;
+        __syncthreads();
+    }
+    int64_t run = threadIdx.x == 0 ? INT64_MIN : s[threadIdx.x - 1];
+    int64_t pre = tpre[blockIdx.x];
+    run = pre > run ? pre : run;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        int64_t i = base + r;
+        run = d[r] > run ? d[r] : run;
+        if (i < n) c[i] = (uint64_t)(run + i);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Column gather into Morton order (the permute! calls of InterpolationInput.jl:211-219),
+// converting to FP64.  Column c of the output lives at cols64 + c * n.
+// ---------------------------------------------------------------------------------
+struct ColPtrs {
+    const void* p[24];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gather(ColPtrs src, int ncols, const int32_t* __restrict__ order,
+                                                   int64_t n, double* __restrict__ cols64) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int32_t o = order[i];
+        o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;   // in-bounds guard (order is a permutation)
+        for (int c = 0; c < ncols; ++c) cols64[(int64_t)c * n + i] = (double)static_cast<const T*>(src.p[c])[o];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_leaf(const double* __restrict__ cols64, int64_t n,
+                                                      Leaf64* __restrict__ leaf) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        Leaf64 L;
+        L.x = cols64[i];
+        L.y = cols64[n + i];
+        L.z = cols64[2 * n + i];
+        L.h = cols64[4 * n + i];
+        leaf[i] = L;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Karras radix tree (BinaryRadixTree.jl:81-281).  Unified ids: internal 1..n-1, leaf k
+// (1-based) -> n-1+k.  All index arithmetic below is 1-based as in the reference.
+// ---------------------------------------------------------------------------------
+__device__ inline int delta(const uint64_t* __restrict__ c, int64_t i, int64_t j) {
+    // _longest_common_prefix_length(codes, i, j), MortonCode.jl:380-387 (1-based)
+    uint64_t a = c[i - 1], b = c[j - 1];
+    if (a == b) return __clzll((uint64_t)i ^ (uint64_t)j);
+    return __clzll(a ^ b);
+}
+
+// _build_child! (BinaryRadixTree.jl:209-243) with _find_range (:148-179) and
+// _split_position (:181-202).  lr holds left at 2*id and right at 2*id+1.
+__global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
+                                                int32_t* __restrict__ lr, int32_t* __restrict__ parent,
+                                                int32_t* __restrict__ range_hi) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n - 1;
+         i += (int64_t)gridDim.x * kBlock) {
+        int dl = (i > 1) ? delta(c, i, i - 1) : -1;
+        int dr = (i < n) ? delta(c, i, i + 1) : -1;
+        int64_t d = (dr - dl > 0) ? 1 : ((dr - dl < 0) ? -1 : 1);
+        int dmin = (1 <= i - d && i - d <= n) ? delta(c, i, i - d) : -1;
+        int64_t lmax = 2;
+        for (;;) {
+            int64_t j = i + lmax * d;
+            if (j < 1 || j > n) break;
+            if (delta(c, i, j) <= dmin) break;
+            lmax *= 2;
+        }
+        int64_t l = 0;
+        for (int64_t t = lmax >> 1; t > 0; t >>= 1) {
+            int64_t j = i + (l + t) * d;
+            if (j >= 1 && j <= n && delta(c, i, j) > dmin) l += t;
+        }
+        int64_t j = i + l * d;
+        int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+        range_hi[i] = (int32_t)hi;
+        int64_t split;
+        if (c[lo - 1] == c[hi - 1]) {
+            split = (lo + hi) >> 1;
+            split = split < lo ? lo : (split > hi - 1 ? hi - 1 : split);
+        } else {
+            int prefix = delta(c, lo, hi);
+            split = lo;
+            int64_t step = hi - lo;
+            while (step > 1) {
+                step = (step + 1) >> 1;
+                int64_t ns = split + step;
+                if (ns < hi && delta(c, lo, ns) > prefix) split = ns;
+            }
+        }
+        const int64_t off = n - 1;
+        int64_t left = (split == lo) ? lo + off : split;
+        int64_t right = (split + 1 == hi) ? hi + off : split + 1;
+        lr[2 * i] = (int32_t)left;
+        lr[2 * i + 1] = (int32_t)right;
+        parent[left] = (int32_t)i;
+        parent[right] = (int32_t)i;
+    }
+}
+
+// _build_adjacent! (BinaryRadixTree.jl:204-207); adj[n] = typemax(Int32).
+__global__ void __launch_bounds__(kBlock) k_adj(const uint64_t* __restrict__ c, int64_t n, int32_t* __restrict__ adj) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n; i += (int64_t)gridDim.x * kBlock)
+        adj[i] = (i == n) ? 2147483647 : delta(c, i, i + 1);
+}
+
+// Internal escapes, Algorithm 2 of Prokopenko & Lebrun-Grandie (BinaryRadixTree.jl:245-260).
+__global__ void __launch_bounds__(kBlock) k_escape_internal(const int32_t* __restrict__ range_hi,
+                                                            const int32_t* __restrict__ adj, int64_t n,
+                                                            int32_t* __restrict__ escape) {
+    const int64_t off = n - 1;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n - 1;
+         i += (int64_t)gridDim.x * kBlock) {
+        int64_t hi = range_hi[i];
+        int32_t e;
+        if (hi == n) e = 0;
+        else if (hi == n - 1) e = (int32_t)(off + n);
+        else {
+            int64_t next = hi + 1;
+            e = (adj[next] < adj[next - 1]) ? (int32_t)(off + next) : (int32_t)next;
+        }
+        escape[i] = e;
+    }
+}
+
+// Leaf escapes (BinaryRadixTree.jl:262-281): left child -> right sibling, right child ->
+// escape of the parent.  Independent per leaf once the internal escapes exist.
+__global__ void __launch_bounds__(kBlock) k_escape_leaf(const int32_t* __restrict__ lr,
+                                                        const int32_t* __restrict__ parent, int64_t n,
+                                                        int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc) {
+    for (int64_t li = (int64_t)blockIdx.x * kBlock + threadIdx.x; li < n; li += (int64_t)gridDim.x * kBlock) {
+        int64_t id = n + li;   // = (n - 1) + (li + 1)
+        int32_t p = parent[id];
+        int32_t e;
+        if (p == 0) e = 0;
+        else e = (lr[2 * p] == (int32_t)id) ? lr[2 * p + 1] : escape[p];
+        escape[id] = e;
+        leaf_esc[li] = e;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// Atomic bottom-up refit (LinearBVH.jl:78-80, _build_internal_aabb! :98-156): one climber
+// per leaf; the second arrival at an internal node combines both children with strict
+// comparisons (ifelse(<) / ifelse(>)), writes the node and climbs on.  A climber that has
+// written a node releases it (agent-scope fence) before its next arrival; the second
+// arrival acquires before reading its children.
+__global__ void __launch_bounds__(kBlock) k_refit(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                                  const int32_t* __restrict__ parent,
+                                                  const int32_t* __restrict__ escape, int64_t n,
+                                                  uint32_t* __restrict__ visited, Node64* nodes) {
+    for (int64_t li = (int64_t)blockIdx.x * kBlock + threadIdx.x; li < n; li += (int64_t)gridDim.x * kBlock) {
+        int32_t p = parent[n + li];
+        bool wrote = false;
+        int64_t steps = 0;
+        while (p > 0 && p < n && ++steps <= n) {   // a parent chain is at most n-1 long
+            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            uint32_t old = __hip_atomic_fetch_add(&visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int32_t l = lr[2 * p], r = lr[2 * p + 1];
+            double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
+            if (l >= n) {
+                const Leaf64 L = leaf[l - n];
+                lmn[0] = lmx[0] = L.x; lmn[1] = lmx[1] = L.y; lmn[2] = lmx[2] = L.z; hl = L.h;
+            } else {
+                const Node64* N = &nodes[l];
+                for (int k = 0; k < 3; ++k) { lmn[k] = N->mn[k]; lmx[k] = N->mx[k]; }
+                hl = N->hmax;
+            }
+            if (r >= n) {
+                const Leaf64 R = leaf[r - n];
+                rmn[0] = rmx[0] = R.x; rmn[1] = rmx[1] = R.y; rmn[2] = rmx[2] = R.z; hr = R.h;
+            } else {
+                const Node64* N = &nodes[r];
+                for (int k = 0; k < 3; ++k) { rmn[k] = N->mn[k]; rmx[k] = N->mx[k]; }
+                hr = N->hmax;
+            }
+            Node64 out;
+            for (int k = 0; k < 3; ++k) {
+                out.mn[k] = lmn[k] < rmn[k] ? lmn[k] : rmn[k];
+                out.mx[k] = lmx[k] > rmx[k] ? lmx[k] : rmx[k];
+            }
+            out.hmax = hl > hr ? hl : hr;
+            out.left = l;
+            out.escape = escape[p];
+            nodes[p] = out;
+            wrote = true;
+            p = parent[p];
+        }
+    }
+}
+
+template <typename T>
+int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* y, const void* z,
+                 int64_t n, double* part, int nparts, int* nan_count, double* box, uint64_t* codes,
+                 int32_t* vals) {
+    hipLaunchKernelGGL(k_bbox_partial<T>, dim3(nparts), dim3(kBlock), 0, c->stream, (const T*)x, (const T*)y,
+                       (const T*)z, n, part, nan_count);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, part, nparts, box);
+    hipLaunchKernelGGL(k_encode<T>, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, c->stream, (const T*)x,
+                       (const T*)y, (const T*)z, n, box, codes, vals);
+    (void)p;
+    return PMX_OK;
+}
+
+}  // namespace
+
+int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
+                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt) {
+    const int ntiles = (int)((n + kTile - 1) / kTile);
+    PMX_TRY(ensure(c, c->hist, sizeof(uint32_t) * ((size_t)kBins * ntiles + kBins)));
+    uint32_t* hist = c->hist.as<uint32_t>();
+    uint32_t* totals = hist + (size_t)kBins * ntiles;
+    uint64_t *kin = keys, *kout = keys_alt;
+    int32_t *vin = vals, *vout = vals_alt;
+    bool alt = false;
+    for (int shift = begin_bit; shift < end_bit; shift += kRadixBits) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, n, shift, ntiles, hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(kBins), dim3(kBlock), 0, c->stream, hist, ntiles, totals);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, vin, n, shift, ntiles,
+                           hist, totals, kout, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        alt = !alt;
+    }
+    PMX_HIP(c, hipGetLastError());
+    *result_in_alt = alt;
+    return PMX_OK;
+}
+
+int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out, int64_t* order_out) {
+    if (!p) return set_error(c, PMX_ERR_ARGUMENT, "particles: null pointer");
+    const int64_t n = p->n;
+    if (n < 1)
+        return set_error(c, PMX_ERR_ARGUMENT, "BinaryRadixTree: enc.codes must be non-empty (got n=0).");
+    if (n > 1073741824LL)
+        return set_error(c, PMX_ERR_ARGUMENT, "BinaryRadixTree: n exceeds the Int32 node-id capacity (n <= 1073741824).");
+    if (p->dtype != PMX_F32 && p->dtype != PMX_F64) return set_error(c, PMX_ERR_ARGUMENT, "particles: dtype");
+    if (p->ncol < 0 || p->ncol > 18) return set_error(c, PMX_ERR_ARGUMENT, "particles: ncol must be 0..18");
+    if (p->kernel < 0 || p->kernel > 5) return set_error(c, PMX_ERR_ARGUMENT, "particles: unknown kernel");
+    if (!p->x || !p->y || !p->z || !p->m || !p->h || !p->rho || (p->ncol > 0 && !p->quant))
+        return set_error(c, PMX_ERR_ARGUMENT, "particles: null column pointer");
+    for (int j = 0; j < p->ncol; ++j)
+        if (!p->quant[j]) return set_error(c, PMX_ERR_ARGUMENT, "particles: null quant column");
+    const int ncols = 6 + p->ncol;
+    const size_t esz = p->dtype == PMX_F64 ? 8 : 4;
+    hipStream_t s = c->stream;
+    c->tree.built = false;
+
+    PMX_HIP(c, hipEventRecord(c->ev[0], s));
+    // ---- inputs on the device (host columns are staged) ----
+    const void* src[24];
+    const void* in[24] = {p->x, p->y, p->z, p->m, p->h, p->rho};
+    for (int j = 0; j < p->ncol; ++j) in[6 + j] = p->quant[j];
+    if (p->mem == PMX_MEM_HOST) {
+        PMX_TRY(ensure(c, c->staging, esz * (size_t)n * ncols));
+        for (int j = 0; j < ncols; ++j) {
+            void* dst = (char*)c->staging.p + esz * (size_t)n * j;
+            PMX_HIP(c, hipMemcpyAsync(dst, in[j], esz * (size_t)n, hipMemcpyHostToDevice, s));
+            src[j] = dst;
+        }
+    } else {
+        for (int j = 0; j < ncols; ++j) src[j] = in[j];
+    }
+    // ---- bbox + encode ----
+    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+    PMX_TRY(ensure(c, c->misc, sizeof(double) * (6 * 1024 + 8) + 64));
+    double* part = c->misc.as<double>();
+    double* box = part + 6 * 1024;
+    int* nan_count = reinterpret_cast<int*>(box + 6);
+    PMX_TRY(ensure(c, c->codes, 8 * (size_t)n));
+    PMX_TRY(ensure(c, c->keys_tmp, 8 * (size_t)n));
+    PMX_TRY(ensure(c, c->order, 4 * (size_t)n));
+    PMX_TRY(ensure(c, c->vals_tmp, 4 * (size_t)n));
+    uint64_t* codes = c->codes.as<uint64_t>();
+    int32_t* order = c->order.as<int32_t>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 1, 0u);
+    if (p->dtype == PMX_F64)
+        launch_front<double>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order);
+    else
+        launch_front<float>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order);
+    // ---- stable sort + de-dup ----
+    bool in_alt = false;
+    PMX_TRY(radix_sort_pairs(c, codes, order, n, c->keys_tmp.as<uint64_t>(), c->vals_tmp.as<int32_t>(), 0, 64,
+                             &in_alt));
+    if (in_alt) {
+        std::swap(c->codes, c->keys_tmp);
+        std::swap(c->order, c->vals_tmp);
+        codes = c->codes.as<uint64_t>();
+        order = c->order.as<int32_t>();
+    }
+    const int ntiles = (int)((n + kTile - 1) / kTile);
+    int64_t* tmax = reinterpret_cast<int64_t*>(c->keys_tmp.p);   // free after the sort
+    hipLaunchKernelGGL(k_dedup_tilemax, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
+    hipLaunchKernelGGL(k_dedup_scan, dim3(1), dim3(kBlock), 0, s, tmax, ntiles);
+    hipLaunchKernelGGL(k_dedup_apply, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
+    PMX_HIP(c, hipEventRecord(c->ev[1], s));
+    // ---- columns into Morton order ----
+    PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
+    double* cols64 = c->cols64.as<double>();
+    ColPtrs cp;
+    for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
+    const unsigned g = grid_for(n, kBlock, 16384);
+    if (p->dtype == PMX_F64)
+        hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, order, n, cols64);
+    else
+        hipLaunchKernelGGL(k_gather<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, order, n, cols64);
+    PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)n));
+    hipLaunchKernelGGL(k_pack_leaf, dim3(g), dim3(kBlock), 0, s, cols64, n, c->leaf64.as<Leaf64>());
+    PMX_HIP(c, hipEventRecord(c->ev[2], s));
+    // ---- radix tree, escapes, refit ----
+    PMX_TRY(ensure(c, c->lr, 8 * (size_t)(2 * n + 2)));
+    PMX_TRY(ensure(c, c->parent, 4 * (size_t)(2 * n + 2)));
+    PMX_TRY(ensure(c, c->escape, 4 * (size_t)(2 * n + 2)));
+    PMX_TRY(ensure(c, c->range_hi, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->adj, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->visited, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->leaf_esc, 4 * (size_t)n));
+    PMX_TRY(ensure(c, c->nodes64, sizeof(Node64) * (size_t)(n + 1)));
+    int32_t* lr = c->lr.as<int32_t>();
+    int32_t* parent = c->parent.as<int32_t>();
+    int32_t* escape = c->escape.as<int32_t>();
+    const unsigned g2 = grid_for(2 * n + 2, kBlock, 16384);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)lr, 4 * n + 4, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)parent, 2 * n + 2, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)escape, 2 * n + 2, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, c->visited.as<uint32_t>(), n + 2, 0u);
+    if (n >= 2) {
+        hipLaunchKernelGGL(k_brt, dim3(g), dim3(kBlock), 0, s, codes, n, lr, parent, c->range_hi.as<int32_t>());
+        hipLaunchKernelGGL(k_adj, dim3(g), dim3(kBlock), 0, s, codes, n, c->adj.as<int32_t>());
+        hipLaunchKernelGGL(k_escape_internal, dim3(g), dim3(kBlock), 0, s, c->range_hi.as<int32_t>(),
+                           c->adj.as<int32_t>(), n, escape);
+    }
+    hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
+    if (n >= 2)
+        hipLaunchKernelGGL(k_refit, dim3(g), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), lr, parent, escape, n,
+                           c->visited.as<uint32_t>(), c->nodes64.as<Node64>());
+    PMX_HIP(c, hipGetLastError());
+    PMX_HIP(c, hipEventRecord(c->ev[3], s));
+    // ---- outputs ----
+    int nan_host = 0;
+    PMX_HIP(c, hipMemcpyAsync(&nan_host, nan_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (codes_out) PMX_HIP(c, hipMemcpyAsync(codes_out, codes, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+    PMX_HIP(c, hipStreamSynchronize(s));
+    if (nan_host > 0) return set_error(c, PMX_ERR_ARGUMENT, "InexactError: NaN particle coordinate (UInt64(floor(NaN)))");
+    if (order_out) {
+        std::vector<int32_t> tmp((size_t)n);
+        PMX_HIP(c, hipMemcpy(tmp.data(), order, 4 * (size_t)n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) order_out[i] = tmp[(size_t)i];
+    }
+    if ((flags & PMX_BUILD_WRITEBACK) && p->mem == PMX_MEM_HOST) {
+        void* outp[24] = {p->x, p->y, p->z, p->m, p->h, p->rho};
+        for (int j = 0; j < p->ncol; ++j) outp[6 + j] = p->quant[j];
+        if (p->dtype == PMX_F64) {
+            for (int j = 0; j < ncols; ++j)
+                PMX_HIP(c, hipMemcpy(outp[j], cols64 + (size_t)j * n, 8 * (size_t)n, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<double> tmp((size_t)n);
+            for (int j = 0; j < ncols; ++j) {
+                PMX_HIP(c, hipMemcpy(tmp.data(), cols64 + (size_t)j * n, 8 * (size_t)n, hipMemcpyDeviceToHost));
+                float* dst = static_cast<float*>(outp[j]);
+                for (int64_t i = 0; i < n; ++i) dst[i] = (float)tmp[(size_t)i];
+            }
+        }
+    }
+    float ms[3];
+    PMX_HIP(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
+    PMX_HIP(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
+    PMX_HIP(c, hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]));
+    c->timings[0] = ms[0] + ms[1] + ms[2];
+    c->timings[1] = ms[0];
+    c->timings[2] = ms[2];
+    c->timings[3] = ms[1];
+    c->tree.n = n;
+    c->tree.root = n >= 2 ? 1 : 0;
+    c->tree.ncol = p->ncol;
+    c->tree.kernel = p->kernel;
+    c->tree.built = true;
+    c->tree.f32_ready = false;   // the FP32 tree is derived lazily by prepare_f32
+    return PMX_OK;
+}
+
+}  // namespace pmx

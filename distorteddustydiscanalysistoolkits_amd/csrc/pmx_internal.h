@@ -1,0 +1,137 @@
+// pmx_internal.h — host-side context and internal entry points of libpartia_mi355x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/partia_mi355x.h"
+#include "pmx_device.h"
+
+namespace pmx {
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// FP32 tree for the compensated path: node boxes rounded outward from the FP64 boxes,
+// hmax rounded up, so every FP32 node test is conservative w.r.t. the FP64 geometry.
+struct __align__(16) Node32 {
+    float mnx, mny, mnz, hmax;
+    float mxx, mxy, mxz;
+    int32_t left;
+};
+
+struct Tree {
+    int64_t n = 0;
+    int32_t root = 0;
+    int32_t ncol = 0;
+    int32_t kernel = 0;
+    bool built = false;
+    bool f32_ready = false;
+};
+
+}  // namespace pmx
+
+struct pmx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    pmx::Tree tree;
+    // kernel tables (device): Inorm per kernel, FP64 and FP32, and grid step dq
+    pmx::DevBuf lut64, lut32;
+    double lut_dq[6] = {0, 0, 0, 0, 0, 0};
+    // Morton-ordered particle data (device)
+    pmx::DevBuf codes, order, keys_tmp, vals_tmp;
+    pmx::DevBuf cols64;        // (6 + ncol) x n doubles: x y z m h rho quant...
+    pmx::DevBuf leaf64;        // Leaf64[n]
+    pmx::DevBuf leaf_esc;      // int32[n]  escape of leaf li (unified id n + li)
+    pmx::DevBuf nodes64;       // Node64[n] by internal id
+    pmx::DevBuf nodes32;       // Node32[n] by internal id
+    pmx::DevBuf node_esc;      // int32[n] escape by internal id
+    pmx::DevBuf leaf32;        // float4[n] x y z h
+    pmx::DevBuf pay32;         // float payload for the compensated path
+    pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
+    pmx::DevBuf visited, adj, range_hi;
+    pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
+    pmx::DevBuf pinned;  // unused placeholder
+    hipEvent_t ev[12] = {};
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace pmx {
+
+int set_error(pmx_ctx* c, int code, const std::string& msg);
+int hip_fail(pmx_ctx* c, hipError_t e, const char* what);
+int ensure(pmx_ctx* c, DevBuf& b, size_t bytes);
+
+#define PMX_HIP(ctx, expr)                                     \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return ::pmx::hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+#define PMX_TRY(expr)              \
+    do {                           \
+        int _rc = (expr);          \
+        if (_rc != PMX_OK) return _rc; \
+    } while (0)
+
+inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return (unsigned)g;
+}
+
+// build (pmx_build.hip)
+int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
+               int64_t* order_out);
+int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
+                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt);
+
+// interpolation (pmx_interp64.hip / pmx_interp32.hip)
+struct CatChunk {
+    int ns, ng, nd, nc;
+    int ss[4];
+    int sh[4];
+    int gs[2];
+    int ds[6];
+    int cs[6];
+    int os, og, od, oc;   // catalog positions of this chunk's first scalar/gradient/div/curl output
+};
+int interp_points_f64(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx,
+                      const double* qy, const double* qz, int64_t npts, double* out, int64_t ldo,
+                      const double* ha_in, double* ha_out);
+int interp_lines_f64(pmx_ctx* c, int strategy, const int* slots, const int* shep, int ncols,
+                     const double* o, const double* d, int64_t nl, double* out, int64_t ldo);
+int nearest_f64(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts,
+                int64_t* idx);
+int neighbors_points_f64(pmx_ctx* c, int strategy, const double* qx, const double* qy,
+                         const double* qz, const double* ha, int64_t npts, int64_t* counts,
+                         const int64_t* offsets, int32_t* idx);
+int neighbors_lines_f64(pmx_ctx* c, int strategy, const double* o, const double* d,
+                        const double* ha, int64_t nl, int64_t* counts, const int64_t* offsets,
+                        int32_t* idx);
+int kernel_eval_f64(pmx_ctx* c, int kernel, const double* q, int64_t n, double* f, double* df);
+
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx,
+                       const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
+int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o,
+                      const double* d, int64_t nl, float* out, int64_t ldo);
+int prepare_f32(pmx_ctx* c);
+
+// line-integrated kernel tables (pmx_lut.cpp)
+void line_table(int kernel, double* Q, double* Inorm);
+
+}  // namespace pmx

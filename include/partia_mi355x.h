@@ -1,0 +1,199 @@
+/*
+ * partia_mi355x.h — C ABI of libpartia_mi355x.so, the MI355X (gfx950) implementation of
+ * Partia.jl's Morton-code LBVH neighbour search and SPH kernel interpolation.
+ *
+ * The library replaces, behind Partia's backend dispatch, the methods a GPU extension adds
+ * to Partia.PointSamples_interpolation / Partia.LineSamples_interpolation (reference:
+ * Partia.jl/ext/CUDAExt/KernelInterpolation/grid_interpolation/PointSamples_interpolation.jl:12
+ * and LineSamples_interpolation.jl:11) and the CPU LBVH build they call
+ * (Partia.jl/src/julia/KernelInterpolation/grid_interpolation/initialize_interpolation.jl:49,
+ * LinearBVH! at Partia.jl/src/julia/KernelInterpolation/interpolation_setup/InterpolationInput.jl:203).
+ * Julia binds these symbols with `ccall`; the Python package binds
+ * them with ctypes.
+ *
+ * Conventions
+ *   - Every entry point returns PMX_OK (0) or a negative pmx_status and never throws or
+ *     aborts across the ABI; pmx_last_error(ctx) describes the last failure.
+ *   - Indices are 0-based.  The reference's 1-based values are index + 1.
+ *   - Pointers are host pointers unless the struct field `mem` says PMX_MEM_DEVICE.
+ *     Host buffers are borrowed for the duration of the call only.
+ *   - A context owns one HIP stream and its device buffers; it is not re-entrant (one
+ *     thread at a time per context).  Calls are synchronous on return.
+ */
+#ifndef PARTIA_MI355X_H
+#define PARTIA_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pmx_status {
+    PMX_OK = 0,
+    PMX_ERR_ARGUMENT = -1,   /* ≙ ArgumentError (bad strategy, n = 0, n > 2^30, NaN input)  */
+    PMX_ERR_DIMENSION = -2,  /* ≙ DimensionMismatch (length / catalog slot mismatch)        */
+    PMX_ERR_STATE = -3,      /* no LBVH built in this context yet                            */
+    PMX_ERR_HIP = -4,        /* HIP runtime failure (≙ ErrorException)                       */
+    PMX_ERR_NOMEM = -5,      /* device allocation failed                                     */
+    PMX_ERR_CAPACITY = -6    /* an output buffer supplied by the caller is too small         */
+} pmx_status;
+
+/* Element type of particle / sample columns (InterpolationInput{3,T}: T ∈ {Float32, Float64}). */
+enum { PMX_F32 = 0, PMX_F64 = 1 };
+/* Where the buffers of a struct live. */
+enum { PMX_MEM_HOST = 0, PMX_MEM_DEVICE = 1 };
+/* SPH kernels (Partia.jl/src/julia/KernelInterpolation/kernel_function/kernels/). */
+enum { PMX_M4 = 0, PMX_M5 = 1, PMX_M6 = 2, PMX_C2 = 3, PMX_C4 = 4, PMX_C6 = 5 };
+/* itpGather / itpScatter / itpSymmetric (interpolation_setup/InterpolationStrategy.jl:3-5). */
+enum { PMX_GATHER = 0, PMX_SCATTER = 1, PMX_SYMMETRIC = 2 };
+/* Arithmetic of the interpolation kernels.
+ *   PMX_PREC_F64  : plain FP64, one thread per sample, stackless escape traversal — the
+ *                   reference GPU design restated (ext/CUDAExt/.../PointSamples_interpolation.jl:32).
+ *   PMX_PREC_F32C : FP32 arithmetic with compensated per-sample sums, Morton-ordered
+ *                   sample tiles, wavefront-cooperative traversal and LDS-staged candidates;
+ *                   neighbour decisions are exact FP64 decisions. */
+enum { PMX_PREC_F64 = 0, PMX_PREC_F32C = 1 };
+/* pmx_build_lbvh flags */
+enum { PMX_BUILD_WRITEBACK = 1 };   /* permute the caller's host columns into Morton order */
+
+/* ≙ InterpolationInput{3,T,V,K,NCOLUMN} (InterpolationInput.jl:43-51). */
+typedef struct pmx_particles {
+    int32_t dtype;           /* PMX_F32 | PMX_F64, shared by all columns                    */
+    int32_t mem;             /* PMX_MEM_HOST | PMX_MEM_DEVICE                               */
+    int64_t n;               /* Npart                                                       */
+    void *x, *y, *z;         /* coord                                                       */
+    void *m, *h, *rho;       /* masses, smoothing lengths, densities                       */
+    int32_t ncol;            /* NCOLUMN                                                     */
+    void *const *quant;      /* ncol column pointers                                        */
+    int32_t kernel;          /* smoothed_kernel: PMX_M4 .. PMX_C6                           */
+} pmx_particles;
+
+/* ≙ InterpolationCatalogConcise (InterpolationCatalog.jl:26-32), 0-based quant slots.
+ * Output order (L = n_scalar + 3 n_grad + n_div + 3 n_curl) is the reference's:
+ * scalars, gradients (x,y,z each), divergences, curls (x,y,z each). */
+typedef struct pmx_catalog {
+    int32_t n_scalar;
+    const int32_t *scalar_slots;
+    const uint8_t *scalar_shepard;  /* 0 only for the names :ρ / :rho (InterpolationCatalog.jl:222-225) */
+    int32_t n_grad;
+    const int32_t *grad_slots;      /* -1 selects the density gradient (reference slot 0) */
+    int32_t n_div;
+    const int32_t *div_slots;       /* 3 per entry */
+    int32_t n_curl;
+    const int32_t *curl_slots;      /* 3 per entry */
+} pmx_catalog;
+
+/* Sample points (≙ PointSamples coordinates, P/src/julia/Grids/grids/PointSamples.jl:25). */
+typedef struct pmx_points {
+    int32_t mem;                     /* PMX_MEM_HOST | PMX_MEM_DEVICE; coordinates are FP64 */
+    int64_t n;
+    const double *x, *y, *z;
+} pmx_points;
+
+/* Sample lines (≙ LineSamples origin/direction, P/src/julia/Grids/grids/LineSamples.jl:31). */
+typedef struct pmx_lines {
+    int32_t mem;
+    int64_t n;
+    const double *ox, *oy, *oz;      /* origins                                    */
+    const double *dx, *dy, *dz;      /* unit directions                            */
+} pmx_lines;
+
+/* Output columns: L pointers of `dtype` elements (host or device per `mem`). */
+typedef struct pmx_output {
+    int32_t dtype;
+    int32_t mem;
+    void *const *cols;
+} pmx_output;
+
+/* --- context ---------------------------------------------------------------------- */
+/* No reference counterpart: creates the device context (stream, buffers, kernel tables). */
+int pmx_create(struct pmx_ctx **out, int32_t device_ordinal);
+void pmx_destroy(struct pmx_ctx *ctx);
+/* Text of the last error on this context (valid until the next call on it). */
+const char *pmx_last_error(const struct pmx_ctx *ctx);
+const char *pmx_version(void);
+
+/* --- build ------------------------------------------------------------------------ */
+/* ≙ LinearBVH!(input, Val(3)) (InterpolationInput.jl:203-224): FP64 bounding-box
+ * normalisation and 21-bit quantisation, 63-bit Morton codes (MortonCode.jl:443-498,
+ * 209-217, 275-305), stable sort + de-dup (sort_by_morton!, MortonCode.jl:181-195),
+ * Karras radix tree with escape links (BinaryRadixTree.jl:81-281) and atomic AABB / hmax
+ * refit (LinearBVH.jl:55-156).  The particle columns are gathered into Morton order on
+ * the device and kept in the context for the interpolation calls.
+ *   codes_out (n, nullable): de-duplicated sorted codes (enc.codes)
+ *   order_out (n, nullable): 0-based original index of each sorted slot (enc.order - 1)
+ *   flags & PMX_BUILD_WRITEBACK with host columns: the caller's x, y, z, m, h, rho and
+ *   quant arrays are overwritten in Morton order, as the reference's permute! does. */
+int pmx_build_lbvh(struct pmx_ctx *ctx, const pmx_particles *p, int32_t flags,
+                   uint64_t *codes_out, int64_t *order_out);
+
+/* --- interpolation (uses the context's current LBVH) ------------------------------ */
+/* ≙ the per-sample loop of PointSamples_interpolation (PointSamples_interpolation.jl:34-185):
+ * nearest-h (LinearBVHQueries.jl:118) for gather / symmetric, then
+ * _general_quantity_interpolate_kernel (general_interpolation.jl:1-669). */
+int pmx_interpolate_points(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                           const pmx_catalog *cat, const pmx_points *pts, const pmx_output *out);
+
+/* ≙ LineSamples_interpolation (LineSamples_interpolation.jl:47-99) with
+ * _line_integrated_quantities_interpolate_kernel (line_integrated_scalar_interpolation.jl:123-166).
+ * Only PMX_SCATTER is accepted, as in the reference (LineSamples_interpolation.jl:48-52);
+ * the catalog must hold scalars only. */
+int pmx_interpolate_lines(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                          const pmx_catalog *cat, const pmx_lines *lines, const pmx_output *out);
+
+/* --- drivers: build + interpolate in one call ------------------------------------- */
+/* ≙ Partia.PointSamples_interpolation(backend, grid, input, catalog, strategy)
+ * (ext/CUDAExt/.../PointSamples_interpolation.jl:12-29): LBVH build (with write-back of
+ * the Morton-ordered columns when they are host arrays), interpolation, synchronise. */
+int pmx_point_samples_interpolation(struct pmx_ctx *ctx, const pmx_particles *p,
+                                    int32_t strategy, int32_t precision, const pmx_catalog *cat,
+                                    const pmx_points *pts, const pmx_output *out);
+/* ≙ Partia.LineSamples_interpolation (ext/CUDAExt/.../LineSamples_interpolation.jl:11-34). */
+int pmx_line_samples_interpolation(struct pmx_ctx *ctx, const pmx_particles *p,
+                                   int32_t strategy, int32_t precision, const pmx_catalog *cat,
+                                   const pmx_lines *lines, const pmx_output *out);
+
+/* --- inspection hooks (tests; ≙ fields of BinaryRadixTree / LinearBVH) ------------- */
+/* Unified node ids as in the reference (internal 1..n-1, leaf k ↦ n-1+k, 0 = none);
+ * arrays have 2n-1 entries, entry j describing unified id j+1 (BinaryRadixTree.jl:8-15). */
+int pmx_export_tree(struct pmx_ctx *ctx, int32_t *left, int32_t *right, int32_t *parent,
+                    int32_t *escape);
+/* node AABB min/max (3 x (n-1) each, FP64) and node_hmax (n-1) (LinearBVH.jl:20-26). */
+int pmx_export_nodes(struct pmx_ctx *ctx, double *node_min, double *node_max, double *node_hmax);
+/* Morton-ordered particle columns held by the context (FP64): x,y,z,m,h,rho then quant. */
+int pmx_export_particles(struct pmx_ctx *ctx, double *const *cols, int32_t ncols);
+/* Nearest leaf (0-based sorted index) of each point (≙ LBVH_find_nearest, LinearBVHQueries.jl:79). */
+int pmx_nearest(struct pmx_ctx *ctx, int32_t precision, const pmx_points *pts, int64_t *idx_out);
+/* Accepted neighbour sets of each point for a strategy (≙ the leaves the traversal macros
+ * hand to the accumulation, LinearBVHTraversalMacros.jl:26-371), CSR in increasing sorted
+ * index.  counts (n) is always written; offsets (n+1) and idx (capacity entries) when
+ * non-null.  ha (n, nullable, host FP64): query smoothing lengths for gather / symmetric;
+ * null means nearest-h.  Returns PMX_ERR_CAPACITY if idx is too small (counts are valid). */
+int pmx_point_neighbors(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                        const pmx_points *pts, const double *ha, int64_t *counts,
+                        int64_t *offsets, int32_t *idx, int64_t capacity);
+/* Same for lines (LinearBVHTraversalMacros.jl:408-796). */
+int pmx_line_neighbors(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                       const pmx_lines *lines, const double *ha, int64_t *counts,
+                       int64_t *offsets, int32_t *idx, int64_t capacity);
+
+/* --- kernel functions (host-callable, evaluated on the device) --------------------- */
+/* f(q), f'(q) of a kernel on the device in FP64 (kernels/{M4..C6}.jl, (::K)(q) and (::_dK)(q)). */
+int pmx_kernel_eval(struct pmx_ctx *ctx, int32_t kernel, int64_t n, const double *q,
+                    double *f, double *df);
+/* The regenerated line-integrated table of a kernel: Q and Inorm, 4096 entries each
+ * (≙ _K_Q / _K_Inorm of line_integrated_kernel_tables.jl).  Host only; no context. */
+int pmx_line_kernel_table(int32_t kernel, double *Q, double *Inorm);
+
+/* --- timing ------------------------------------------------------------------------ */
+/* Device time (ms, HIP events on the context stream) of the phases of the last build /
+ * interpolation call: [0] build total, [1] encode+sort, [2] tree+refit, [3] column gather,
+ * [4] nearest-h, [5] interpolation kernel, [6] last interpolation call total. */
+int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,151 @@
+"""GPU parity of the plain FP64 path against the oracle (CPU FP64 restatement).
+
+Bit-exact: Morton codes, sort permutation, tree arrays, node boxes / hmax, Morton-ordered
+columns, nearest leaves and accepted neighbour sets.  Fields: the FP64 path evaluates the
+reference's FP64 operation order, so it is required to agree with the oracle to 1e-13
+normwise (it is in practice bit-identical except where sqrt/division rounding differ).
+"""
+import numpy as np
+import pytest
+
+from distorteddustydiscanalysistoolkits_amd import _lib, disc
+from oracle import oracle as O
+
+from .helpers import CAT_C2, csr_sets, cyl_points, disc_columns, oracle_tree, rel_norms
+
+pytestmark = pytest.mark.gpu
+
+N = 20000
+
+
+@pytest.fixture(scope="module", params=[np.float64, np.float32], ids=["f64", "f32"])
+def built(request, ctx):
+    dt = request.param
+    cols, quant = disc_columns(N, seed=3, warped=True, dtype=dt)
+    print(f"[parity] build n={N} dtype={dt.__name__}", flush=True)
+    codes, order = ctx.build(*cols, quant=quant, kernel=0, want_codes=True, want_order=True)
+    tree, ocols, oquant = oracle_tree(cols, quant)
+    return dict(ctx=ctx, codes=codes, order=order, tree=tree, ocols=ocols, oquant=oquant, dtype=dt)
+
+
+def test_codes_and_order_bitexact(built):
+    t = built["tree"]
+    assert np.array_equal(built["codes"], t.codes)
+    assert np.array_equal(built["order"], t.order - 1)
+
+
+def test_tree_arrays_bitexact(built):
+    left, right, parent, escape = built["ctx"].export_tree()
+    t = built["tree"]
+    assert np.array_equal(left, t.left)
+    assert np.array_equal(right, t.right)
+    assert np.array_equal(parent, t.parent)
+    assert np.array_equal(escape, t.escape)
+
+
+def test_nodes_exact(built):
+    mn, mx, hm = built["ctx"].export_nodes()
+    t = built["tree"]
+    for k in range(3):
+        assert np.array_equal(mn[k], t.node_min[k])
+        assert np.array_equal(mx[k], t.node_max[k])
+    assert np.array_equal(hm, t.node_hmax)
+
+
+def test_columns_in_morton_order(built):
+    got = built["ctx"].export_particles()
+    ref = built["ocols"] + built["oquant"]
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+
+
+def test_nearest_bitexact(built):
+    qx, qy, qz = cyl_points(24, 24, 12)
+    idx = built["ctx"].nearest(qx, qy, qz)
+    ref, _ = built["tree"].nearest(qx, qy, qz)
+    assert np.array_equal(idx, ref)
+
+
+@pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
+def test_point_neighbor_sets_bitexact(built, strategy):
+    qx, qy, qz = cyl_points(16, 16, 8)
+    s = O.STRATEGIES[strategy]
+    off, idx = built["ctx"].point_neighbors(s, qx, qy, qz)
+    roff, ridx = built["tree"].point_neighbors("M4", strategy, qx, qy, qz)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(idx, ridx)
+    assert off[-1] > 0
+
+
+def test_line_neighbor_sets_bitexact(built):
+    o, d = disc.faceon_lines(24)
+    off, idx = built["ctx"].line_neighbors(_lib.PMX_SCATTER, o, d)
+    roff, ridx = built["tree"].line_neighbors("M4", "scatter", o, d)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(idx, ridx)
+    assert off[-1] > 0
+
+
+@pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
+def test_points_f64_fields(built, strategy):
+    qx, qy, qz = cyl_points(20, 20, 10)
+    cat = _lib.catalog_struct(**CAT_C2)
+    outs = [np.empty(qx.size) for _ in range(8)]
+    built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F64, cat, qx, qy, qz, outs)
+    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    for j, (g, r) in enumerate(zip(outs, ref)):
+        assert np.array_equal(np.isnan(g), np.isnan(r)), f"field {j}: NaN pattern differs"
+        linf, l2 = rel_norms(g, r)
+        assert linf <= 1e-13, f"field {j}: rel Linf {linf}"
+
+
+def test_lines_f64_fields(built):
+    o, d = disc.faceon_lines(32)
+    cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
+    outs = [np.empty(o[0].size) for _ in range(2)]
+    built["ctx"].interpolate_lines(_lib.PMX_SCATTER, _lib.PMX_PREC_F64, cat, o, d, outs)
+    ref = built["tree"].line_samples("M4", "scatter", [0, 3], [0, 1], o, d)
+    for g, r in zip(outs, ref):
+        assert np.array_equal(np.isnan(g), np.isnan(r))
+        linf, _ = rel_norms(g, r)
+        assert linf <= 1e-13
+
+
+@pytest.mark.parametrize("kernel", range(6))
+def test_kernel_eval_matches_oracle(ctx, kernel):
+    q = np.linspace(0.0, 3.2, 2001)
+    f, df = ctx.kernel_eval(kernel, q)
+    rf = np.array([O.kernel_f(kernel, x) for x in q])
+    rd = np.array([O.kernel_df(kernel, x) for x in q])
+    assert np.array_equal(f, rf)
+    assert np.array_equal(df, rd)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 300])
+def test_small_and_duplicate_sets(ctx, n):
+    rng = np.random.default_rng(n)
+    x = rng.random(n)
+    y = rng.random(n)
+    z = np.zeros(n)                       # flat axis -> f = 0.5
+    if n > 2:
+        x[: n // 2] = 0.25                # duplicated positions -> de-dup path
+        y[: n // 2] = 0.75
+    m = np.full(n, 1.0 / max(n, 1))
+    h = 0.2 + 0.1 * rng.random(n)
+    rho = 1.0 + rng.random(n)
+    q = [rng.random(n)]
+    codes, order = ctx.build(x, y, z, m, h, rho, quant=q, want_codes=True, want_order=True)
+    tree, _, _ = oracle_tree([x, y, z, m, h, rho], q)
+    assert np.array_equal(codes, tree.codes)
+    assert np.array_equal(order, tree.order - 1)
+    left, right, parent, escape = ctx.export_tree()
+    assert np.array_equal(escape, tree.escape) and np.array_equal(parent, tree.parent)
+    g = np.linspace(0.0, 1.0, 7)
+    qx, qy, qz = [a.ravel() for a in np.meshgrid(g, g, np.array([0.0, 0.1]), indexing="ij")]
+    cat = _lib.catalog_struct(scalar_slots=[0], grad_slots=[-1])
+    outs = [np.empty(qx.size) for _ in range(4)]
+    ctx.interpolate_points(_lib.PMX_SYMMETRIC, _lib.PMX_PREC_F64, cat, qx, qy, qz, outs)
+    ref = tree.point_samples("M4", "symmetric", dict(scalar_slots=[0], grad_slots=[-1]), qx, qy, qz)
+    for gv, r in zip(outs, ref):
+        assert np.array_equal(np.isnan(gv), np.isnan(r))
+        assert rel_norms(gv, r)[0] <= 1e-13
