@@ -25,7 +25,8 @@ EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_bu
            "pmx_interpolate_points", "pmx_interpolate_lines", "pmx_point_samples_interpolation",
            "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
            "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
-           "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings")
+           "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings", "pmx_point_traversal_stats",
+           "pmx_line_traversal_stats")
 
 
 class PmxError(RuntimeError):
@@ -112,6 +113,8 @@ def lib():
         L.pmx_kernel_eval.argtypes = [vp, i32, i64, vp, vp, vp]
         L.pmx_line_kernel_table.argtypes = [i32, vp, vp]
         L.pmx_last_timings.argtypes = [vp, vp, i32]
+        L.pmx_point_traversal_stats.argtypes = [vp, i32, C.POINTER(Points), vp]
+        L.pmx_line_traversal_stats.argtypes = [vp, C.POINTER(Lines), vp]
         _lib = L
     return _lib
 
@@ -293,6 +296,18 @@ class Context:
         self._check(lib().pmx_line_neighbors(self._h, int(strategy), int(precision), C.byref(ln), _ptr(hap),
                                              counts.ctypes.data, off.ctypes.data, idx.ctypes.data, total))
         return off, idx[:total]
+
+    def point_stats(self, strategy, qx, qy, qz):
+        pts = self._points(qx, qy, qz)
+        st = np.zeros(5, np.int64)
+        self._check(lib().pmx_point_traversal_stats(self._h, int(strategy), C.byref(pts), st.ctypes.data))
+        return st
+
+    def line_stats(self, origin, direction):
+        ln = self._lines(origin, direction)
+        st = np.zeros(5, np.int64)
+        self._check(lib().pmx_line_traversal_stats(self._h, C.byref(ln), st.ctypes.data))
+        return st
 
     def kernel_eval(self, kernel, q):
         q = np.ascontiguousarray(q, np.float64)

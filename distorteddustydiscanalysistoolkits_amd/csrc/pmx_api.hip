@@ -428,16 +428,17 @@ int pmx_export_particles(pmx_ctx* c, double* const* cols, int32_t ncols) {
 int pmx_nearest(pmx_ctx* c, int32_t precision, const pmx_points* pts, int64_t* idx_out) {
     PMX_TRY(require_tree(c));
     PMX_HIP(c, hipSetDevice(c->device));
-    (void)precision;
+    if (precision != PMX_PREC_F64 && precision != PMX_PREC_F32C) return set_error(c, PMX_ERR_ARGUMENT, "unknown precision");
     PMX_TRY(check_points(c, pts));
     if (!idx_out) return set_error(c, PMX_ERR_ARGUMENT, "nearest: null output");
     const int64_t n = pts->n;
     if (n == 0) return PMX_OK;
     const double *qx, *qy, *qz;
     PMX_TRY(stage_points(c, pts, &qx, &qy, &qz));
-    PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)n));
-    PMX_TRY(nearest_f64(c, qx, qy, qz, n, c->scratch_b.as<int64_t>()));
-    PMX_HIP(c, hipMemcpyAsync(idx_out, c->scratch_b.p, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    PMX_TRY(ensure(c, c->scratch_c, 8 * (size_t)n));
+    if (precision == PMX_PREC_F32C) PMX_TRY(nearest_f32c(c, qx, qy, qz, n, c->scratch_c.as<int64_t>()));
+    else PMX_TRY(nearest_f64(c, qx, qy, qz, n, c->scratch_c.as<int64_t>()));
+    PMX_HIP(c, hipMemcpyAsync(idx_out, c->scratch_c.p, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     PMX_HIP(c, hipStreamSynchronize(c->stream));
     return PMX_OK;
 }
@@ -479,7 +480,7 @@ int pmx_point_neighbors(pmx_ctx* c, int32_t strategy, int32_t precision, const p
                         int64_t* counts, int64_t* offsets, int32_t* idx, int64_t capacity) {
     PMX_TRY(require_tree(c));
     PMX_HIP(c, hipSetDevice(c->device));
-    (void)precision;
+    if (precision != PMX_PREC_F64 && precision != PMX_PREC_F32C) return set_error(c, PMX_ERR_ARGUMENT, "unknown precision");
     if (strategy < 0 || strategy > 2) return set_error(c, PMX_ERR_ARGUMENT, "unknown interpolation strategy");
     PMX_TRY(check_points(c, pts));
     const int64_t n = pts->n;
@@ -487,6 +488,8 @@ int pmx_point_neighbors(pmx_ctx* c, int32_t strategy, int32_t precision, const p
     if (n > 0) PMX_TRY(stage_points(c, pts, &qx, &qy, &qz));
     return neighbors_common(c, n, ha, counts, offsets, idx, capacity,
                             [&](const double* dha, int64_t* cnt, const int64_t* off, int32_t* ix) {
+                                if (precision == PMX_PREC_F32C)
+                                    return neighbors_points_f32c(c, strategy, qx, qy, qz, dha, n, cnt, off, ix);
                                 return neighbors_points_f64(c, strategy, qx, qy, qz, dha, n, cnt, off, ix);
                             });
 }
@@ -495,7 +498,7 @@ int pmx_line_neighbors(pmx_ctx* c, int32_t strategy, int32_t precision, const pm
                        int64_t* counts, int64_t* offsets, int32_t* idx, int64_t capacity) {
     PMX_TRY(require_tree(c));
     PMX_HIP(c, hipSetDevice(c->device));
-    (void)precision;
+    if (precision != PMX_PREC_F64 && precision != PMX_PREC_F32C) return set_error(c, PMX_ERR_ARGUMENT, "unknown precision");
     if (strategy < 0 || strategy > 2) return set_error(c, PMX_ERR_ARGUMENT, "unknown interpolation strategy");
     PMX_TRY(check_lines(c, lines));
     const int64_t n = lines->n;
@@ -503,8 +506,51 @@ int pmx_line_neighbors(pmx_ctx* c, int32_t strategy, int32_t precision, const pm
     if (n > 0) PMX_TRY(stage_lines(c, lines, &o, &d));
     return neighbors_common(c, n, ha, counts, offsets, idx, capacity,
                             [&](const double* dha, int64_t* cnt, const int64_t* off, int32_t* ix) {
+                                if (precision == PMX_PREC_F32C) {
+                                    if (strategy != kScatter)
+                                        return set_error(c, PMX_ERR_ARGUMENT,
+                                                         "compensated line neighbours support itpScatter only");
+                                    return neighbors_lines_f32c(c, o, d, n, cnt, off, ix);
+                                }
                                 return neighbors_lines_f64(c, strategy, o, d, dha, n, cnt, off, ix);
                             });
+}
+
+int pmx_point_traversal_stats(pmx_ctx* c, int32_t strategy, const pmx_points* pts, int64_t* stats) {
+    PMX_TRY(require_tree(c));
+    PMX_HIP(c, hipSetDevice(c->device));
+    if (strategy < 0 || strategy > 2) return set_error(c, PMX_ERR_ARGUMENT, "unknown interpolation strategy");
+    PMX_TRY(check_points(c, pts));
+    if (!stats) return set_error(c, PMX_ERR_ARGUMENT, "stats: null");
+    for (int k = 0; k < 5; ++k) stats[k] = 0;
+    if (pts->n == 0) return PMX_OK;
+    const double *qx, *qy, *qz;
+    PMX_TRY(stage_points(c, pts, &qx, &qy, &qz));
+    PMX_TRY(ensure(c, c->scratch_b, 64));
+    unsigned long long* tot = c->scratch_b.as<unsigned long long>();
+    PMX_HIP(c, hipMemsetAsync(tot, 0, 40, c->stream));
+    PMX_TRY(traversal_stats_f64(c, 0, strategy, qx, qy, qz, pts->n, tot));
+    PMX_HIP(c, hipMemcpyAsync(stats, tot, 40, hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP(c, hipStreamSynchronize(c->stream));
+    return PMX_OK;
+}
+
+int pmx_line_traversal_stats(pmx_ctx* c, const pmx_lines* lines, int64_t* stats) {
+    PMX_TRY(require_tree(c));
+    PMX_HIP(c, hipSetDevice(c->device));
+    PMX_TRY(check_lines(c, lines));
+    if (!stats) return set_error(c, PMX_ERR_ARGUMENT, "stats: null");
+    for (int k = 0; k < 5; ++k) stats[k] = 0;
+    if (lines->n == 0) return PMX_OK;
+    const double *o, *d;
+    PMX_TRY(stage_lines(c, lines, &o, &d));
+    PMX_TRY(ensure(c, c->scratch_b, 64));
+    unsigned long long* tot = c->scratch_b.as<unsigned long long>();
+    PMX_HIP(c, hipMemsetAsync(tot, 0, 40, c->stream));
+    PMX_TRY(traversal_stats_f64(c, 1, kScatter, o, d, nullptr, lines->n, tot));
+    PMX_HIP(c, hipMemcpyAsync(stats, tot, 40, hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP(c, hipStreamSynchronize(c->stream));
+    return PMX_OK;
 }
 
 int pmx_kernel_eval(pmx_ctx* c, int32_t kernel, int64_t n, const double* q, double* f, double* df) {

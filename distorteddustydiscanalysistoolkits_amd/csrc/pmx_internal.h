@@ -124,12 +124,19 @@ int neighbors_lines_f64(pmx_ctx* c, int strategy, const double* o, const double*
                         const double* ha, int64_t nl, int64_t* counts, const int64_t* offsets,
                         int32_t* idx);
 int kernel_eval_f64(pmx_ctx* c, int kernel, const double* q, int64_t n, double* f, double* df);
+int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, const double* b, const double* cc,
+                        int64_t n, unsigned long long* tot);
 
 int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx,
                        const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
 int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o,
                       const double* d, int64_t nl, float* out, int64_t ldo);
 int prepare_f32(pmx_ctx* c);
+int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx);
+int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,
+                          const double* ha, int64_t npts, int64_t* counts, const int64_t* offsets, int32_t* idx);
+int neighbors_lines_f32c(pmx_ctx* c, const double* o, const double* d, int64_t nl, int64_t* counts,
+                         const int64_t* offsets, int32_t* idx);
 
 // line-integrated kernel tables (pmx_lut.cpp)
 void line_table(int kernel, double* Q, double* Inorm);

@@ -28,10 +28,16 @@ struct PartView {
     int64_t n;
 };
 
+// Traversal statistics for the traffic model (instrumented kernels only).
+struct Stats {
+    int64_t internal, leaves;
+};
+
 // Stackless DFS over left / escape links (LinearBVHTraversalMacros.jl:26-371); leaves are
 // reached in increasing sorted index, the reference's accumulation order.
 template <int S, typename Visit>
-__device__ inline void traverse_point(const TreeView& T, const double p[3], double Kv, double radius2, Visit& visit) {
+__device__ inline void traverse_point(const TreeView& T, const double p[3], double Kv, double radius2, Visit& visit,
+                                      Stats* st = nullptr) {
     const int64_t n = T.n;
     if (T.root == 0) {
         for (int64_t li = 0; li < n; ++li) {
@@ -51,10 +57,12 @@ __device__ inline void traverse_point(const TreeView& T, const double p[3], doub
             const Leaf64 L = T.leaf[li];
             double r2 = radius2_rule<S>(Kv, radius2, L.h);
             double d2 = d2_point_leaf(p, L);
+            if (st) ++st->leaves;
             if (d2 <= r2) visit(li, d2, L.h);
             node = T.leaf_esc[li];
             continue;
         }
+        if (st) ++st->internal;
         const Node64 nd = T.nodes[node];
         double r2 = radius2_rule<S>(Kv, radius2, nd.hmax);
         double d2n = d2_point_aabb(p, nd);
@@ -64,7 +72,7 @@ __device__ inline void traverse_point(const TreeView& T, const double p[3], doub
 
 template <int S, typename Visit>
 __device__ inline void traverse_line(const TreeView& T, const double o[3], const double dv[3], double Kv,
-                                     double radius2, Visit& visit) {
+                                     double radius2, Visit& visit, Stats* st = nullptr) {
     const int64_t n = T.n;
     if (T.root == 0) {
         for (int64_t li = 0; li < n; ++li) {
@@ -84,10 +92,12 @@ __device__ inline void traverse_line(const TreeView& T, const double o[3], const
             const Leaf64 L = T.leaf[li];
             double r2 = radius2_rule<S>(Kv, radius2, L.h);
             double d2 = d2_line_leaf(o, dv, L);
+            if (st) ++st->leaves;
             if (d2 <= r2) visit(li, d2, L.h);
             node = T.leaf_esc[li];
             continue;
         }
+        if (st) ++st->internal;
         const Node64 nd = T.nodes[node];
         double r2 = radius2_rule<S>(Kv, radius2, nd.hmax);
         double d2n = d2_line_aabb_lb(o, dv, nd);
@@ -97,7 +107,7 @@ __device__ inline void traverse_line(const TreeView& T, const double o[3], const
 
 // LBVH_find_nearest (LinearBVHQueries.jl:79-95 / 118-141): the pruning radius is the
 // running best d2; strict `<` keeps the smallest sorted index among equal distances.
-__device__ inline int64_t nearest_leaf(const TreeView& T, const double p[3]) {
+__device__ inline int64_t nearest_leaf(const TreeView& T, const double p[3], Stats* st = nullptr) {
     const int64_t n = T.n;
     int64_t best = -1;
     double bd = __builtin_inf();
@@ -115,10 +125,12 @@ __device__ inline int64_t nearest_leaf(const TreeView& T, const double p[3]) {
         if (node >= n) {
             const int64_t li = node - n;
             double d2 = d2_point_leaf(p, T.leaf[li]);
+            if (st) ++st->leaves;
             if (d2 < bd) { bd = d2; best = li; }
             node = T.leaf_esc[li];
             continue;
         }
+        if (st) ++st->internal;
         const Node64 nd = T.nodes[node];
         double d2n = d2_point_aabb(p, nd);
         node = (d2n > bd) ? nd.escape : nd.left;
@@ -397,6 +409,63 @@ __global__ void __launch_bounds__(kBlock) k_kernel_eval(const double* __restrict
     }
 }
 
+// Visit counts of the FP64 traversal: [0] nearest-h internal, [1] nearest-h leaves,
+// [2] internal nodes tested, [3] leaves tested, [4] leaves accepted (sums over samples).
+__device__ inline void block_add_stats(const int64_t v[5], unsigned long long* tot) {
+    __shared__ int64_t s[5][kBlock];
+    for (int k = 0; k < 5; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) atomicAdd(&tot[threadIdx.x], (unsigned long long)s[threadIdx.x][0]);
+}
+
+template <int S>
+__global__ void __launch_bounds__(kBlock) k_point_stats(TreeView T, double Kv, const double* __restrict__ qx,
+                                                        const double* __restrict__ qy, const double* __restrict__ qz,
+                                                        int64_t npts, unsigned long long* __restrict__ tot) {
+    int64_t v[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npts; i += (int64_t)gridDim.x * kBlock) {
+        const double p[3] = {qx[i], qy[i], qz[i]};
+        double radius2 = 0.0;
+        if (S != kScatter) {
+            Stats sn{0, 0};
+            const int64_t b = nearest_leaf(T, p, &sn);
+            v[0] += sn.internal;
+            v[1] += sn.leaves;
+            const double r = Kv * (b < 0 ? __builtin_nan("") : T.leaf[b].h);
+            radius2 = r * r;
+        }
+        Stats st{0, 0};
+        CountVisit cv{0, nullptr};
+        traverse_point<S>(T, p, Kv, radius2, cv, &st);
+        v[2] += st.internal;
+        v[3] += st.leaves;
+        v[4] += cv.cnt;
+    }
+    block_add_stats(v, tot);
+}
+
+__global__ void __launch_bounds__(kBlock) k_line_stats(TreeView T, double Kv, const double* __restrict__ o,
+                                                       const double* __restrict__ dvec, int64_t nl,
+                                                       unsigned long long* __restrict__ tot) {
+    int64_t v[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nl; i += (int64_t)gridDim.x * kBlock) {
+        const double oo[3] = {o[i], o[nl + i], o[2 * nl + i]};
+        const double dd[3] = {dvec[i], dvec[nl + i], dvec[2 * nl + i]};
+        Stats st{0, 0};
+        CountVisit cv{0, nullptr};
+        traverse_line<kScatter>(T, oo, dd, Kv, 0.0, cv, &st);
+        v[2] += st.internal;
+        v[3] += st.leaves;
+        v[4] += cv.cnt;
+    }
+    block_add_stats(v, tot);
+}
+
 TreeView tree_view(pmx_ctx* c) {
     return TreeView{c->nodes64.as<Node64>(), c->leaf64.as<Leaf64>(), c->leaf_esc.as<int32_t>(), c->tree.n,
                     c->tree.root};
@@ -500,6 +569,24 @@ int neighbors_lines_f64(pmx_ctx* c, int strategy, const double* o, const double*
         hipLaunchKernelGGL(k_line_nb<kScatter>, g, b, 0, c->stream, T, Kv, o, d, ha, nl, counts, offsets, idx);
     else
         hipLaunchKernelGGL(k_line_nb<kSymmetric>, g, b, 0, c->stream, T, Kv, o, d, ha, nl, counts, offsets, idx);
+    PMX_HIP(c, hipGetLastError());
+    return PMX_OK;
+}
+
+int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, const double* b, const double* cc,
+                        int64_t n, unsigned long long* tot) {
+    const dim3 g(grid_for(n, kBlock, 1 << 16)), bl(kBlock);
+    const double Kv = support(c->tree.kernel);
+    TreeView T = tree_view(c);
+    if (kind == 1) {
+        hipLaunchKernelGGL(k_line_stats, g, bl, 0, c->stream, T, Kv, a, b, n, tot);
+    } else if (strategy == kGather) {
+        hipLaunchKernelGGL(k_point_stats<kGather>, g, bl, 0, c->stream, T, Kv, a, b, cc, n, tot);
+    } else if (strategy == kScatter) {
+        hipLaunchKernelGGL(k_point_stats<kScatter>, g, bl, 0, c->stream, T, Kv, a, b, cc, n, tot);
+    } else {
+        hipLaunchKernelGGL(k_point_stats<kSymmetric>, g, bl, 0, c->stream, T, Kv, a, b, cc, n, tot);
+    }
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }

@@ -15,6 +15,8 @@ Dust: H_d = 0.3 H, Sigma_d = 0.01 Sigma_g, v_phi lowered by 0.5 %.
 """
 from __future__ import annotations
 
+from fractions import Fraction
+
 import numpy as np
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -103,12 +105,22 @@ def make_dust(n: int, seed: int = 0, warped: bool = False, gas_mass: float = 1e-
 # Sample geometries (host-side flatten, as Partia's StructuredGrid flatten runs on the
 # host: P/src/julia/Grids/grids/transform.jl:53-72, StructuredGrid.jl:126-204).
 # ------------------------------------------------------------------------------------
+def _fma(x: float, y: float, z: float) -> float:
+    """Correctly rounded x*y + z (exact rational arithmetic)."""
+    return float(Fraction(x) * Fraction(y) + Fraction(z))
+
+
 def linrange(a: float, b: float, n: int) -> np.ndarray:
-    """LinRange{Float64}(a, b, n): element j = (1-t) a + t b with t = (j-1)/(n-1)."""
+    """LinRange{Float64}(a, b, n) element values: t = j/(n-1), fma(t, b, fma(-t, a, a))
+    (Julia Base.lerpi)."""
     if n == 1:
         return np.array([a], np.float64)
-    t = np.arange(n, dtype=np.float64) / (n - 1)
-    return (1.0 - t) * a + t * b
+    a, b = float(a), float(b)
+    out = np.empty(n, np.float64)
+    for j in range(n):
+        t = j / (n - 1)
+        out[j] = _fma(t, b, _fma(-t, a, a))
+    return out
 
 
 def cylindrical_axes(s=(1.0, 100.0, 128), phi=(0.0, 2 * np.pi, 128), z=(-5.0, 5.0, 32)):

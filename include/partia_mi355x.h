@@ -179,6 +179,14 @@ int pmx_line_neighbors(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
                        const pmx_lines *lines, const double *ha, int64_t *counts,
                        int64_t *offsets, int32_t *idx, int64_t capacity);
 
+/* Visit statistics of the FP64 traversal (traffic model of the roofline report):
+ * stats[5] = {nearest-h internal nodes, nearest-h leaves, internal nodes tested, leaves
+ * tested, leaves accepted}, summed over all samples.  No reference counterpart (the
+ * reference has no instrumentation); used by the benchmark only. */
+int pmx_point_traversal_stats(struct pmx_ctx *ctx, int32_t strategy, const pmx_points *pts,
+                              int64_t *stats);
+int pmx_line_traversal_stats(struct pmx_ctx *ctx, const pmx_lines *lines, int64_t *stats);
+
 /* --- kernel functions (host-callable, evaluated on the device) --------------------- */
 /* f(q), f'(q) of a kernel on the device in FP64 (kernels/{M4..C6}.jl, (::K)(q) and (::_dK)(q)). */
 int pmx_kernel_eval(struct pmx_ctx *ctx, int32_t kernel, int64_t n, const double *q,

@@ -1,0 +1,55 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the public
+header declares, rejects calls without a device cleanly, and its host-side line-kernel
+table agrees with the oracle's and with the reference's own table."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from distorteddustydiscanalysistoolkits_amd import _lib
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "partia_mi355x.h")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "line_lut_sample.npz")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pmx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 18
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) <= set(syms)
+
+
+def test_version_and_error_without_context():
+    lib = _lib.lib()
+    assert b"gfx950" in lib.pmx_version()
+    assert lib.pmx_last_error(None) == b"null context"
+    assert lib.pmx_build_lbvh(None, None, 0, None, None) == _lib.PMX_ERR_ARGUMENT
+
+
+@pytest.mark.parametrize("kernel", range(6))
+def test_line_table_matches_oracle_and_reference(kernel):
+    Q, In = _lib.line_kernel_table(kernel)
+    oQ, oIn = O.line_table(kernel)
+    assert np.array_equal(Q, oQ)
+    assert np.max(np.abs(In - oIn)) <= 4e-16 * np.abs(oIn).max()
+    f = np.load(GOLDEN)
+    nm = ["M4_spline", "M5_spline", "M6_spline", "C2_Wendland", "C4_Wendland", "C6_Wendland"][kernel]
+    idx = f["index"]
+    assert np.array_equal(Q[idx], f[nm + "_Q"])
+    assert np.max(np.abs(In[idx] - f[nm + "_Inorm"])) <= 1e-10 * f[nm + "_Inorm"].max()
+
+
+def test_line_table_rejects_bad_kernel():
+    assert _lib.lib().pmx_line_kernel_table(7, None, None) == _lib.PMX_ERR_ARGUMENT

@@ -149,3 +149,63 @@ def test_small_and_duplicate_sets(ctx, n):
     for gv, r in zip(outs, ref):
         assert np.array_equal(np.isnan(gv), np.isnan(r))
         assert rel_norms(gv, r)[0] <= 1e-13
+
+
+# ---------------------------------------------------------------------------------------
+# FP32 compensated path: neighbour decisions are exact FP64 decisions; fields within the
+# stated normwise tolerance of the FP64 oracle.
+# ---------------------------------------------------------------------------------------
+F32C_TOL = 1e-6     # normwise relative L-inf and L2 per field vs the FP64 oracle
+
+
+def test_f32c_nearest_bitexact(built):
+    qx, qy, qz = cyl_points(24, 24, 12)
+    idx = built["ctx"].nearest(qx, qy, qz, precision=_lib.PMX_PREC_F32C)
+    ref, _ = built["tree"].nearest(qx, qy, qz)
+    assert np.array_equal(idx, ref)
+
+
+@pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
+def test_f32c_point_neighbor_sets_bitexact(built, strategy):
+    qx, qy, qz = cyl_points(16, 16, 8)
+    off, idx = built["ctx"].point_neighbors(O.STRATEGIES[strategy], qx, qy, qz, precision=_lib.PMX_PREC_F32C)
+    roff, ridx = built["tree"].point_neighbors("M4", strategy, qx, qy, qz)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(idx, ridx)
+
+
+def test_f32c_line_neighbor_sets_bitexact(built):
+    o, d = disc.faceon_lines(24)
+    off, idx = built["ctx"].line_neighbors(_lib.PMX_SCATTER, o, d, precision=_lib.PMX_PREC_F32C)
+    roff, ridx = built["tree"].line_neighbors("M4", "scatter", o, d)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(idx, ridx)
+
+
+@pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
+def test_f32c_points_fields(built, strategy):
+    qx, qy, qz = cyl_points(20, 20, 10)
+    cat = _lib.catalog_struct(**CAT_C2)
+    outs = [np.empty(qx.size) for _ in range(8)]
+    built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, cat, qx, qy, qz, outs)
+    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    worst = []
+    for j, (g, r) in enumerate(zip(outs, ref)):
+        assert np.array_equal(np.isnan(g), np.isnan(r)), f"field {j}: NaN pattern differs"
+        worst.append(rel_norms(g, r))
+    print(f"[f32c points {strategy}] rel (Linf, L2) per field: {worst}")
+    for j, (linf, l2) in enumerate(worst):
+        assert linf <= F32C_TOL and l2 <= F32C_TOL, f"field {j}: {linf} {l2}"
+
+
+def test_f32c_lines_fields(built):
+    o, d = disc.faceon_lines(32)
+    cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
+    outs = [np.empty(o[0].size) for _ in range(2)]
+    built["ctx"].interpolate_lines(_lib.PMX_SCATTER, _lib.PMX_PREC_F32C, cat, o, d, outs)
+    ref = built["tree"].line_samples("M4", "scatter", [0, 3], [0, 1], o, d)
+    for g, r in zip(outs, ref):
+        assert np.array_equal(np.isnan(g), np.isnan(r))
+        linf, l2 = rel_norms(g, r)
+        print(f"[f32c lines] rel Linf {linf:.3e} L2 {l2:.3e}")
+        assert linf <= F32C_TOL and l2 <= F32C_TOL
