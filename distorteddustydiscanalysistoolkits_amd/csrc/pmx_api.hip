@@ -191,14 +191,12 @@ int deliver(pmx_ctx* c, const void* buf, bool buf_is_f32, int L, int64_t n, cons
     return PMX_OK;
 }
 
-void upload_tables(pmx_ctx* c, std::vector<double>& h64, std::vector<float>& h32) {
+void upload_tables(pmx_ctx* c, std::vector<double>& h64) {
     h64.resize(6 * kLutN);
-    h32.resize(6 * kLutN);
     std::vector<double> Q(kLutN);
     for (int k = 0; k < 6; ++k) {
         line_table(k, Q.data(), h64.data() + k * kLutN);
         c->lut_dq[k] = Q[1] - Q[0];
-        for (int i = 0; i < kLutN; ++i) h32[k * kLutN + i] = (float)h64[k * kLutN + i];
     }
 }
 
@@ -236,11 +234,9 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { delete c; return PMX_ERR_HIP; }
     std::vector<double> h64;
-    std::vector<float> h32;
-    upload_tables(c, h64, h32);
-    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK || ensure(c, c->lut32, 4 * h32.size()) != PMX_OK ||
-        hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->lut32.p, h32.data(), 4 * h32.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    upload_tables(c, h64);
+    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK ||
+        hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess) {
         pmx_destroy(c);
         return PMX_ERR_HIP;
     }
@@ -252,9 +248,9 @@ void pmx_destroy(pmx_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->lut64, &c->lut32, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
-                      &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->node_esc, &c->leaf32, &c->pay32,
-                      &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->scratch_a,
+    DevBuf* bufs[] = {&c->lut64, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
+                      &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
+                      &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
                       &c->pinned};
     for (DevBuf* b : bufs)

@@ -429,7 +429,7 @@ __device__ inline int delta(const uint64_t* __restrict__ c, int64_t i, int64_t j
 // _split_position (:181-202).  lr holds left at 2*id and right at 2*id+1.
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
-                                                int32_t* __restrict__ range_hi) {
+                                                int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n - 1;
          i += (int64_t)gridDim.x * kBlock) {
         int dl = (i > 1) ? delta(c, i, i - 1) : -1;
@@ -451,6 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         int64_t j = i + l * d;
         int64_t lo = i < j ? i : j, hi = i < j ? j : i;
         range_hi[i] = (int32_t)hi;
+        range_lo[i] = (int32_t)lo;
         int64_t split;
         if (c[lo - 1] == c[hi - 1]) {
             split = (lo + hi) >> 1;
@@ -520,54 +521,106 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, i
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = v;
 }
 
-// Atomic bottom-up refit (LinearBVH.jl:78-80, _build_internal_aabb! :98-156): one climber
-// per leaf; the second arrival at an internal node combines both children with strict
-// comparisons (ifelse(<) / ifelse(>)), writes the node and climbs on.  A climber that has
-// written a node releases it (agent-scope fence) before its next arrival; the second
-// arrival acquires before reading its children.
-__global__ void __launch_bounds__(kBlock) k_refit(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
-                                                  const int32_t* __restrict__ parent,
-                                                  const int32_t* __restrict__ escape, int64_t n,
-                                                  uint32_t* __restrict__ visited, Node64* nodes) {
-    for (int64_t li = (int64_t)blockIdx.x * kBlock + threadIdx.x; li < n; li += (int64_t)gridDim.x * kBlock) {
-        int32_t p = parent[n + li];
-        bool wrote = false;
-        int64_t steps = 0;
-        while (p > 0 && p < n && ++steps <= n) {   // a parent chain is at most n-1 long
-            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            uint32_t old = __hip_atomic_fetch_add(&visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old == 0u) break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const int32_t l = lr[2 * p], r = lr[2 * p + 1];
-            double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
-            if (l >= n) {
-                const Leaf64 L = leaf[l - n];
-                lmn[0] = lmx[0] = L.x; lmn[1] = lmx[1] = L.y; lmn[2] = lmx[2] = L.z; hl = L.h;
-            } else {
-                const Node64* N = &nodes[l];
-                for (int k = 0; k < 3; ++k) { lmn[k] = N->mn[k]; lmx[k] = N->mx[k]; }
-                hl = N->hmax;
+// Bottom-up refit (LinearBVH.jl:78-80, _build_internal_aabb! :98-156), in rounds.
+// A climber sits on a completed node; it moves to the parent and the second arrival at
+// a parent combines both children with strict comparisons (ifelse(<) / ifelse(>)),
+// writes the parent and continues in the next round.  Round r only reads nodes completed
+// in rounds < r (or leaves), and consecutive rounds are separate kernel launches on one
+// stream, so node data needs no fences; the arrival counter is the only shared atomic.
+// Round 0 starts from every leaf; later rounds from the list the previous round appended.
+__device__ inline void combine_node(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                    const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p) {
+    const int32_t l = lr[2 * p], r = lr[2 * p + 1];
+    double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
+    if (l >= n) {
+        const Leaf64 L = leaf[l - n];
+        lmn[0] = lmx[0] = L.x; lmn[1] = lmx[1] = L.y; lmn[2] = lmx[2] = L.z; hl = L.h;
+    } else {
+        const Node64 N = nodes[l];
+        for (int k = 0; k < 3; ++k) { lmn[k] = N.mn[k]; lmx[k] = N.mx[k]; }
+        hl = N.hmax;
+    }
+    if (r >= n) {
+        const Leaf64 R = leaf[r - n];
+        rmn[0] = rmx[0] = R.x; rmn[1] = rmx[1] = R.y; rmn[2] = rmx[2] = R.z; hr = R.h;
+    } else {
+        const Node64 N = nodes[r];
+        for (int k = 0; k < 3; ++k) { rmn[k] = N.mn[k]; rmx[k] = N.mx[k]; }
+        hr = N.hmax;
+    }
+    Node64 out;
+    for (int k = 0; k < 3; ++k) {
+        out.mn[k] = lmn[k] < rmn[k] ? lmn[k] : rmn[k];
+        out.mx[k] = lmx[k] > rmx[k] ? lmx[k] : rmx[k];
+    }
+    out.hmax = hl > hr ? hl : hr;
+    out.left = l;
+    out.escape = escape[p];
+    nodes[p] = out;
+}
+
+// Append `p` (when `take`) to the next round's list with one atomic per wavefront.
+__device__ inline void wave_append(bool take, int32_t p, int32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    const uint64_t m = __ballot(take);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (take) list[base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_refit_first(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                                        const int32_t* __restrict__ parent,
+                                                        const int32_t* __restrict__ escape, int64_t n,
+                                                        uint32_t* __restrict__ visited, Node64* nodes,
+                                                        int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t iters = (n + stride - 1) / stride;   // uniform trip count keeps ballots convergent
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t li = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        bool take = false;
+        int32_t p = 0;
+        if (li < n) {
+            p = parent[n + li];
+            if (p > 0 && p < n) {
+                const uint32_t old = atomicAdd(&visited[p], 1u);
+                if (old == 1u) {
+                    combine_node(leaf, lr, escape, n, nodes, p);
+                    take = true;
+                }
             }
-            if (r >= n) {
-                const Leaf64 R = leaf[r - n];
-                rmn[0] = rmx[0] = R.x; rmn[1] = rmx[1] = R.y; rmn[2] = rmx[2] = R.z; hr = R.h;
-            } else {
-                const Node64* N = &nodes[r];
-                for (int k = 0; k < 3; ++k) { rmn[k] = N->mn[k]; rmx[k] = N->mx[k]; }
-                hr = N->hmax;
-            }
-            Node64 out;
-            for (int k = 0; k < 3; ++k) {
-                out.mn[k] = lmn[k] < rmn[k] ? lmn[k] : rmn[k];
-                out.mx[k] = lmx[k] > rmx[k] ? lmx[k] : rmx[k];
-            }
-            out.hmax = hl > hr ? hl : hr;
-            out.left = l;
-            out.escape = escape[p];
-            nodes[p] = out;
-            wrote = true;
-            p = parent[p];
         }
+        wave_append(take, p, next, next_count);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_refit_round(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                                        const int32_t* __restrict__ parent,
+                                                        const int32_t* __restrict__ escape, int64_t n,
+                                                        uint32_t* __restrict__ visited, Node64* nodes,
+                                                        const int32_t* __restrict__ cur, const uint32_t* __restrict__ cur_count,
+                                                        int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+    const int64_t count = *cur_count;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t iters = (count + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t k = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        bool take = false;
+        int32_t p = 0;
+        if (k < count) {
+            const int32_t x = cur[k];
+            p = (x > 0 && x < n) ? parent[x] : 0;
+            if (p > 0 && p < n) {
+                const uint32_t old = atomicAdd(&visited[p], 1u);
+                if (old == 1u) {
+                    combine_node(leaf, lr, escape, n, nodes, p);
+                    take = true;
+                }
+            }
+        }
+        wave_append(take, p, next, next_count);
     }
 }
 
@@ -606,6 +659,40 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
     }
     PMX_HIP(c, hipGetLastError());
     *result_in_alt = alt;
+    return PMX_OK;
+}
+
+// Launch the refit rounds: the list of round r has at most n/2 entries, rounds continue
+// until a round appends nothing (the root was completed).  The tree height bounds the number
+// of rounds; batches of launches are issued before the host checks the last count.
+int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
+    hipStream_t s = c->stream;
+    constexpr int kBatch = 48;
+    PMX_TRY(ensure(c, c->scratch_c, 4 * (size_t)(n + 2) + 4 * 4096));
+    int32_t* listA = c->keys_tmp.as<int32_t>();              // free after the sort (8n bytes)
+    int32_t* listB = listA + (n / 2 + 64);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(c->scratch_c.p);
+    hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, 4096, 0u);
+    const unsigned gr = std::max(1u, std::min(g, 2048u));
+    hipLaunchKernelGGL(k_refit_first, dim3(gr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
+                       c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
+                       c->nodes64.as<Node64>(), listA, &counts[0]);
+    int round = 0;
+    for (;;) {
+        for (int b = 0; b < kBatch && round < 4000; ++b, ++round) {
+            int32_t* cur = (round & 1) ? listB : listA;
+            int32_t* nxt = (round & 1) ? listA : listB;
+            hipLaunchKernelGGL(k_refit_round, dim3(std::max(1u, std::min(g / 4, 1024u))), dim3(kBlock), 0, s,
+                               c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(), c->parent.as<int32_t>(),
+                               c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(), c->nodes64.as<Node64>(), cur,
+                               &counts[round], nxt, &counts[round + 1]);
+        }
+        uint32_t last = 0;
+        PMX_HIP(c, hipMemcpyAsync(&last, &counts[round], 4, hipMemcpyDeviceToHost, s));
+        PMX_HIP(c, hipStreamSynchronize(s));
+        if (last == 0) break;
+        if (round >= 4000) return set_error(c, PMX_ERR_HIP, "refit did not converge (tree deeper than 4000)");
+    }
     return PMX_OK;
 }
 
@@ -649,8 +736,8 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     double* part = c->misc.as<double>();
     double* box = part + 6 * 1024;
     int* nan_count = reinterpret_cast<int*>(box + 6);
-    PMX_TRY(ensure(c, c->codes, 8 * (size_t)n));
-    PMX_TRY(ensure(c, c->keys_tmp, 8 * (size_t)n));
+    PMX_TRY(ensure(c, c->codes, 8 * (size_t)n + 2048));
+    PMX_TRY(ensure(c, c->keys_tmp, 8 * (size_t)n + 2048));   // also holds the refit lists (n + 128 ints)
     PMX_TRY(ensure(c, c->order, 4 * (size_t)n));
     PMX_TRY(ensure(c, c->vals_tmp, 4 * (size_t)n));
     uint64_t* codes = c->codes.as<uint64_t>();
@@ -694,6 +781,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->parent, 4 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->escape, 4 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->range_hi, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->range_lo, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->adj, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->visited, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->leaf_esc, 4 * (size_t)n));
@@ -707,15 +795,14 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)escape, 2 * n + 2, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, c->visited.as<uint32_t>(), n + 2, 0u);
     if (n >= 2) {
-        hipLaunchKernelGGL(k_brt, dim3(g), dim3(kBlock), 0, s, codes, n, lr, parent, c->range_hi.as<int32_t>());
+        hipLaunchKernelGGL(k_brt, dim3(g), dim3(kBlock), 0, s, codes, n, lr, parent, c->range_hi.as<int32_t>(),
+                           c->range_lo.as<int32_t>());
         hipLaunchKernelGGL(k_adj, dim3(g), dim3(kBlock), 0, s, codes, n, c->adj.as<int32_t>());
         hipLaunchKernelGGL(k_escape_internal, dim3(g), dim3(kBlock), 0, s, c->range_hi.as<int32_t>(),
                            c->adj.as<int32_t>(), n, escape);
     }
     hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
-    if (n >= 2)
-        hipLaunchKernelGGL(k_refit, dim3(g), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), lr, parent, escape, n,
-                           c->visited.as<uint32_t>(), c->nodes64.as<Node64>());
+    if (n >= 2) PMX_TRY(refit_rounds(c, n, g));
     PMX_HIP(c, hipGetLastError());
     PMX_HIP(c, hipEventRecord(c->ev[3], s));
     // ---- outputs ----

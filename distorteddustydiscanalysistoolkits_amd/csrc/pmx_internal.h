@@ -30,6 +30,10 @@ struct __align__(16) Node32 {
     float mnx, mny, mnz, hmax;
     float mxx, mxy, mxz;
     int32_t left;
+    int32_t escape;
+    int32_t lo;       // first leaf (0-based sorted index) of the node's range
+    int32_t cnt;      // number of leaves in the range
+    int32_t pad;
 };
 
 struct Tree {
@@ -49,7 +53,7 @@ struct pmx_ctx {
     std::string err;
     pmx::Tree tree;
     // kernel tables (device): Inorm per kernel, FP64 and FP32, and grid step dq
-    pmx::DevBuf lut64, lut32;
+    pmx::DevBuf lut64;
     double lut_dq[6] = {0, 0, 0, 0, 0, 0};
     // Morton-ordered particle data (device)
     pmx::DevBuf codes, order, keys_tmp, vals_tmp;
@@ -58,11 +62,10 @@ struct pmx_ctx {
     pmx::DevBuf leaf_esc;      // int32[n]  escape of leaf li (unified id n + li)
     pmx::DevBuf nodes64;       // Node64[n] by internal id
     pmx::DevBuf nodes32;       // Node32[n] by internal id
-    pmx::DevBuf node_esc;      // int32[n] escape by internal id
     pmx::DevBuf leaf32;        // float4[n] x y z h
     pmx::DevBuf pay32;         // float payload for the compensated path
     pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
-    pmx::DevBuf visited, adj, range_hi;
+    pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
     pmx::DevBuf pinned;  // unused placeholder
     hipEvent_t ev[12] = {};

@@ -1,39 +1,40 @@
 // pmx_interp32.hip — FP32 compensated interpolation path for gfx950.
 //
-// Design (one wavefront = one tile of 64 Morton-ordered samples):
-//   * Samples are sorted by a 30-bit Morton code of their position in the particle box, so a
-//     tile covers a compact region and its 64 traversals overlap.
+// One wavefront = one tile of 64 samples taken in Morton order of their positions.
 //   * The wavefront walks the LBVH once for the whole tile (stackless left / escape links,
-//     wave-uniform node id, scalar loads): an internal node is entered if any lane's
-//     conservative FP32 test needs it, a leaf is kept if any lane accepts it.
-//   * Accepted leaves are staged in LDS in batches of 64 together with their payload
-//     (position, h, m/rho, the catalog's field columns), loaded cooperatively by the lanes;
-//     each lane then accumulates the batch for its own sample.
-//   * Neighbour decisions are the reference's FP64 decisions: an FP32 test with a relative
-//     guard band decides the clear cases, the band is re-evaluated with the reference's FP64
-//     formula on the FP64 particle columns.  Node tests use boxes rounded outwards and hmax
-//     rounded up, so pruning is conservative; the accepted set equals the FP64 predicate set.
-//   * Kernel weights and field terms are FP32; per-sample sums are compensated (TwoSum).
-//     Derivative sums accumulate A_b - A_ref (A_ref = first neighbour's value), which leaves
-//     f - A~ b unchanged mathematically and removes the cancellation of large means.
-//     Line-integrated (column) sums accumulate in FP64, as does the Shepard division.
+//     wave-uniform node ids): a node is entered if any lane's conservative FP32 test needs
+//     it.  A node covering at most `bucket` leaves is not descended: its leaves, contiguous
+//     in Morton order, are loaded by the lanes together (coalesced) into LDS with their
+//     payload, and every lane then tests and accumulates them for its own sample.
+//   * Neighbour decisions equal the reference's FP64 decisions: an FP32 test with a relative
+//     guard band decides the clear cases; inside the band the reference's FP64 formula is
+//     evaluated on the FP64 particle columns.  Node boxes are rounded outwards and hmax up,
+//     so pruning is conservative: the accepted set is the FP64 predicate set.
+//   * Point weights, gradients and field terms are FP32 with compensated (TwoSum) sums per
+//     sample; derivative sums accumulate A_b - A_ref with A_ref the value of the nearest
+//     particle (first accepted neighbour for scatter), which leaves f - A~ b unchanged and
+//     removes the cancellation of large means.  Line-integrated weights and column sums and
+//     all Shepard divisions are FP64.
 // Reference semantics: general_interpolation.jl:1-669 (points), line_integrated_scalar_
 // interpolation.jl:123-166 (lines), LinearBVHQueries.jl:118-141 (nearest-h).
 #include <hip/hip_runtime.h>
 
-#include <vector>
+#include <algorithm>
 
 #include "pmx_internal.h"
 
 namespace pmx {
+
+int g_bucket_points = 16;
+int g_bucket_lines = 16;
+
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kMaxBucket = 64;
 constexpr float kEps = 4e-6f;     // guard band of FP32 distance / radius comparisons
+constexpr int kMaxF = 9;          // distinct field columns staged per leaf (points)
 
-// ---------------------------------------------------------------------------------------
-// FP32 tree derived from the FP64 tree
-// ---------------------------------------------------------------------------------------
 // neighbouring floats towards -inf / +inf (finite, non-NaN input)
 __device__ inline float next_down(float f) {
     if (f == 0.0f) return -__uint_as_float(1u);
@@ -52,8 +53,10 @@ __device__ inline float f32_up(double d) {
     return f;
 }
 
-__global__ void __launch_bounds__(256) k_prep_nodes(const Node64* __restrict__ n64, int64_t n, Node32* __restrict__ n32,
-                                                    int32_t* __restrict__ esc) {
+// FP32 tree: boxes rounded outwards, hmax rounded up, leaf range of each node.
+__global__ void __launch_bounds__(256) k_prep_nodes(const Node64* __restrict__ n64, const int32_t* __restrict__ rlo,
+                                                    const int32_t* __restrict__ rhi, int64_t n,
+                                                    Node32* __restrict__ n32) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x + 1; i < n; i += (int64_t)gridDim.x * 256) {
         const Node64 a = n64[i];
         Node32 b;
@@ -61,31 +64,32 @@ __global__ void __launch_bounds__(256) k_prep_nodes(const Node64* __restrict__ n
         b.mxx = f32_up(a.mx[0]); b.mxy = f32_up(a.mx[1]); b.mxz = f32_up(a.mx[2]);
         b.hmax = f32_up(a.hmax);
         b.left = a.left;
+        b.escape = a.escape;
+        b.lo = rlo[i] - 1;
+        b.cnt = rhi[i] - rlo[i] + 1;
+        b.pad = 0;
         n32[i] = b;
-        esc[i] = a.escape;
     }
 }
 
-// leaf32: {x_hi, y_hi, z_hi, h}; leaflo: {x_lo, y_lo, z_lo, 1/h}; payload column 0 = m/rho,
-// columns 1.. = quant columns, column 1+ncol = rho.
+// leaf: {x_hi, y_hi, z_hi, h}; leaflo: {x_lo, y_lo, z_lo, 1/h}; pay column 0 = m/rho,
+// columns 1.. = quant columns, column 1+ncol = rho; w64 = m/rho in FP64.
 __global__ void __launch_bounds__(256) k_prep_leaves(const double* __restrict__ cols, int64_t n, int ncol,
                                                      float4* __restrict__ leaf, float4* __restrict__ leaflo,
-                                                     float* __restrict__ pay, int* __restrict__ has_lo) {
-    int lo_seen = 0;
+                                                     float* __restrict__ pay, double* __restrict__ w64) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const double x = cols[i], y = cols[n + i], z = cols[2 * n + i];
         const double m = cols[3 * n + i], h = cols[4 * n + i], rho = cols[5 * n + i];
         const float xh = (float)x, yh = (float)y, zh = (float)z;
-        const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
-        lo_seen |= (xl != 0.0f) | (yl != 0.0f) | (zl != 0.0f);
-        const float hf = (float)h;
-        leaf[i] = make_float4(xh, yh, zh, hf);
-        leaflo[i] = make_float4(xl, yl, zl, (float)(1.0 / h));
-        pay[i] = (float)(m / rho);
+        leaf[i] = make_float4(xh, yh, zh, (float)h);
+        leaflo[i] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
+                                (float)(1.0 / h));
+        const double w = m / rho;
+        pay[i] = (float)w;
+        w64[i] = w;
         for (int c = 0; c < ncol; ++c) pay[(int64_t)(1 + c) * n + i] = (float)cols[(int64_t)(6 + c) * n + i];
         pay[(int64_t)(1 + ncol) * n + i] = (float)rho;
     }
-    if (lo_seen) atomicOr(has_lo, 1);
 }
 
 // 30-bit Morton key of a sample position inside the particle box (clamped), for tiling.
@@ -170,6 +174,7 @@ __device__ inline void shape_df32(float q, float& f, float& df) {
     }
 }
 
+
 // Compensated accumulator (TwoSum; -ffp-contract=off keeps it exact).
 struct CSum {
     float s, c;
@@ -182,58 +187,58 @@ struct CSum {
     __device__ inline double value() const { return (double)s + (double)c; }
 };
 
-// ---------------------------------------------------------------------------------------
-// Tile state helpers
-// ---------------------------------------------------------------------------------------
 struct T32 {
     const Node32* __restrict__ nodes;
-    const int32_t* __restrict__ node_esc;
     const float4* __restrict__ leaf;
     const float4* __restrict__ leaflo;
     const int32_t* __restrict__ leaf_esc;
     const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant
     const float* __restrict__ pay;         // m/rho | quant... | rho
+    const double* __restrict__ w64;        // m/rho in FP64
     int64_t n;
     int32_t root;
-    int32_t has_lo;
+    int32_t bucket;
 };
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
-
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// conservative FP32 squared distance from a point (hi + lo) to a box rounded outwards,
-// scaled down by the guard factor so that it never exceeds the exact FP64 distance
-__device__ inline float d2_box32(const float ph[3], const float pl[3], const Node32& b) {
-    const float mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float lo_gap = (mn[k] - ph[k]) - pl[k];
-        const float hi_gap = (ph[k] - mx[k]) + pl[k];
-        const float t = lo_gap > 0.0f ? lo_gap : (hi_gap > 0.0f ? hi_gap : 0.0f);
-        s += t * t;
-    }
-    return s * (1.0f - 2.0f * kEps);
-}
-
-// Exact reference point-leaf d2 in FP64 (_squared_distance_point_coords).
-__device__ inline double d2_exact64(const double p[3], const T32& T, int64_t li) {
+// Union walk of the tile over left / escape links.  need(nd) is each lane's conservative
+// node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
+template <class Need, class Bucket>
+__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket) {
     const int64_t n = T.n;
-    double s = 0.0, d;
-    d = p[0] - T.cols64[li]; s += d * d;
-    d = p[1] - T.cols64[n + li]; s += d * d;
-    d = p[2] - T.cols64[2 * n + li]; s += d * d;
-    return s;
-}
-
-template <int S>
-__device__ inline double r2_exact64(double Kv, double radius2, double hb) {
-    return radius2_rule<S>(Kv, radius2, hb);
+    if (!any_valid) return;
+    if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
+        bucket(0, (int)n);
+        return;
+    }
+    int node = T.root;
+    int steps = 0;
+    while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
+        if (node >= n) {
+            const int li = node - (int)n;
+            bucket(li, 1);
+            node = uniform(T.leaf_esc[li]);
+            continue;
+        }
+        const Node32 nd = T.nodes[node];
+        if (!ballot(need(nd))) {
+            node = uniform(nd.escape);
+            continue;
+        }
+        const int lo = uniform(nd.lo), cnt = uniform(nd.cnt);
+        if (cnt <= T.bucket && cnt >= 1 && lo >= 0 && lo + cnt <= n) {
+            bucket(lo, cnt);
+            node = uniform(nd.escape);
+            continue;
+        }
+        node = uniform(nd.left);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
-// Point queries: nearest-h and the union traversal of a tile
+// Points
 // ---------------------------------------------------------------------------------------
 struct PointLane {
     double p64[3];
@@ -252,148 +257,134 @@ __device__ inline PointLane point_lane(const double* qx, const double* qy, const
     return q;
 }
 
-__device__ inline float d2_leaf32(const PointLane& q, const float4& L, const float4& Lo) {
-    const float dx = (q.ph[0] - L.x) + (q.pl[0] - Lo.x), dy = (q.ph[1] - L.y) + (q.pl[1] - Lo.y),
-                dz = (q.ph[2] - L.z) + (q.pl[2] - Lo.z);
+// conservative FP32 squared distance from a point (hi + lo) to an outward-rounded box,
+// scaled down so it never exceeds the exact distance
+__device__ inline float d2_box32(const PointLane& q, const Node32& b) {
+    const float mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float lo_gap = (mn[k] - q.ph[k]) - q.pl[k];
+        const float hi_gap = (q.ph[k] - mx[k]) + q.pl[k];
+        const float t = lo_gap > 0.0f ? lo_gap : (hi_gap > 0.0f ? hi_gap : 0.0f);
+        s += t * t;
+    }
+    return s * (1.0f - 2.0f * kEps);
+}
+
+// Exact reference point-leaf d2 in FP64 (_squared_distance_point_coords).
+__device__ inline double d2_exact64(const double p[3], const T32& T, int64_t li) {
+    const int64_t n = T.n;
+    double s = 0.0, d;
+    d = p[0] - T.cols64[li]; s += d * d;
+    d = p[1] - T.cols64[n + li]; s += d * d;
+    d = p[2] - T.cols64[2 * n + li]; s += d * d;
+    return s;
+}
+
+// Positions of a bucket staged in LDS.
+struct BPos {
+    float x[kMaxBucket], y[kMaxBucket], z[kMaxBucket], h[kMaxBucket];
+    float xl[kMaxBucket], yl[kMaxBucket], zl[kMaxBucket], invh[kMaxBucket];
+};
+
+__device__ inline void stage_pos(const T32& T, BPos& P, int lo, int cnt) {
+    const int lane = threadIdx.x;
+    if (lane < cnt) {
+        const float4 L = T.leaf[lo + lane];
+        const float4 Lo = T.leaflo[lo + lane];
+        P.x[lane] = L.x; P.y[lane] = L.y; P.z[lane] = L.z; P.h[lane] = L.w;
+        P.xl[lane] = Lo.x; P.yl[lane] = Lo.y; P.zl[lane] = Lo.z; P.invh[lane] = Lo.w;
+    }
+}
+
+__device__ inline float d2_pos32(const PointLane& q, const BPos& P, int j, float& dx, float& dy, float& dz) {
+    dx = (q.ph[0] - P.x[j]) + (q.pl[0] - P.xl[j]);
+    dy = (q.ph[1] - P.y[j]) + (q.pl[1] - P.yl[j]);
+    dz = (q.ph[2] - P.z[j]) + (q.pl[2] - P.zl[j]);
     return dx * dx + dy * dy + dz * dz;
 }
 
-// LBVH_find_nearest_h (LinearBVHQueries.jl:118-141) for a tile: exact FP64 argmin of d2
-// with ties to the smaller sorted index (leaves are reached in increasing index order and
-// only a strictly smaller FP64 distance replaces the best).  FP32 distances only select
-// the leaves whose FP64 distance must be computed and the nodes that may be skipped.
-__device__ inline int64_t nearest32(const T32& T, const PointLane& q) {
-    const int64_t n = T.n;
+// LBVH_find_nearest_h (LinearBVHQueries.jl:118-141) for a tile: exact FP64 argmin of d2,
+// ties to the smaller sorted index (leaves come in increasing index order and only a
+// strictly smaller FP64 distance replaces the best).
+__device__ inline int64_t nearest32(const T32& T, const PointLane& q, BPos& P) {
     int64_t best = -1;
     double best64 = __builtin_inf();
     float best32 = __builtin_inff();
-    if (T.root == 0) {
-        for (int64_t li = 0; li < n; ++li) {
-            const double d2 = d2_exact64(q.p64, T, li);
-            if (q.valid && d2 < best64) { best64 = d2; best = li; }
-        }
-        return best;
-    }
-    int node = T.root;
-    int steps = 0;
-    while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
-        if (node >= n) {
-            const int li = node - (int)n;
-            const float d2 = d2_leaf32(q, T.leaf[li], T.leaflo[li]);
+    auto need = [&](const Node32& nd) {
+        return q.valid && (best < 0 || !(d2_box32(q, nd) > best32 * (1.0f + kEps)));
+    };
+    auto bucket = [&](int lo, int cnt) {
+        stage_pos(T, P, lo, cnt);
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            float dx, dy, dz;
+            const float d2 = d2_pos32(q, P, j, dx, dy, dz);
             const bool maybe = q.valid && (best < 0 || d2 <= best32 * (1.0f + kEps));
             if (ballot(maybe)) {
                 if (maybe) {
-                    const double e = d2_exact64(q.p64, T, li);
-                    if (e < best64) { best64 = e; best = li; best32 = f32_up(e); }
+                    const double e = d2_exact64(q.p64, T, lo + j);
+                    if (e < best64) { best64 = e; best = lo + j; best32 = f32_up(e); }
                 }
             }
-            node = uniform(T.leaf_esc[li]);
-            continue;
         }
-        const Node32 nd = T.nodes[node];
-        const float d2n = d2_box32(q.ph, q.pl, nd);
-        const bool need = q.valid && (best < 0 || !(d2n > best32 * (1.0f + kEps)));
-        node = ballot(need) ? uniform(nd.left) : uniform(T.node_esc[node]);
-    }
+        __syncthreads();
+    };
+    walk(T, ballot(q.valid) != 0, need, bucket);
     return best;
 }
 
-// Union traversal of a tile for the accepted sets of a strategy
-// (LinearBVHTraversalMacros.jl:26-371).  Every leaf accepted by at least one lane is
-// appended to the LDS batch (cidx, cmask = lanes accepting); on_batch(ncand) consumes a
-// full batch and at the end the remainder.
-template <int S, class OnBatch>
-__device__ inline void union_points(const T32& T, const PointLane& q, double Kv, double radius2_64, float radius2_32,
-                                    int32_t* cidx, uint64_t* cmask, OnBatch&& on_batch) {
-    const int64_t n = T.n;
-    const float Kvf = (float)Kv;
-    const int lane = threadIdx.x;
-    int ncand = 0;
-    auto visit_leaf = [&](int li) {
-        const float4 L = T.leaf[li];
-        const float d2 = d2_leaf32(q, L, T.leaflo[li]);
-        const float rb = Kvf * L.w, r2b = rb * rb;
-        const float r2 = S == kGather ? radius2_32 : (S == kScatter ? r2b : fmaxf(radius2_32, r2b));
-        bool acc = q.valid && d2 < r2 * (1.0f - kEps);
-        const bool band = q.valid && !acc && !(d2 > r2 * (1.0f + kEps));
-        if (ballot(band)) {
-            if (band) {
-                const double e = d2_exact64(q.p64, T, li);
-                acc = e <= r2_exact64<S>(Kv, radius2_64, T.cols64[4 * n + li]);
-            }
-        }
-        const uint64_t m = ballot(acc);
-        if (m) {
-            if (lane == 0) { cidx[ncand] = li; cmask[ncand] = m; }
-            ++ncand;
-            if (ncand == kWave) {
-                __syncthreads();
-                on_batch(ncand);
-                ncand = 0;
-            }
-        }
-    };
-    if (T.root == 0) {
-        for (int li = 0; li < (int)n; ++li) visit_leaf(li);
-    } else {
-        int node = T.root;
-        int steps = 0;
-        while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
-            if (node >= n) {
-                const int li = node - (int)n;
-                visit_leaf(li);
-                node = uniform(T.leaf_esc[li]);
-                continue;
-            }
-            const Node32 nd = T.nodes[node];
-            const float d2n = d2_box32(q.ph, q.pl, nd);
-            const float rb = Kvf * nd.hmax, r2b = rb * rb;
-            const float r2 = S == kGather ? radius2_32 : (S == kScatter ? r2b : fmaxf(radius2_32, r2b));
-            const bool need = q.valid && !(d2n > r2 * (1.0f + kEps));
-            node = ballot(need) ? uniform(nd.left) : uniform(T.node_esc[node]);
+// Accept test of one staged leaf with the strategy's radius rule; band -> exact FP64.
+template <int S>
+__device__ inline bool accept_point(const T32& T, const PointLane& q, float d2, float hb32, int64_t li, float Kvf,
+                                    double Kv, float radius2_32, double radius2_64) {
+    const float rb = Kvf * hb32, r2b = rb * rb;
+    const float r2 = S == kGather ? radius2_32 : (S == kScatter ? r2b : fmaxf(radius2_32, r2b));
+    bool acc = q.valid && d2 < r2 * (1.0f - kEps);
+    const bool band = q.valid && !acc && !(d2 > r2 * (1.0f + kEps));
+    if (ballot(band)) {
+        if (band) {
+            const double e = d2_exact64(q.p64, T, li);
+            acc = e <= radius2_rule<S>(Kv, radius2_64, T.cols64[4 * T.n + li]);
         }
     }
-    if (ncand > 0) {
-        __syncthreads();
-        on_batch(ncand);
-    }
+    return acc;
 }
 
-// ---------------------------------------------------------------------------------------
-// Point kernel: accumulation
-// ---------------------------------------------------------------------------------------
-constexpr int kMaxF = 12;   // distinct field columns staged per candidate
+template <int S>
+__device__ inline bool node_need_point(const PointLane& q, const Node32& nd, float Kvf, float radius2_32) {
+    const float rb = Kvf * nd.hmax, r2b = rb * rb;
+    const float r2 = S == kGather ? radius2_32 : (S == kScatter ? r2b : fmaxf(radius2_32, r2b));
+    return q.valid && !(d2_box32(q, nd) > r2 * (1.0f + kEps));
+}
 
 struct PointCat {
-    // distinct payload columns staged in LDS: payload column index (1.. = quant, 1+ncol = rho)
     int nf;
-    int fcol[kMaxF];
+    int fcol[kMaxF];  // payload column of each staged field (1.. = quant, 1+ncol = rho)
     int ns, ng, nd, nc;
-    int sf[4];        // scalar -> staged field
-    int sh[4];
-    int gf[2];        // gradient -> staged field (density gradient uses the rho column)
-    int df[6];        // divergence components -> staged fields
-    int cf[6];        // curl components -> staged fields
+    int sf[2], sh[2];
+    int gf;           // gradient -> staged field (the density gradient uses the rho column)
+    int df[3];
+    int cf[3];
     int os, og, od, oc;
 };
 
-struct Cand {
-    float x, y, z, h;
-    float xl, yl, zl, invh;
-    float w;
-    float f[kMaxF];
+struct BPay {
+    float w[kMaxBucket];
+    float f[kMaxF][kMaxBucket];
 };
 
-template <int K, int S>
+template <int K, int S, bool DERIV>
 __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const double* __restrict__ qx,
                                                     const double* __restrict__ qy, const double* __restrict__ qz,
                                                     const int32_t* __restrict__ perm, int64_t npts,
                                                     float* __restrict__ out, int64_t ldo) {
-    __shared__ Cand cand[kWave];
-    __shared__ uint64_t cmask[kWave];
-    __shared__ int32_t cidx[kWave];
+    __shared__ BPos P;
+    __shared__ BPay Y;
     const int lane = threadIdx.x;
     const double Kv = support(K);
+    const float Kvf = (float)Kv;
     const float sig = norm3<float>(K);
     const int64_t n = T.n;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
@@ -404,114 +395,109 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
         const int64_t qi = valid ? perm[slot] : 0;
         const PointLane q = point_lane(qx, qy, qz, valid, qi);
 
-        double ha64 = 0.0, radius2_64 = 0.0;
-        float radius2_32 = 0.0f;
+        double radius2_64 = 0.0;
+        float radius2_32 = 0.0f, invha = 0.0f;
+        int64_t nearest = -1;
         if (S != kScatter) {
-            const int64_t best = nearest32(T, q);
-            ha64 = best < 0 ? __builtin_nan("") : T.cols64[4 * n + best];
+            nearest = nearest32(T, q, P);
+            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * n + nearest];
             const double radius = Kv * ha64;
             radius2_64 = radius * radius;
             radius2_32 = (float)radius2_64;
+            invha = (float)(1.0 / ha64);
         }
-        const float ha32 = (float)ha64;
-        const float invha = 1.0f / ha32;
         const float invha3 = invha * invha * invha;
         const float invha4 = (invha * invha) * (invha * invha);
 
-        CSum S1{0, 0}, sc[4] = {}, b[3] = {}, gfa[2][3] = {}, gs[2] = {}, dfa[2] = {}, ds[2][3] = {}, cfa[2][3] = {},
-             cs[2][3] = {};
-        // shift references of the derivative sums (first accepted neighbour's values)
-        float refg[2] = {0, 0}, refd[2][3] = {}, refc[2][3] = {};
+        CSum S1{0, 0}, sc[2] = {}, b[3] = {}, gf[3] = {}, gs{0, 0}, df{0, 0}, ds[3] = {}, cf[3] = {}, cs[3] = {};
+        float refg = 0.0f, refd[3] = {0, 0, 0}, refc[3] = {0, 0, 0};
         bool have_ref = false;
+        if (DERIV && nearest >= 0) {
+            if (cat.ng) refg = T.pay[(int64_t)cat.fcol[cat.gf] * n + nearest];
+            for (int k = 0; k < 3; ++k) {
+                if (cat.nd) refd[k] = T.pay[(int64_t)cat.fcol[cat.df[k]] * n + nearest];
+                if (cat.nc) refc[k] = T.pay[(int64_t)cat.fcol[cat.cf[k]] * n + nearest];
+            }
+            have_ref = true;
+        }
 
-        auto on_batch = [&](int ncand) {
-            if (lane < ncand) {   // cooperative payload load
-                const int li = cidx[lane];
-                const float4 L = T.leaf[li];
-                const float4 Lo = T.leaflo[li];
-                Cand& c = cand[lane];
-                c.x = L.x; c.y = L.y; c.z = L.z; c.h = L.w;
-                c.xl = Lo.x; c.yl = Lo.y; c.zl = Lo.z; c.invh = Lo.w;
-                c.w = T.pay[li];
-                for (int f = 0; f < cat.nf; ++f) c.f[f] = T.pay[(int64_t)cat.fcol[f] * n + li];
+        auto need = [&](const Node32& nd) { return node_need_point<S>(q, nd, Kvf, radius2_32); };
+        auto bucket = [&](int lo, int cnt) {
+            stage_pos(T, P, lo, cnt);
+            if (lane < cnt) {
+                const int64_t li = lo + lane;
+                Y.w[lane] = T.pay[li];
+                for (int f = 0; f < cat.nf; ++f) Y.f[f][lane] = T.pay[(int64_t)cat.fcol[f] * n + li];
             }
             __syncthreads();
-            for (int j = 0; j < ncand; ++j) {
-                if (!((cmask[j] >> lane) & 1ull)) continue;
-                const Cand& c = cand[j];
-                const float dx = (q.ph[0] - c.x) + (q.pl[0] - c.xl), dy = (q.ph[1] - c.y) + (q.pl[1] - c.yl),
-                            dz = (q.ph[2] - c.z) + (q.pl[2] - c.zl);
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                const float r = __builtin_amdgcn_sqrtf(d2);
-                float W, gcoef;
+            for (int j = 0; j < cnt; ++j) {
+                float dx, dy, dz;
+                const float d2 = d2_pos32(q, P, j, dx, dy, dz);
+                if (!accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64)) continue;
+                const float invh = P.invh[j];
+                const float rinv = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+                const float r = d2 * rinv;
+                float W, G;
                 {
                     float fb, dfb;
-                    shape_df32<K>(r * c.invh, fb, dfb);
-                    const float ib2 = c.invh * c.invh;
-                    const float Wb = (ib2 * c.invh) * (sig * fb);
+                    shape_df32<K>(r * invh, fb, dfb);
+                    const float ib2 = invh * invh;
+                    const float Wb = (ib2 * invh) * (sig * fb);
                     const float Gb = (ib2 * ib2) * (sig * dfb);
                     if (S == kScatter) {
-                        W = Wb; gcoef = Gb;
+                        W = Wb; G = Gb;
                     } else {
                         float fa, dfa;
                         shape_df32<K>(r * invha, fa, dfa);
                         const float Wa = invha3 * (sig * fa);
                         const float Ga = invha4 * (sig * dfa);
-                        if (S == kGather) { W = Wa; gcoef = Ga; }
-                        else { W = 0.5f * (Wa + Wb); gcoef = 0.5f * (Ga + Gb); }
+                        if (S == kGather) { W = Wa; G = Ga; }
+                        else { W = 0.5f * (Wa + Wb); G = 0.5f * (Ga + Gb); }
                     }
                 }
-                const float w = c.w * W;
+                const float wb = Y.w[j];
+                const float w = wb * W;
                 S1.add(w);
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    if (s < cat.ns) sc[s].add(c.f[cat.sf[s]] * w);
-                if (cat.ng + cat.nd + cat.nc == 0) continue;
+                if (cat.ns > 0) sc[0].add(Y.f[cat.sf[0]][j] * w);
+                if (cat.ns > 1) sc[1].add(Y.f[cat.sf[1]][j] * w);
+                if (!DERIV) continue;
                 if (!have_ref) {
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        if (s < cat.ng) refg[s] = c.f[cat.gf[s]];
-                        if (s < cat.nd)
-                            for (int k = 0; k < 3; ++k) refd[s][k] = c.f[cat.df[3 * s + k]];
-                        if (s < cat.nc)
-                            for (int k = 0; k < 3; ++k) refc[s][k] = c.f[cat.cf[3 * s + k]];
+                    if (cat.ng) refg = Y.f[cat.gf][j];
+                    for (int k = 0; k < 3; ++k) {
+                        if (cat.nd) refd[k] = Y.f[cat.df[k]][j];
+                        if (cat.nc) refc[k] = Y.f[cat.cf[k]][j];
                     }
                     have_ref = true;
                 }
-                // (m/rho) grad W = (m/rho) f'(q) sigma h^-4 / r * Delta (zero at r == 0)
-                const float cg = r > 0.0f ? c.w * gcoef * __builtin_amdgcn_rcpf(r) : 0.0f;
+                // (m/rho) grad W = (m/rho) f'(q) sigma h^-4 Delta / r  (zero at r == 0)
+                const float cg = wb * G * rinv;
                 const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
                 b[0].add(gx); b[1].add(gy); b[2].add(gz);
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    if (s < cat.ng) {
-                        const float A = c.f[cat.gf[s]] - refg[s];
-                        gfa[s][0].add(A * gx); gfa[s][1].add(A * gy); gfa[s][2].add(A * gz);
-                        gs[s].add(A * w);
-                    }
-                    if (s < cat.nd) {
-                        const float Ax = c.f[cat.df[3 * s]] - refd[s][0];
-                        const float Ay = c.f[cat.df[3 * s + 1]] - refd[s][1];
-                        const float Az = c.f[cat.df[3 * s + 2]] - refd[s][2];
-                        dfa[s].add(Ax * gx + Ay * gy + Az * gz);
-                        ds[s][0].add(Ax * w); ds[s][1].add(Ay * w); ds[s][2].add(Az * w);
-                    }
-                    if (s < cat.nc) {
-                        const float Ax = c.f[cat.cf[3 * s]] - refc[s][0];
-                        const float Ay = c.f[cat.cf[3 * s + 1]] - refc[s][1];
-                        const float Az = c.f[cat.cf[3 * s + 2]] - refc[s][2];
-                        cfa[s][0].add(Ay * gz - Az * gy);
-                        cfa[s][1].add(Az * gx - Ax * gz);
-                        cfa[s][2].add(Ax * gy - Ay * gx);
-                        cs[s][0].add(Ax * w); cs[s][1].add(Ay * w); cs[s][2].add(Az * w);
-                    }
+                if (cat.ng) {
+                    const float A = Y.f[cat.gf][j] - refg;
+                    gf[0].add(A * gx); gf[1].add(A * gy); gf[2].add(A * gz);
+                    gs.add(A * w);
+                }
+                if (cat.nd) {
+                    const float Ax = Y.f[cat.df[0]][j] - refd[0], Ay = Y.f[cat.df[1]][j] - refd[1],
+                                Az = Y.f[cat.df[2]][j] - refd[2];
+                    df.add(Ax * gx + Ay * gy + Az * gz);
+                    ds[0].add(Ax * w); ds[1].add(Ay * w); ds[2].add(Az * w);
+                }
+                if (cat.nc) {
+                    const float Ax = Y.f[cat.cf[0]][j] - refc[0], Ay = Y.f[cat.cf[1]][j] - refc[1],
+                                Az = Y.f[cat.cf[2]][j] - refc[2];
+                    cf[0].add(Ay * gz - Az * gy);
+                    cf[1].add(Az * gx - Ax * gz);
+                    cf[2].add(Ax * gy - Ay * gx);
+                    cs[0].add(Ax * w); cs[1].add(Ay * w); cs[2].add(Az * w);
                 }
             }
             __syncthreads();
         };
-        union_points<S>(T, q, Kv, radius2_64, radius2_32, cidx, cmask, on_batch);
+        walk(T, ballot(valid) != 0, need, bucket);
 
-        // ---- finalisation (general_interpolation.jl:574-668) in FP64 from the compensated sums
+        // finalisation (general_interpolation.jl:574-668) in FP64 from the compensated sums
         if (valid) {
             const double s1 = S1.value();
             if (s1 == 0.0) {
@@ -522,31 +508,26 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                 for (int j = 0; j < 3 * cat.nc; ++j) out[(cat.oc + j) * ldo + qi] = nan;
             } else {
                 const double inv = 1.0 / s1;
-                const double bx = b[0].value(), by = b[1].value(), bz = b[2].value();
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (j >= cat.ns) break;
-                    const double v = sc[j].value();
-                    out[(cat.os + j) * ldo + qi] = (float)(cat.sh[j] ? v * inv : v);
-                }
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    if (j < cat.ng) {
-                        const double A = gs[j].value() * inv;
-                        out[(cat.og + 3 * j) * ldo + qi] = (float)(gfa[j][0].value() - bx * A);
-                        out[(cat.og + 3 * j + 1) * ldo + qi] = (float)(gfa[j][1].value() - by * A);
-                        out[(cat.og + 3 * j + 2) * ldo + qi] = (float)(gfa[j][2].value() - bz * A);
+                if (cat.ns > 0) out[cat.os * ldo + qi] = (float)(cat.sh[0] ? sc[0].value() * inv : sc[0].value());
+                if (cat.ns > 1) out[(cat.os + 1) * ldo + qi] = (float)(cat.sh[1] ? sc[1].value() * inv : sc[1].value());
+                if (DERIV) {
+                    const double bx = b[0].value(), by = b[1].value(), bz = b[2].value();
+                    if (cat.ng) {
+                        const double A = gs.value() * inv;
+                        out[cat.og * ldo + qi] = (float)(gf[0].value() - bx * A);
+                        out[(cat.og + 1) * ldo + qi] = (float)(gf[1].value() - by * A);
+                        out[(cat.og + 2) * ldo + qi] = (float)(gf[2].value() - bz * A);
                     }
-                    if (j < cat.nd) {
-                        const double Ax = ds[j][0].value() * inv, Ay = ds[j][1].value() * inv, Az = ds[j][2].value() * inv;
-                        out[(cat.od + j) * ldo + qi] = (float)(dfa[j].value() - (Ax * bx + Ay * by + Az * bz));
+                    if (cat.nd) {
+                        const double Ax = ds[0].value() * inv, Ay = ds[1].value() * inv, Az = ds[2].value() * inv;
+                        out[cat.od * ldo + qi] = (float)(df.value() - (Ax * bx + Ay * by + Az * bz));
                     }
-                    if (j < cat.nc) {
-                        const double Ax = cs[j][0].value() * inv, Ay = cs[j][1].value() * inv, Az = cs[j][2].value() * inv;
+                    if (cat.nc) {
+                        const double Ax = cs[0].value() * inv, Ay = cs[1].value() * inv, Az = cs[2].value() * inv;
                         const double ex = Ay * bz - Az * by, ey = Az * bx - Ax * bz, ez = Ax * by - Ay * bx;
-                        out[(cat.oc + 3 * j) * ldo + qi] = (float)(-(cfa[j][0].value() - ex));
-                        out[(cat.oc + 3 * j + 1) * ldo + qi] = (float)(-(cfa[j][1].value() - ey));
-                        out[(cat.oc + 3 * j + 2) * ldo + qi] = (float)(-(cfa[j][2].value() - ez));
+                        out[cat.oc * ldo + qi] = (float)(-(cf[0].value() - ex));
+                        out[(cat.oc + 1) * ldo + qi] = (float)(-(cf[1].value() - ey));
+                        out[(cat.oc + 2) * ldo + qi] = (float)(-(cf[2].value() - ez));
                     }
                 }
             }
@@ -554,17 +535,17 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
     }
 }
 
-// Neighbour lists / counts of the compensated path (inspection hook): the same union
-// traversal and decisions as k_points32, recording accepted leaves in DFS order.
+// Neighbour lists / counts of the compensated path (inspection hook): same walk and
+// decisions as k_points32, recording accepted leaves in increasing sorted index.
 template <int S>
 __global__ void __launch_bounds__(kWave) k_points32_nb(T32 T, double Kv, const double* __restrict__ qx,
                                                        const double* __restrict__ qy, const double* __restrict__ qz,
                                                        const double* __restrict__ ha_in, int64_t npts,
                                                        int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
                                                        int32_t* __restrict__ idx) {
-    __shared__ uint64_t cmask[kWave];
-    __shared__ int32_t cidx[kWave];
+    __shared__ BPos P;
     const int lane = threadIdx.x;
+    const float Kvf = (float)Kv;
     const int64_t n = T.n;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -576,118 +557,51 @@ __global__ void __launch_bounds__(kWave) k_points32_nb(T32 T, double Kv, const d
             double ha;
             if (ha_in) ha = valid ? ha_in[qi] : 0.0;
             else {
-                const int64_t best = nearest32(T, q);
+                const int64_t best = nearest32(T, q, P);
                 ha = best < 0 ? __builtin_nan("") : T.cols64[4 * n + best];
             }
             const double r = Kv * ha;
             radius2_64 = r * r;
         }
-        int64_t cnt = 0;
+        const float radius2_32 = (float)radius2_64;
+        int64_t cnt_acc = 0;
         int32_t* dst = (valid && offsets) ? idx + offsets[qi] : nullptr;
-        auto on_batch = [&](int ncand) {
-            for (int j = 0; j < ncand; ++j)
-                if ((cmask[j] >> lane) & 1ull) {
-                    if (dst) dst[cnt] = cidx[j];
-                    ++cnt;
+        auto need = [&](const Node32& nd) { return node_need_point<S>(q, nd, Kvf, radius2_32); };
+        auto bucket = [&](int lo, int cnt) {
+            stage_pos(T, P, lo, cnt);
+            __syncthreads();
+            for (int j = 0; j < cnt; ++j) {
+                float dx, dy, dz;
+                const float d2 = d2_pos32(q, P, j, dx, dy, dz);
+                if (accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64)) {
+                    if (dst) dst[cnt_acc] = lo + j;
+                    ++cnt_acc;
                 }
+            }
             __syncthreads();
         };
-        union_points<S>(T, q, Kv, radius2_64, (float)radius2_64, cidx, cmask, on_batch);
-        if (valid) counts[qi] = cnt;
+        walk(T, ballot(valid) != 0, need, bucket);
+        if (valid) counts[qi] = cnt_acc;
     }
 }
 
 __global__ void __launch_bounds__(kWave) k_nearest32(T32 T, const double* __restrict__ qx, const double* __restrict__ qy,
                                                      const double* __restrict__ qz, int64_t npts,
                                                      int64_t* __restrict__ idx) {
+    __shared__ BPos P;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t qi = tile * kWave + threadIdx.x;
         const bool valid = qi < npts;
         const PointLane q = point_lane(qx, qy, qz, valid, valid ? qi : 0);
-        const int64_t best = nearest32(T, q);
+        const int64_t best = nearest32(T, q, P);
         if (valid) idx[qi] = best;
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Line kernel (scatter; line_integrated_scalar_interpolation.jl:123-166)
+// Lines (scatter; line_integrated_scalar_interpolation.jl:123-166)
 // ---------------------------------------------------------------------------------------
-struct LineCat32 {
-    int n;
-    int fcol[8];    // payload column of each output (1.. = quant)
-    int shep[8];
-};
-
-struct LCand {
-    float x, y, z, h;
-    float xl, yl, zl, invh;
-    float w;
-    float f[8];
-};
-
-// Exact reference line-leaf d2 in FP64 (_squared_distance_line_coords).
-__device__ inline double d2_line_exact64(const double o[3], const double dv[3], const T32& T, int64_t li) {
-    const int64_t n = T.n;
-    double D2 = 0.0, Dm = 0.0, d;
-    d = T.cols64[li] - o[0]; D2 += d * d; Dm += d * dv[0];
-    d = T.cols64[n + li] - o[1]; D2 += d * d; Dm += d * dv[1];
-    d = T.cols64[2 * n + li] - o[2]; D2 += d * d; Dm += d * dv[2];
-    return D2 - Dm * Dm;
-}
-
-// FP32 perpendicular distance^2 of a point from a line (cross-product form, no cancellation)
-// and |Delta|^2 (for the rounding bound of the reference's D2 - Dm^2 form).
-__device__ inline float d2_line32(const float oh[3], const float ol[3], const float dv[3], float x, float y, float z,
-                                  float xl, float yl, float zl, float& D2) {
-    const float ax = (x - oh[0]) + (xl - ol[0]), ay = (y - oh[1]) + (yl - ol[1]), az = (z - oh[2]) + (zl - ol[2]);
-    D2 = ax * ax + ay * ay + az * az;
-    const float cx = ay * dv[2] - az * dv[1], cy = az * dv[0] - ax * dv[2], cz = ax * dv[1] - ay * dv[0];
-    return cx * cx + cy * cy + cz * cz;
-}
-
-// Conservative lower bound of the squared distance between a line and a box (rounded
-// outwards).  Axis-aligned directions use the exact rectangle distance in the normal plane;
-// other directions use the bounding-sphere bound of the reference with guard factors.
-__device__ inline float d2_line_box32(const float oh[3], const float ol[3], const float dv[3], int axis,
-                                      const Node32& b) {
-    const float mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
-    if (axis >= 0) {
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k == axis) continue;
-            const float lo_gap = (mn[k] - oh[k]) - ol[k];
-            const float hi_gap = (oh[k] - mx[k]) + ol[k];
-            const float t = lo_gap > 0.0f ? lo_gap : (hi_gap > 0.0f ? hi_gap : 0.0f);
-            s += t * t;
-        }
-        return s * (1.0f - 2.0f * kEps);
-    }
-    float c[3], R2 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        c[k] = 0.5f * (mn[k] + mx[k]);
-        const float hh = 0.5f * (mx[k] - mn[k]);
-        R2 += hh * hh;
-    }
-    float D2;
-    const float dc2 = d2_line32(oh, ol, dv, c[0], c[1], c[2], 0.0f, 0.0f, 0.0f, D2);
-    const float dc = __builtin_amdgcn_sqrtf(dc2) * (1.0f - kEps);
-    const float R = __builtin_amdgcn_sqrtf(R2) * (1.0f + kEps) + 1e-30f;
-    if (dc <= R) return 0.0f;
-    const float g = (dc - R) * (1.0f - kEps);
-    return g * g;
-}
-
-__device__ inline float lin_lut32(float q, float inv_dq, const float* __restrict__ I) {
-    const float idxf = q * inv_dq + 1.0f;
-    int i = (int)idxf;
-    i = i < 1 ? 1 : (i > kLutN - 1 ? kLutN - 1 : i);
-    const float t = idxf - (float)i;
-    return I[i - 1] * (1.0f - t) + I[i] * t;
-}
-
 struct LineLane {
     double o64[3], d64[3];
     float oh[3], ol[3], dv[3];
@@ -713,75 +627,96 @@ __device__ inline LineLane line_lane(const double* o, const double* dvec, int64_
     return L;
 }
 
-// Union traversal of a tile of lines, scatter rule r = K h (LinearBVHTraversalMacros.jl:539-644).
-template <class OnBatch>
-__device__ inline void union_lines(const T32& T, const LineLane& q, double Kv, int32_t* cidx, uint64_t* cmask,
-                                   OnBatch&& on_batch) {
+// Exact reference line-leaf d2 in FP64 (_squared_distance_line_coords).
+__device__ inline double d2_line_exact64(const double o[3], const double dv[3], const T32& T, int64_t li) {
     const int64_t n = T.n;
-    const float Kvf = (float)Kv;
-    const int lane = threadIdx.x;
-    int ncand = 0;
-    auto visit_leaf = [&](int li) {
-        const float4 L = T.leaf[li];
-        const float4 Lo = T.leaflo[li];
-        float D2;
-        const float d2 = d2_line32(q.oh, q.ol, q.dv, L.x, L.y, L.z, Lo.x, Lo.y, Lo.z, D2);
-        const float rb = Kvf * L.w, r2 = rb * rb;
-        // band: FP32 relative error of d2 plus the FP64 rounding of the reference's D2 - Dm^2
-        const float tol = r2 * kEps + 1e-14f * D2;
-        bool acc = q.valid && d2 < r2 - tol;
-        const bool band = q.valid && !acc && !(d2 > r2 + tol);
-        if (ballot(band)) {
-            if (band) {
-                const double e = d2_line_exact64(q.o64, q.d64, T, li);
-                const double rr = Kv * T.cols64[4 * n + li];
-                acc = e <= rr * rr;
-            }
-        }
-        const uint64_t m = ballot(acc);
-        if (m) {
-            if (lane == 0) { cidx[ncand] = li; cmask[ncand] = m; }
-            ++ncand;
-            if (ncand == kWave) {
-                __syncthreads();
-                on_batch(ncand);
-                ncand = 0;
-            }
-        }
-    };
-    if (T.root == 0) {
-        for (int li = 0; li < (int)n; ++li) visit_leaf(li);
-    } else {
-        int node = T.root;
-        int steps = 0;
-        while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
-            if (node >= n) {
-                const int li = node - (int)n;
-                visit_leaf(li);
-                node = uniform(T.leaf_esc[li]);
-                continue;
-            }
-            const Node32 nd = T.nodes[node];
-            const float d2n = d2_line_box32(q.oh, q.ol, q.dv, q.axis, nd);
-            const float rb = Kvf * nd.hmax;
-            const bool need = q.valid && !(d2n > rb * rb * (1.0f + kEps));
-            node = ballot(need) ? uniform(nd.left) : uniform(T.node_esc[node]);
-        }
-    }
-    if (ncand > 0) {
-        __syncthreads();
-        on_batch(ncand);
-    }
+    double D2 = 0.0, Dm = 0.0, d;
+    d = T.cols64[li] - o[0]; D2 += d * d; Dm += d * dv[0];
+    d = T.cols64[n + li] - o[1]; D2 += d * d; Dm += d * dv[1];
+    d = T.cols64[2 * n + li] - o[2]; D2 += d * d; Dm += d * dv[2];
+    return D2 - Dm * Dm;
 }
+
+// FP32 perpendicular distance^2 of a point from a line (cross-product form, no cancellation)
+// and |Delta|^2 (for the rounding bound of the reference's D2 - Dm^2 form).
+__device__ inline float d2_line32(const LineLane& q, float x, float y, float z, float xl, float yl, float zl,
+                                  float& D2) {
+    const float ax = (x - q.oh[0]) + (xl - q.ol[0]), ay = (y - q.oh[1]) + (yl - q.ol[1]),
+                az = (z - q.oh[2]) + (zl - q.ol[2]);
+    D2 = ax * ax + ay * ay + az * az;
+    const float cx = ay * q.dv[2] - az * q.dv[1], cy = az * q.dv[0] - ax * q.dv[2], cz = ax * q.dv[1] - ay * q.dv[0];
+    return cx * cx + cy * cy + cz * cz;
+}
+
+// Conservative lower bound of the squared distance between a line and an outward-rounded
+// box: axis-aligned directions use the exact rectangle distance in the normal plane, other
+// directions the reference's bounding-sphere bound with guard factors.
+__device__ inline float d2_line_box32(const LineLane& q, const Node32& b) {
+    const float mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
+    if (q.axis >= 0) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k == q.axis) continue;
+            const float lo_gap = (mn[k] - q.oh[k]) - q.ol[k];
+            const float hi_gap = (q.oh[k] - mx[k]) + q.ol[k];
+            const float t = lo_gap > 0.0f ? lo_gap : (hi_gap > 0.0f ? hi_gap : 0.0f);
+            s += t * t;
+        }
+        return s * (1.0f - 2.0f * kEps);
+    }
+    float c[3], R2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c[k] = 0.5f * (mn[k] + mx[k]);
+        const float hh = 0.5f * (mx[k] - mn[k]);
+        R2 += hh * hh;
+    }
+    float D2;
+    const float dc2 = d2_line32(q, c[0], c[1], c[2], 0.0f, 0.0f, 0.0f, D2);
+    const float dc = __builtin_amdgcn_sqrtf(dc2) * (1.0f - kEps);
+    const float R = __builtin_amdgcn_sqrtf(R2) * (1.0f + kEps) + 1e-30f;
+    if (dc <= R) return 0.0f;
+    const float gap = (dc - R) * (1.0f - kEps);
+    return gap * gap;
+}
+
+__device__ inline bool accept_line(const T32& T, const LineLane& q, float d2, float D2, float hb32, int64_t li,
+                                   float Kvf, double Kv) {
+    const float rb = Kvf * hb32, r2 = rb * rb;
+    // band: FP32 relative error of d2 plus the FP64 rounding of the reference's D2 - Dm^2
+    const float tol = r2 * kEps + 1e-14f * D2;
+    bool acc = q.valid && d2 < r2 - tol;
+    const bool band = q.valid && !acc && !(d2 > r2 + tol);
+    if (ballot(band)) {
+        if (band) {
+            const double e = d2_line_exact64(q.o64, q.d64, T, li);
+            const double rr = Kv * T.cols64[4 * T.n + li];
+            acc = e <= rr * rr;
+        }
+    }
+    return acc;
+}
+
+struct LineCat32 {
+    int n;
+    int fcol[8];    // payload column of each output (1.. = quant)
+    int shep[8];
+};
+
+struct BLine {
+    double w64[kMaxBucket];
+    double invh64[kMaxBucket];
+    float f[8][kMaxBucket];
+};
 
 template <int K>
 __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const double* __restrict__ o,
                                                    const double* __restrict__ dvec, const int32_t* __restrict__ perm,
-                                                   int64_t nl, const float* __restrict__ lut, float inv_dq,
+                                                   int64_t nl, const double* __restrict__ lut, double dq,
                                                    float* __restrict__ out, int64_t ldo) {
-    __shared__ LCand cand[kWave];
-    __shared__ uint64_t cmask[kWave];
-    __shared__ int32_t cidx[kWave];
+    __shared__ BPos P;
+    __shared__ BLine Y;
     const int lane = threadIdx.x;
     const double Kv = support(K);
     const float Kvf = (float)Kv;
@@ -794,34 +729,36 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
         const int64_t qi = valid ? perm[slot] : 0;
         const LineLane q = line_lane(o, dvec, nl, valid, qi);
         double S1 = 0.0, acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        auto on_batch = [&](int ncand) {
-            if (lane < ncand) {
-                const int li = cidx[lane];
-                const float4 L = T.leaf[li];
-                const float4 Lo = T.leaflo[li];
-                LCand& c = cand[lane];
-                c.x = L.x; c.y = L.y; c.z = L.z; c.h = L.w;
-                c.xl = Lo.x; c.yl = Lo.y; c.zl = Lo.z; c.invh = Lo.w;
-                c.w = T.pay[li];
-                for (int f = 0; f < cat.n; ++f) c.f[f] = T.pay[(int64_t)cat.fcol[f] * n + li];
+        auto need = [&](const Node32& nd) {
+            const float rb = Kvf * nd.hmax;
+            return q.valid && !(d2_line_box32(q, nd) > rb * rb * (1.0f + kEps));
+        };
+        auto bucket = [&](int lo, int cnt) {
+            stage_pos(T, P, lo, cnt);
+            if (lane < cnt) {
+                const int64_t li = lo + lane;
+                Y.w64[lane] = T.w64[li];
+                Y.invh64[lane] = 1.0 / T.cols64[4 * n + li];
+                for (int f = 0; f < cat.n; ++f) Y.f[f][lane] = T.pay[(int64_t)cat.fcol[f] * n + li];
             }
             __syncthreads();
-            for (int j = 0; j < ncand; ++j) {
-                if (!((cmask[j] >> lane) & 1ull)) continue;
-                const LCand& c = cand[j];
+            for (int j = 0; j < cnt; ++j) {
                 float D2;
-                const float d2 = d2_line32(q.oh, q.ol, q.dv, c.x, c.y, c.z, c.xl, c.yl, c.zl, D2);
-                const float qq = __builtin_amdgcn_sqrtf(d2) * c.invh;
-                const float I = qq >= Kvf ? 0.0f : lin_lut32(qq, inv_dq, lut);
-                const double wd = (double)(c.w * (c.invh * I));
-                S1 += wd;
+                const float d2 = d2_line32(q, P.x[j], P.y[j], P.z[j], P.xl[j], P.yl[j], P.zl[j], D2);
+                if (!accept_line(T, q, d2, D2, P.h[j], lo + j, Kvf, Kv)) continue;
+                // line_integrated_kernel_function: invh * Inorm(r * invh), evaluated in FP64
+                const double invh = Y.invh64[j];
+                const double qq = sqrt((double)d2) * invh;
+                const double I = qq >= Kv ? 0.0 : lin_lut_f64(qq, dq, lut);
+                const double w = Y.w64[j] * (invh * I);
+                S1 += w;
 #pragma unroll
                 for (int f = 0; f < 8; ++f)
-                    if (f < cat.n) acc[f] += (double)c.f[f] * wd;
+                    if (f < cat.n) acc[f] += (double)Y.f[f][j] * w;
             }
             __syncthreads();
         };
-        union_lines(T, q, Kv, cidx, cmask, on_batch);
+        walk(T, ballot(valid) != 0, need, bucket);
         if (valid) {
             if (S1 == 0.0) {
                 for (int f = 0; f < cat.n; ++f) out[f * ldo + qi] = __builtin_nanf("");
@@ -839,45 +776,54 @@ __global__ void __launch_bounds__(kWave) k_lines32_nb(T32 T, double Kv, const do
                                                       const double* __restrict__ dvec, int64_t nl,
                                                       int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
                                                       int32_t* __restrict__ idx) {
-    __shared__ uint64_t cmask[kWave];
-    __shared__ int32_t cidx[kWave];
-    const int lane = threadIdx.x;
+    __shared__ BPos P;
+    const float Kvf = (float)Kv;
     const int64_t ntiles = (nl + kWave - 1) / kWave;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t qi = tile * kWave + lane;
+        const int64_t qi = tile * kWave + threadIdx.x;
         const bool valid = qi < nl;
         const LineLane q = line_lane(o, dvec, nl, valid, valid ? qi : 0);
-        int64_t cnt = 0;
+        int64_t cnt_acc = 0;
         int32_t* dst = (valid && offsets) ? idx + offsets[qi] : nullptr;
-        auto on_batch = [&](int ncand) {
-            for (int j = 0; j < ncand; ++j)
-                if ((cmask[j] >> lane) & 1ull) {
-                    if (dst) dst[cnt] = cidx[j];
-                    ++cnt;
+        auto need = [&](const Node32& nd) {
+            const float rb = Kvf * nd.hmax;
+            return q.valid && !(d2_line_box32(q, nd) > rb * rb * (1.0f + kEps));
+        };
+        auto bucket = [&](int lo, int cnt) {
+            stage_pos(T, P, lo, cnt);
+            __syncthreads();
+            for (int j = 0; j < cnt; ++j) {
+                float D2;
+                const float d2 = d2_line32(q, P.x[j], P.y[j], P.z[j], P.xl[j], P.yl[j], P.zl[j], D2);
+                if (accept_line(T, q, d2, D2, P.h[j], lo + j, Kvf, Kv)) {
+                    if (dst) dst[cnt_acc] = lo + j;
+                    ++cnt_acc;
                 }
+            }
             __syncthreads();
         };
-        union_lines(T, q, Kv, cidx, cmask, on_batch);
-        if (valid) counts[qi] = cnt;
+        walk(T, ballot(valid) != 0, need, bucket);
+        if (valid) counts[qi] = cnt_acc;
     }
 }
 
-T32 tree32(pmx_ctx* c) {
+T32 tree32(pmx_ctx* c, int bucket) {
     T32 T;
     T.nodes = c->nodes32.as<Node32>();
-    T.node_esc = c->node_esc.as<int32_t>();
     T.leaf = c->leaf32.as<float4>();
     T.leaflo = c->leaf32.as<float4>() + c->tree.n;
     T.leaf_esc = c->leaf_esc.as<int32_t>();
     T.cols64 = c->cols64.as<double>();
     T.pay = c->pay32.as<float>();
+    T.w64 = reinterpret_cast<const double*>(c->pay32.as<float>() + (size_t)c->tree.n * (2 + c->tree.ncol) +
+                                            ((size_t)c->tree.n * (2 + c->tree.ncol) & 1));
     T.n = c->tree.n;
     T.root = c->tree.root;
-    T.has_lo = 1;
+    T.bucket = std::max(1, std::min(bucket, kMaxBucket));
     return T;
 }
 
-// Morton order of the samples (queries): permutation into c->scratch_b
+// Morton order of the samples (queries): permutation in c->scratch_a or c->scratch_b
 int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm) {
     PMX_TRY(ensure(c, c->scratch_a, 8 * (size_t)npts + 4 * (size_t)npts + 64));
     PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)npts + 4 * (size_t)npts + 64));
@@ -899,28 +845,37 @@ int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, 
     return PMX_OK;
 }
 
-template <int K>
+unsigned tile_grid(int64_t nq) {
+    const int64_t ntiles = (nq + kWave - 1) / kWave;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
+}
+
+template <int K, bool DERIV>
 void launch_points32(pmx_ctx* c, int S, const PointCat& cat, const double* qx, const double* qy, const double* qz,
                      const int32_t* perm, int64_t npts, float* out, int64_t ldo) {
-    const int64_t ntiles = (npts + kWave - 1) / kWave;
-    const dim3 g((unsigned)std::min<int64_t>(ntiles, 1 << 20)), b(kWave);
-    T32 T = tree32(c);
+    const dim3 g(tile_grid(npts)), b(kWave);
+    T32 T = tree32(c, g_bucket_points);
     if (S == kGather)
-        hipLaunchKernelGGL((k_points32<K, kGather>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo);
+        hipLaunchKernelGGL((k_points32<K, kGather, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo);
     else if (S == kScatter)
-        hipLaunchKernelGGL((k_points32<K, kScatter>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo);
+        hipLaunchKernelGGL((k_points32<K, kScatter, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo);
     else
-        hipLaunchKernelGGL((k_points32<K, kSymmetric>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo);
+        hipLaunchKernelGGL((k_points32<K, kSymmetric, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out,
+                           ldo);
+}
+
+template <int K>
+void launch_points32_k(pmx_ctx* c, int S, const PointCat& cat, const double* qx, const double* qy, const double* qz,
+                       const int32_t* perm, int64_t npts, float* out, int64_t ldo) {
+    if (cat.ng + cat.nd + cat.nc > 0) launch_points32<K, true>(c, S, cat, qx, qy, qz, perm, npts, out, ldo);
+    else launch_points32<K, false>(c, S, cat, qx, qy, qz, perm, npts, out, ldo);
 }
 
 template <int K>
 void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const double* d, const int32_t* perm,
                     int64_t nl, float* out, int64_t ldo) {
-    const int64_t ntiles = (nl + kWave - 1) / kWave;
-    const dim3 g((unsigned)std::min<int64_t>(ntiles, 1 << 20)), b(kWave);
-    const float inv_dq = (float)(1.0 / c->lut_dq[K]);
-    hipLaunchKernelGGL(k_lines32<K>, g, b, 0, c->stream, tree32(c), cat, o, d, perm, nl,
-                       c->lut32.as<float>() + (size_t)K * kLutN, inv_dq, out, ldo);
+    hipLaunchKernelGGL(k_lines32<K>, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines), cat, o, d,
+                       perm, nl, c->lut64.as<double>() + (size_t)K * kLutN, c->lut_dq[K], out, ldo);
 }
 
 }  // namespace
@@ -929,18 +884,17 @@ int prepare_f32(pmx_ctx* c) {
     if (c->tree.f32_ready) return PMX_OK;
     const int64_t n = c->tree.n;
     const int ncol = c->tree.ncol;
+    const size_t payf = (size_t)n * (2 + ncol);
     PMX_TRY(ensure(c, c->nodes32, sizeof(Node32) * (size_t)(n + 1)));
-    PMX_TRY(ensure(c, c->node_esc, 4 * (size_t)(n + 1)));
     PMX_TRY(ensure(c, c->leaf32, 2 * sizeof(float4) * (size_t)n));
-    PMX_TRY(ensure(c, c->pay32, 4 * (size_t)n * (2 + ncol)));
-    PMX_TRY(ensure(c, c->misc, sizeof(double) * (6 * 1024 + 8) + 64));
-    int* has_lo = reinterpret_cast<int*>(c->misc.as<double>() + 6 * 1024 + 7);
+    PMX_TRY(ensure(c, c->pay32, 4 * (payf + (payf & 1)) + 8 * (size_t)n));
     const dim3 g(grid_for(n, 256, 16384)), b(256);
     if (n >= 2)
-        hipLaunchKernelGGL(k_prep_nodes, g, b, 0, c->stream, c->nodes64.as<Node64>(), n, c->nodes32.as<Node32>(),
-                           c->node_esc.as<int32_t>());
+        hipLaunchKernelGGL(k_prep_nodes, g, b, 0, c->stream, c->nodes64.as<Node64>(), c->range_lo.as<int32_t>(),
+                           c->range_hi.as<int32_t>(), n, c->nodes32.as<Node32>());
     hipLaunchKernelGGL(k_prep_leaves, g, b, 0, c->stream, c->cols64.as<double>(), n, ncol, c->leaf32.as<float4>(),
-                       c->leaf32.as<float4>() + n, c->pay32.as<float>(), has_lo);
+                       c->leaf32.as<float4>() + n, c->pay32.as<float>(),
+                       reinterpret_cast<double*>(c->pay32.as<float>() + payf + (payf & 1)));
     PMX_HIP(c, hipGetLastError());
     c->tree.f32_ready = true;
     return PMX_OK;
@@ -949,29 +903,39 @@ int prepare_f32(pmx_ctx* c) {
 int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const double* qx, const double* qy,
                        const double* qz, int64_t npts, float* out, int64_t ldo) {
     const int ncol = c->tree.ncol;
-    PointCat cat{};
-    auto field = [&](int paycol) {
-        for (int f = 0; f < cat.nf; ++f)
-            if (cat.fcol[f] == paycol) return f;
-        cat.fcol[cat.nf] = paycol;
-        return cat.nf++;
-    };
-    cat.ns = ch.ns; cat.ng = ch.ng; cat.nd = ch.nd; cat.nc = ch.nc;
-    for (int j = 0; j < ch.ns; ++j) { cat.sf[j] = field(1 + ch.ss[j]); cat.sh[j] = ch.sh[j]; }
-    for (int j = 0; j < ch.ng; ++j) cat.gf[j] = field(ch.gs[j] < 0 ? 1 + ncol : 1 + ch.gs[j]);
-    for (int j = 0; j < 3 * ch.nd; ++j) cat.df[j] = field(1 + ch.ds[j]);
-    for (int j = 0; j < 3 * ch.nc; ++j) cat.cf[j] = field(1 + ch.cs[j]);
-    if (cat.nf > kMaxF) return set_error(c, PMX_ERR_ARGUMENT, "catalog chunk uses too many distinct columns");
-    cat.os = ch.os; cat.og = ch.og; cat.od = ch.od; cat.oc = ch.oc;
     int32_t* perm = nullptr;
     PMX_TRY(sort_queries(c, qx, qy, qz, npts, &perm));
-    switch (c->tree.kernel) {
-    case kM4: launch_points32<kM4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
-    case kM5: launch_points32<kM5>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
-    case kM6: launch_points32<kM6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
-    case kC2: launch_points32<kC2>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
-    case kC4: launch_points32<kC4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
-    default: launch_points32<kC6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+    // split the chunk into sub-chunks of <= 2 scalars, 1 gradient, 1 divergence, 1 curl
+    const int nsub = std::max({(ch.ns + 1) / 2, ch.ng, ch.nd, ch.nc, 1});
+    for (int k = 0; k < nsub; ++k) {
+        PointCat cat{};
+        auto field = [&](int paycol) {
+            for (int f = 0; f < cat.nf; ++f)
+                if (cat.fcol[f] == paycol) return f;
+            cat.fcol[cat.nf] = paycol;
+            return cat.nf++;
+        };
+        cat.ns = std::max(0, std::min(2, ch.ns - 2 * k));
+        cat.ng = k < ch.ng ? 1 : 0;
+        cat.nd = k < ch.nd ? 1 : 0;
+        cat.nc = k < ch.nc ? 1 : 0;
+        for (int j = 0; j < cat.ns; ++j) { cat.sf[j] = field(1 + ch.ss[2 * k + j]); cat.sh[j] = ch.sh[2 * k + j]; }
+        if (cat.ng) cat.gf = field(ch.gs[k] < 0 ? 1 + ncol : 1 + ch.gs[k]);
+        if (cat.nd) for (int j = 0; j < 3; ++j) cat.df[j] = field(1 + ch.ds[3 * k + j]);
+        if (cat.nc) for (int j = 0; j < 3; ++j) cat.cf[j] = field(1 + ch.cs[3 * k + j]);
+        cat.os = ch.os + 2 * k;
+        cat.og = ch.og + 3 * k;
+        cat.od = ch.od + k;
+        cat.oc = ch.oc + 3 * k;
+        if (cat.ns + cat.ng + cat.nd + cat.nc == 0 && k > 0) continue;
+        switch (c->tree.kernel) {
+        case kM4: launch_points32_k<kM4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        case kM5: launch_points32_k<kM5>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        case kM6: launch_points32_k<kM6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        case kC2: launch_points32_k<kC2>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        case kC4: launch_points32_k<kC4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        default: launch_points32_k<kC6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo); break;
+        }
     }
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
@@ -1002,9 +966,8 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
 
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx) {
     PMX_TRY(prepare_f32(c));
-    const int64_t ntiles = (npts + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_nearest32, dim3((unsigned)std::min<int64_t>(ntiles, 1 << 20)), dim3(kWave), 0, c->stream,
-                       tree32(c), qx, qy, qz, npts, idx);
+    hipLaunchKernelGGL(k_nearest32, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_points), qx, qy,
+                       qz, npts, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }
@@ -1012,10 +975,9 @@ int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* q
 int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,
                           const double* ha, int64_t npts, int64_t* counts, const int64_t* offsets, int32_t* idx) {
     PMX_TRY(prepare_f32(c));
-    const int64_t ntiles = (npts + kWave - 1) / kWave;
-    const dim3 g((unsigned)std::min<int64_t>(ntiles, 1 << 20)), b(kWave);
+    const dim3 g(tile_grid(npts)), b(kWave);
     const double Kv = support(c->tree.kernel);
-    T32 T = tree32(c);
+    T32 T = tree32(c, g_bucket_points);
     if (strategy == kGather)
         hipLaunchKernelGGL(k_points32_nb<kGather>, g, b, 0, c->stream, T, Kv, qx, qy, qz, ha, npts, counts, offsets, idx);
     else if (strategy == kScatter)
@@ -1030,9 +992,8 @@ int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const doub
 int neighbors_lines_f32c(pmx_ctx* c, const double* o, const double* d, int64_t nl, int64_t* counts,
                          const int64_t* offsets, int32_t* idx) {
     PMX_TRY(prepare_f32(c));
-    const int64_t ntiles = (nl + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_lines32_nb, dim3((unsigned)std::min<int64_t>(ntiles, 1 << 20)), dim3(kWave), 0, c->stream,
-                       tree32(c), support(c->tree.kernel), o, d, nl, counts, offsets, idx);
+    hipLaunchKernelGGL(k_lines32_nb, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines),
+                       support(c->tree.kernel), o, d, nl, counts, offsets, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }
