@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark of the LBVH build + SPH line-integrated interpolation hot path on one MI355X.
+"""Benchmark of the LBVH build + SPH interpolation hot path on one MI355X.
 
-Workload (default, the 16M-particle configuration the headline metric is quoted on):
-  warped disc, 1.6e7 particles stored FP32, Wendland C4 kernel, face-on 1024 x 1024 column
-  map (lines along +z), scalars (rho: column density, not Shepard-normalised; vz: column-
-  averaged, Shepard-normalised), itpScatter.
+Workloads
+  lines  (default; the 16M-particle configuration the headline metric is quoted on):
+         warped disc, 1.6e7 particles stored FP32, Wendland C4 kernel, face-on 1024 x 1024
+         column map (lines along +z), scalars rho (column density, not Shepard-normalised)
+         and vz (column-averaged, Shepard-normalised), itpScatter.
+  points flat disc, 4e6 particles stored FP32, M4 cubic spline, 256 x 256 x 64 cylindrical
+         grid, scalar rho, grad rho, div v, curl v (8 outputs), itpSymmetric.
 A step is one LBVH build from device-resident particle columns followed by one
-interpolation of all map lines into device-resident outputs (the work of
-LineSamples_interpolation with the data already in HBM).
+interpolation of all samples into device-resident outputs (the work of
+LineSamples_interpolation / StructuredGrid_interpolation with the data already in HBM).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32c|f64|both]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lines|points]
+                    [--precision best|f32c|f64|both]
 
-Prints one JSON line.  `value` is map lines (grid points) per second over whole steps.
+Prints one JSON line.  `value` is grid points (map lines or grid nodes) per second over
+whole steps (build + interpolation).
 """
 from __future__ import annotations
 
@@ -27,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+KERNEL_M4, KERNEL_C4 = 0, 4
 
 
 def parse():
@@ -34,16 +41,81 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="lines", choices=["lines", "points"])
     ap.add_argument("--precision", default="best", choices=["best", "f32c", "f64", "both"])
-    ap.add_argument("--particles", type=int, default=16_000_000)
-    ap.add_argument("--map", type=int, default=1024)
+    ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--map", type=int, default=1024, help="lines workload: map side")
+    ap.add_argument("--grid", default="256,256,64", help="points workload: ns,nphi,nz")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-lines", type=int, default=16384, help="lines in the CPU baseline sample")
+    ap.add_argument("--cpu-samples", type=int, default=16384, help="samples in the CPU baseline leg")
     return ap.parse_args()
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+class Workload:
+    """Device-resident particles + samples of one benchmark configuration."""
+
+    def __init__(self, a, torch, dev):
+        from distorteddustydiscanalysistoolkits_amd import _lib, disc
+        self.kind = a.workload
+        t0 = time.time()
+        if self.kind == "lines":
+            self.n = a.particles or 16_000_000
+            self.d = disc.make_disc(self.n, seed=0, warped=True)
+            self.qnames = ("rho", "vz")
+            self.kernel = KERNEL_C4
+            self.kname = "C4"
+            o, dv = disc.faceon_lines(a.map)
+            self.samples = (o, dv)
+            self.P = a.map * a.map
+            self.dsamples = ([torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in o],
+                             [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in dv])
+            self.cat_d = dict(scalar_slots=[0, 1], scalar_shepard=[0, 1])
+            self.L = 2
+            self.strategy = _lib.PMX_SCATTER
+            self.fields = ("rho_column", "vz_column")
+            self.desc = (f"warped disc {self.n} particles (FP32 storage), Wendland C4, face-on {a.map}x{a.map} "
+                         "column map: column density + column-averaged vz, itpScatter")
+        else:
+            self.n = a.particles or 4_000_000
+            self.d = disc.make_disc(self.n, seed=0, warped=False)
+            self.qnames = ("rho", "vx", "vy", "vz")
+            self.kernel = KERNEL_M4
+            self.kname = "M4"
+            ns, nphi, nz = (int(v) for v in a.grid.split(","))
+            sa, pa, za = disc.cylindrical_axes((1.0, 100.0, ns), (0.0, 2 * np.pi, nphi), (-5.0, 5.0, nz))
+            q = disc.flatten_cylindrical(sa, pa, za)
+            self.samples = q
+            self.P = q[0].size
+            self.dsamples = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in q]
+            self.cat_d = dict(scalar_slots=[0], scalar_shepard=[0], grad_slots=[0], div_slots=[1, 2, 3],
+                              curl_slots=[1, 2, 3])
+            self.L = 8
+            self.strategy = _lib.PMX_SYMMETRIC
+            self.fields = ("rho", "grad_rho_x", "grad_rho_y", "grad_rho_z", "div_v", "curl_v_x", "curl_v_y",
+                           "curl_v_z")
+            self.desc = (f"flat disc {self.n} particles (FP32 storage), M4 cubic spline, cylindrical "
+                         f"{ns}x{nphi}x{nz} grid: rho, grad rho, div v, curl v, itpSymmetric")
+        self.cat = _lib.catalog_struct(**self.cat_d)
+        names = ("x", "y", "z", "m", "h", "rho")
+        self.host32 = {k: np.ascontiguousarray(self.d[k], np.float32) for k in names + self.qnames}
+        self.dcols = [torch.from_numpy(self.host32[k]).to(dev) for k in names]
+        self.dquant = [torch.from_numpy(self.host32[k]).to(dev) for k in self.qnames]
+        log(f"[bench] workload {self.kind}: n={self.n} samples={self.P} prepared in {time.time() - t0:.1f}s")
+
+    def interpolate(self, ctx, prec, outs):
+        if self.kind == "lines":
+            ctx.interpolate_lines(self.strategy, prec, self.cat, self.dsamples[0], self.dsamples[1], outs)
+        else:
+            ctx.interpolate_points(self.strategy, prec, self.cat, *self.dsamples, outs)
+
+    def stats(self, ctx):
+        if self.kind == "lines":
+            return ctx.line_stats(*self.dsamples)
+        return ctx.point_stats(self.strategy, *self.dsamples)
 
 
 def main():
@@ -54,98 +126,103 @@ def main():
         return 2
     import torch
 
-    from distorteddustydiscanalysistoolkits_amd import _lib, disc
+    from distorteddustydiscanalysistoolkits_amd import _lib
 
     dev = torch.device("cuda:0")
-    n, M = a.particles, a.map
-    t0 = time.time()
-    d = disc.make_disc(n, seed=0, warped=True)
-    names = ("x", "y", "z", "m", "h", "rho")
-    host32 = {k: np.ascontiguousarray(d[k], np.float32) for k in names + ("vz",)}
-    log(f"[bench] disc generated n={n} in {time.time() - t0:.1f}s")
-    dcols = [torch.from_numpy(host32[k]).to(dev) for k in names]
-    dquant = [torch.from_numpy(host32["rho"]).to(dev), torch.from_numpy(host32["vz"]).to(dev)]
-    o, dvec = disc.faceon_lines(M)
-    do = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in o]
-    dd = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in dvec]
-    P = M * M
-    cat = _lib.catalog_struct(scalar_slots=[0, 1], scalar_shepard=[0, 1])
-    KERNEL_C4 = 4
+    w = Workload(a, torch, dev)
     ctx = _lib.Context(0)
 
     def run(prec, steps, warmup):
-        outs = [torch.empty(P, dtype=torch.float32 if prec == _lib.PMX_PREC_F32C else torch.float64, device=dev)
-                for _ in range(2)]
+        outs = [torch.empty(w.P, dtype=torch.float32 if prec == _lib.PMX_PREC_F32C else torch.float64, device=dev)
+                for _ in range(w.L)]
 
         def step():
-            ctx.build(*dcols, quant=dquant, kernel=KERNEL_C4)
+            ctx.build(*w.dcols, quant=w.dquant, kernel=w.kernel)
             tb = ctx.timings()[0]
-            ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat, do, dd, outs)
-            return tb, ctx.timings()[5]
+            w.interpolate(ctx, prec, outs)
+            t = ctx.timings()
+            return tb, t[5], t[7]
 
         for _ in range(warmup):
             step()
         torch.cuda.synchronize()
-        tb_list, ti_list = [], []
+        tb_list, ti_list, tk_list = [], [], []
         t = time.perf_counter()
         for _ in range(steps):
-            tb, ti = step()
+            tb, ti, tk = step()
             tb_list.append(tb)
             ti_list.append(ti)
+            tk_list.append(tk)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, build_ms=float(np.median(tb_list)),
-                    interp_ms=float(np.median(ti_list)), outs=[o_.double().cpu().numpy() for o_ in outs])
+                    interp_ms=float(np.median(ti_list)), kernel_ms=float(np.mean(tk_list)),
+                    outs=[o_.double().cpu().numpy() for o_ in outs])
 
     precs = {"f64": [_lib.PMX_PREC_F64], "f32c": [_lib.PMX_PREC_F32C],
-             "both": [_lib.PMX_PREC_F32C, _lib.PMX_PREC_F64], "best": [_lib.PMX_PREC_F32C, _lib.PMX_PREC_F64]}[a.precision]
+             "both": [_lib.PMX_PREC_F32C, _lib.PMX_PREC_F64],
+             "best": [_lib.PMX_PREC_F32C, _lib.PMX_PREC_F64]}[a.precision]
     results = {}
     for prec in precs:
         key = "f32c" if prec == _lib.PMX_PREC_F32C else "f64"
         try:
             log(f"[bench] timing precision={key} steps={a.steps} warmup={a.warmup}")
             results[key] = run(prec, a.steps, a.warmup)
-            log(f"[bench] {key}: {results[key]['ms_per_step']:.2f} ms/step "
-                f"(build {results[key]['build_ms']:.2f} ms, interp {results[key]['interp_ms']:.2f} ms)")
+            r = results[key]
+            log(f"[bench] {key}: {r['ms_per_step']:.2f} ms/step (build {r['build_ms']:.2f} ms, interp "
+                f"{r['interp_ms']:.2f} ms, kernels {r['kernel_ms']:.2f} ms)")
         except _lib.PmxError as e:
             log(f"[bench] precision {key} unavailable: {e}")
     main_key = "f32c" if "f32c" in results else "f64"
     r = results[main_key]
 
-    # traversal statistics -> algorithmic bytes of the line kernel (per launch)
-    st = ctx.line_stats(do, dd)
+    # Algorithmic bytes of the interpolation kernels per launch: traversal records (FP32 node
+    # 36 B, leaf 20 B; FP64 node 64 B, leaf 36 B), one particle record per accepted pair
+    # ((6 + k) columns x 4 B, x 8 B on the FP64 path), sample input and outputs.
+    st = w.stats(ctx)
     n_int, n_leaf, n_acc = int(st[2]), int(st[3]), int(st[4])
-    ncolq = 2
-    if main_key == "f64":
-        # node record 64 B, leaf record 32 B + escape 4 B, accepted: m, rho, 2 quant (8 B each),
-        # line in 48 B, outputs 2 x 8 B
-        bytes_launch = 64 * n_int + 36 * n_leaf + 8 * (2 + ncolq) * n_acc + P * (48 + 8 * 2)
+    n_near_int, n_near_leaf = int(st[0]), int(st[1])
+    k = len(w.qnames)
+    if w.kind == "lines":
+        sample_bytes = 48
     else:
-        bytes_launch = 32 * n_int + 20 * n_leaf + 4 * (1 + ncolq) * n_acc + P * (48 + 4 * 2)
-    achieved = bytes_launch / (r["interp_ms"] * 1e-3) / 1e9
+        sample_bytes = 24
+    if main_key == "f64":
+        rec, node, leaf, outb = 8 * (6 + k), 64, 36, 8
+    else:
+        rec, node, leaf, outb = 4 * (6 + k), 36, 20, 4
+    trav_bytes = node * (n_int + n_near_int) + leaf * (n_leaf + n_near_leaf)
+    pair_bytes = rec * n_acc
+    bytes_launch = trav_bytes + pair_bytes + w.P * (sample_bytes + outb * w.L)
+    t_kernel = r["kernel_ms"] * 1e-3
+    achieved = bytes_launch / t_kernel / 1e9
+    flops_pair = 103 if w.kind == "points" else 30
+    flops = flops_pair * n_acc
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"line_{main_key}")
+            traffic = json.load(open(tfile)).get(f"{w.kind}_{main_key}")
         except Exception:
             traffic = None
 
     errors = {}
     if "f32c" in results and "f64" in results:
         from tests.helpers import rel_norms
-        for j, nm in enumerate(("rho_column", "vz_column")):
+        for j, nm in enumerate(w.fields):
             linf, l2 = rel_norms(results["f32c"]["outs"][j], results["f64"]["outs"][j])
             errors[nm] = {"rel_linf": linf, "rel_l2": l2}
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(d, o, dvec, M, a.cpu_lines)
+        cpu = cpu_baseline(w, a.cpu_samples)
 
+    P, n = w.P, w.n
     line = {
         "metric": "interp grid points/s + LBVH build particles/s at 16M parts, FP32c vs FP64",
         "value": P / (r["ms_per_step"] * 1e-3),
-        "unit": "grid points/s (1024^2 column-map lines per second, build + interpolation per step)",
+        "unit": ("grid points/s (column-map lines per second, build + interpolation per step)" if w.kind == "lines"
+                 else "grid points/s (grid nodes per second, build + interpolation per step)"),
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -154,23 +231,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if main_key == "f32c" else "f64",
-        "data": "synthetic (deterministic SplitMix64 warped disc, seed 0)",
-        "config": {"workload": f"warped disc {n} particles (FP32 storage), Wendland C4, face-on {M}x{M} "
-                               "column map: column density + column-averaged vz, itpScatter",
-                   "particles": n, "lines": P, "path": main_key},
+        "data": f"synthetic (deterministic SplitMix64 {'warped' if w.kind == 'lines' else 'flat'} disc, seed 0)",
+        "config": {"workload": w.desc, "particles": n, "samples": P, "path": main_key},
         "build_particles_per_s": n / (r["build_ms"] * 1e-3),
         "interp_points_per_s": P / (r["interp_ms"] * 1e-3),
         "build_ms": r["build_ms"],
         "interp_ms": r["interp_ms"],
-        "paths": {k: {"ms_per_step": v["ms_per_step"], "build_ms": v["build_ms"], "interp_ms": v["interp_ms"],
-                      "grid_points_per_s": P / (v["ms_per_step"] * 1e-3),
-                      "interp_points_per_s": P / (v["interp_ms"] * 1e-3),
-                      "build_particles_per_s": n / (v["build_ms"] * 1e-3)} for k, v in results.items()},
+        "kernel_ms": r["kernel_ms"],
+        "paths": {kk: {"ms_per_step": v["ms_per_step"], "build_ms": v["build_ms"], "interp_ms": v["interp_ms"],
+                       "kernel_ms": v["kernel_ms"],
+                       "grid_points_per_s": P / (v["ms_per_step"] * 1e-3),
+                       "interp_points_per_s": P / (v["interp_ms"] * 1e-3),
+                       "build_particles_per_s": n / (v["build_ms"] * 1e-3)} for kk, v in results.items()},
         "field_error_f32c_vs_f64": errors or None,
-        "traversal": {"internal_tested": n_int, "leaves_tested": n_leaf, "leaves_accepted": n_acc},
+        "traversal": {"nearest_internal": n_near_int, "nearest_leaves": n_near_leaf, "internal_tested": n_int,
+                      "leaves_tested": n_leaf, "leaves_accepted": n_acc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"line interpolation ({main_key})", "algorithmic_bytes_per_launch": bytes_launch},
+                     "kernel": f"{w.kind} interpolation ({main_key})",
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "pair_stream_GBs": pair_bytes / t_kernel / 1e9,
+                     "pair_stream_frac": pair_bytes / t_kernel / 1e9 / HBM_PEAK_GBS,
+                     "valu_TFLOPs": flops / t_kernel / 1e12,
+                     "valu_frac": flops / t_kernel / 1e12 / FP32_PEAK_TFLOPS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -178,32 +261,35 @@ def main():
     return 0
 
 
-def cpu_baseline(d, o, dvec, M, nsample):
+def cpu_baseline(w, nsample):
     """The oracle (C FP64 restatement of the reference CPU algorithm, OpenMP where the reference
-    threads) on this host: the full LBVH build of the same particle set, then the line kernel
-    on an evenly strided sample of the map lines; whole-step throughput is extrapolated as
-    P / (t_build + t_sample * P / sample)."""
+    threads) on this host: the full LBVH build of the same particle set, then the sample
+    kernel on an evenly strided subset of the samples; whole-step throughput is extrapolated
+    as P / (t_build + t_sample * P / sample)."""
     from oracle import oracle as O
     nthreads = min(os.cpu_count() or 1, 16)
-    cols = [np.array(d[k], np.float32).astype(np.float64) for k in ("x", "y", "z", "m", "h", "rho")]
-    quant = [np.array(d["rho"], np.float32).astype(np.float64), np.array(d["vz"], np.float32).astype(np.float64)]
+    cols = [w.host32[k].astype(np.float64) for k in ("x", "y", "z", "m", "h", "rho")]
+    quant = [w.host32[k].astype(np.float64) for k in w.qnames]
     log(f"[bench] cpu baseline: oracle build n={cols[0].size} threads={nthreads}")
     t = time.perf_counter()
     tree = O.LBVH(*cols, quant=quant, nthreads=nthreads)
     tb = time.perf_counter() - t
-    P = M * M
+    P = w.P
     step = max(1, P // nsample)
     sel = np.arange(0, P, step)[:nsample]
-    oo = [v[sel] for v in o]
-    dd = [v[sel] for v in dvec]
-    log(f"[bench] cpu baseline: oracle lines sample={sel.size}")
+    log(f"[bench] cpu baseline: oracle samples={sel.size}")
     t = time.perf_counter()
-    tree.line_samples("C4", "scatter", [0, 1], [0, 1], oo, dd, nthreads=nthreads)
+    if w.kind == "lines":
+        o, dv = w.samples
+        tree.line_samples(w.kname, "scatter", [0, 1], [0, 1], [v[sel] for v in o], [v[sel] for v in dv],
+                          nthreads=nthreads)
+    else:
+        tree.point_samples(w.kname, "symmetric", w.cat_d, *[v[sel] for v in w.samples], nthreads=nthreads)
     ti = time.perf_counter() - t
     total = tb + ti * P / sel.size
     return {"value": P / total, "unit": "grid points/s", "cores": nthreads, "kind": "port",
-            "sample": f"full LBVH build of {cols[0].size} particles ({tb:.2f} s) + {sel.size} of {P} lines "
-                      f"({ti:.2f} s), extrapolated to the full map",
+            "sample": f"full LBVH build of {cols[0].size} particles ({tb:.2f} s) + {sel.size} of {P} samples "
+                      f"({ti:.2f} s), extrapolated to all samples",
             "build_particles_per_s": cols[0].size / tb, "interp_points_per_s": sel.size / ti}
 
 

@@ -290,9 +290,13 @@ int pmx_interpolate_points(pmx_ctx* c, int32_t strategy, int32_t precision, cons
     PMX_HIP(c, hipEventRecord(c->ev[5], c->stream));
     if (f32) {
         PMX_TRY(prepare_f32(c));
+        int32_t* perm = nullptr;
+        PMX_TRY(sort_queries(c, qx, qy, qz, n, &perm));
+        PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         for (const CatChunk& ch : chunks)
-            PMX_TRY(interp_points_f32c(c, strategy, ch, qx, qy, qz, n, c->outbuf.as<float>(), n));
+            PMX_TRY(interp_points_f32c(c, strategy, ch, perm, qx, qy, qz, n, c->outbuf.as<float>(), n));
     } else {
+        PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         // ha from the first chunk is reused by later chunks (same nearest-h search result)
         double* ha = nullptr;
         if (chunks.size() > 1 && strategy != kScatter) {
@@ -307,9 +311,11 @@ int pmx_interpolate_points(pmx_ctx* c, int32_t strategy, int32_t precision, cons
     PMX_TRY(deliver(c, c->outbuf.p, f32, L, n, out));
     PMX_HIP(c, hipEventRecord(c->ev[7], c->stream));
     PMX_HIP(c, hipStreamSynchronize(c->stream));
-    float a = 0, b = 0;
+    float a = 0, b = 0, k = 0;
     PMX_HIP(c, hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
     PMX_HIP(c, hipEventElapsedTime(&b, c->ev[4], c->ev[7]));
+    PMX_HIP(c, hipEventElapsedTime(&k, c->ev[8], c->ev[6]));
+    c->timings[7] = k;
     c->timings[4] = 0.0;
     c->timings[5] = a;
     c->timings[6] = b;
@@ -339,17 +345,23 @@ int pmx_interpolate_lines(pmx_ctx* c, int32_t strategy, int32_t precision, const
     PMX_HIP(c, hipEventRecord(c->ev[5], c->stream));
     if (f32) {
         PMX_TRY(prepare_f32(c));
-        PMX_TRY(interp_lines_f32c(c, fc.ss.data(), fc.sh.data(), L, o, d, n, c->outbuf.as<float>(), n));
+        int32_t* perm = nullptr;
+        PMX_TRY(sort_queries(c, o, o + n, o + 2 * n, n, &perm));
+        PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
+        PMX_TRY(interp_lines_f32c(c, fc.ss.data(), fc.sh.data(), L, perm, o, d, n, c->outbuf.as<float>(), n));
     } else {
+        PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         PMX_TRY(interp_lines_f64(c, strategy, fc.ss.data(), fc.sh.data(), L, o, d, n, c->outbuf.as<double>(), n));
     }
     PMX_HIP(c, hipEventRecord(c->ev[6], c->stream));
     PMX_TRY(deliver(c, c->outbuf.p, f32, L, n, out));
     PMX_HIP(c, hipEventRecord(c->ev[7], c->stream));
     PMX_HIP(c, hipStreamSynchronize(c->stream));
-    float a = 0, b = 0;
+    float a = 0, b = 0, k = 0;
     PMX_HIP(c, hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
     PMX_HIP(c, hipEventElapsedTime(&b, c->ev[4], c->ev[7]));
+    PMX_HIP(c, hipEventElapsedTime(&k, c->ev[8], c->ev[6]));
+    c->timings[7] = k;
     c->timings[5] = a;
     c->timings[6] = b;
     return PMX_OK;

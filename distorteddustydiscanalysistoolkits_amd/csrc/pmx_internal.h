@@ -130,10 +130,13 @@ int kernel_eval_f64(pmx_ctx* c, int kernel, const double* q, int64_t n, double* 
 int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, const double* b, const double* cc,
                         int64_t n, unsigned long long* tot);
 
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx,
+// Morton order of the samples (FP32-compensated path tiles); the permutation lives in the
+// context's scratch buffers until the next call that uses them.
+int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm);
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm, const double* qx,
                        const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
-int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o,
-                      const double* d, int64_t nl, float* out, int64_t ldo);
+int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,
+                      const double* o, const double* d, int64_t nl, float* out, int64_t ldo);
 int prepare_f32(pmx_ctx* c);
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx);
 int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,

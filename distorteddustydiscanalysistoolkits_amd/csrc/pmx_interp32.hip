@@ -11,10 +11,11 @@
 //     evaluated on the FP64 particle columns.  Node boxes are rounded outwards and hmax up,
 //     so pruning is conservative: the accepted set is the FP64 predicate set.
 //   * Point weights, gradients and field terms are FP32 with compensated (TwoSum) sums per
-//     sample; derivative sums accumulate A_b - A_ref with A_ref the value of the nearest
-//     particle (first accepted neighbour for scatter), which leaves f - A~ b unchanged and
-//     removes the cancellation of large means.  Line-integrated weights and column sums and
-//     all Shepard divisions are FP64.
+//     sample; derivative sums accumulate A_b - A_ref, formed in FP64 and rounded once, with
+//     A_ref one particle's value per tile, which leaves f - A~ b unchanged and removes the
+//     cancellation of large means.  Line maps test pairs in FP32; accepted pairs get their
+//     perpendicular distance, weight and column sums in FP64 (the weight is a small share of
+//     the work), and all Shepard divisions are FP64.
 // Reference semantics: general_interpolation.jl:1-669 (points), line_integrated_scalar_
 // interpolation.jl:123-166 (lines), LinearBVHQueries.jl:118-141 (nearest-h).
 #include <hip/hip_runtime.h>
@@ -359,12 +360,16 @@ __device__ inline bool node_need_point(const PointLane& q, const Node32& nd, flo
     return q.valid && !(d2_box32(q, nd) > r2 * (1.0f + kEps));
 }
 
+// Scalars are staged from the FP32 payload; the fields under a derivative are staged as
+// FP32 differences A_b - A_ref computed in FP64 from the FP64 columns, with A_ref one
+// particle's value per tile (the shift leaves every derivative estimate unchanged and keeps
+// the FP32 rounding relative to the local variation instead of the field's magnitude).
 struct PointCat {
-    int nf;
-    int fcol[kMaxF];  // payload column of each staged field (1.. = quant, 1+ncol = rho)
     int ns, ng, nd, nc;
-    int sf[2], sh[2];
-    int gf;           // gradient -> staged field (the density gradient uses the rho column)
+    int scol[2], sh[2];   // payload column of each scalar (1.. = quant, 1+ncol = rho)
+    int nder;
+    int dcol[kMaxF];      // FP64 column of each derivative field (5 = rho, 6.. = quant)
+    int gf;               // gradient -> derivative field
     int df[3];
     int cf[3];
     int os, og, od, oc;
@@ -372,7 +377,8 @@ struct PointCat {
 
 struct BPay {
     float w[kMaxBucket];
-    float f[kMaxF][kMaxBucket];
+    float s[2][kMaxBucket];
+    float d[kMaxF][kMaxBucket];
 };
 
 template <int K, int S, bool DERIV>
@@ -410,24 +416,23 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
         const float invha4 = (invha * invha) * (invha * invha);
 
         CSum S1{0, 0}, sc[2] = {}, b[3] = {}, gf[3] = {}, gs{0, 0}, df{0, 0}, ds[3] = {}, cf[3] = {}, cs[3] = {};
-        float refg = 0.0f, refd[3] = {0, 0, 0}, refc[3] = {0, 0, 0};
+        double ref64[kMaxF];
         bool have_ref = false;
-        if (DERIV && nearest >= 0) {
-            if (cat.ng) refg = T.pay[(int64_t)cat.fcol[cat.gf] * n + nearest];
-            for (int k = 0; k < 3; ++k) {
-                if (cat.nd) refd[k] = T.pay[(int64_t)cat.fcol[cat.df[k]] * n + nearest];
-                if (cat.nc) refc[k] = T.pay[(int64_t)cat.fcol[cat.cf[k]] * n + nearest];
-            }
-            have_ref = true;
-        }
 
         auto need = [&](const Node32& nd) { return node_need_point<S>(q, nd, Kvf, radius2_32); };
         auto bucket = [&](int lo, int cnt) {
+            if (DERIV && !have_ref) {      // tile reference: first leaf of the first bucket (uniform)
+                for (int f = 0; f < cat.nder; ++f) ref64[f] = T.cols64[(int64_t)cat.dcol[f] * n + lo];
+                have_ref = true;
+            }
             stage_pos(T, P, lo, cnt);
             if (lane < cnt) {
                 const int64_t li = lo + lane;
                 Y.w[lane] = T.pay[li];
-                for (int f = 0; f < cat.nf; ++f) Y.f[f][lane] = T.pay[(int64_t)cat.fcol[f] * n + li];
+                for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
+                if (DERIV)
+                    for (int f = 0; f < cat.nder; ++f)
+                        Y.d[f][lane] = (float)(T.cols64[(int64_t)cat.dcol[f] * n + li] - ref64[f]);
             }
             __syncthreads();
             for (int j = 0; j < cnt; ++j) {
@@ -458,35 +463,25 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                 const float wb = Y.w[j];
                 const float w = wb * W;
                 S1.add(w);
-                if (cat.ns > 0) sc[0].add(Y.f[cat.sf[0]][j] * w);
-                if (cat.ns > 1) sc[1].add(Y.f[cat.sf[1]][j] * w);
+                if (cat.ns > 0) sc[0].add(Y.s[0][j] * w);
+                if (cat.ns > 1) sc[1].add(Y.s[1][j] * w);
                 if (!DERIV) continue;
-                if (!have_ref) {
-                    if (cat.ng) refg = Y.f[cat.gf][j];
-                    for (int k = 0; k < 3; ++k) {
-                        if (cat.nd) refd[k] = Y.f[cat.df[k]][j];
-                        if (cat.nc) refc[k] = Y.f[cat.cf[k]][j];
-                    }
-                    have_ref = true;
-                }
                 // (m/rho) grad W = (m/rho) f'(q) sigma h^-4 Delta / r  (zero at r == 0)
                 const float cg = wb * G * rinv;
                 const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
                 b[0].add(gx); b[1].add(gy); b[2].add(gz);
                 if (cat.ng) {
-                    const float A = Y.f[cat.gf][j] - refg;
+                    const float A = Y.d[cat.gf][j];
                     gf[0].add(A * gx); gf[1].add(A * gy); gf[2].add(A * gz);
                     gs.add(A * w);
                 }
                 if (cat.nd) {
-                    const float Ax = Y.f[cat.df[0]][j] - refd[0], Ay = Y.f[cat.df[1]][j] - refd[1],
-                                Az = Y.f[cat.df[2]][j] - refd[2];
+                    const float Ax = Y.d[cat.df[0]][j], Ay = Y.d[cat.df[1]][j], Az = Y.d[cat.df[2]][j];
                     df.add(Ax * gx + Ay * gy + Az * gz);
                     ds[0].add(Ax * w); ds[1].add(Ay * w); ds[2].add(Az * w);
                 }
                 if (cat.nc) {
-                    const float Ax = Y.f[cat.cf[0]][j] - refc[0], Ay = Y.f[cat.cf[1]][j] - refc[1],
-                                Az = Y.f[cat.cf[2]][j] - refc[2];
+                    const float Ax = Y.d[cat.cf[0]][j], Ay = Y.d[cat.cf[1]][j], Az = Y.d[cat.cf[2]][j];
                     cf[0].add(Ay * gz - Az * gy);
                     cf[1].add(Az * gx - Ax * gz);
                     cf[2].add(Ax * gy - Ay * gx);
@@ -746,9 +741,15 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
                 float D2;
                 const float d2 = d2_line32(q, P.x[j], P.y[j], P.z[j], P.xl[j], P.yl[j], P.zl[j], D2);
                 if (!accept_line(T, q, d2, D2, P.h[j], lo + j, Kvf, Kv)) continue;
-                // line_integrated_kernel_function: invh * Inorm(r * invh), evaluated in FP64
+                // line_integrated_kernel_function: invh * Inorm(r * invh), evaluated in FP64 with
+                // the perpendicular distance from the hi + lo particle position
+                const double ax = ((double)P.x[j] - q.o64[0]) + (double)P.xl[j];
+                const double ay = ((double)P.y[j] - q.o64[1]) + (double)P.yl[j];
+                const double az = ((double)P.z[j] - q.o64[2]) + (double)P.zl[j];
+                const double cx = ay * q.d64[2] - az * q.d64[1], cy = az * q.d64[0] - ax * q.d64[2],
+                             cz = ax * q.d64[1] - ay * q.d64[0];
                 const double invh = Y.invh64[j];
-                const double qq = sqrt((double)d2) * invh;
+                const double qq = sqrt(cx * cx + cy * cy + cz * cz) * invh;
                 const double I = qq >= Kv ? 0.0 : lin_lut_f64(qq, dq, lut);
                 const double w = Y.w64[j] * (invh * I);
                 S1 += w;
@@ -823,28 +824,6 @@ T32 tree32(pmx_ctx* c, int bucket) {
     return T;
 }
 
-// Morton order of the samples (queries): permutation in c->scratch_a or c->scratch_b
-int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm) {
-    PMX_TRY(ensure(c, c->scratch_a, 8 * (size_t)npts + 4 * (size_t)npts + 64));
-    PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)npts + 4 * (size_t)npts + 64));
-    uint64_t* k0 = c->scratch_a.as<uint64_t>();
-    int32_t* v0 = reinterpret_cast<int32_t*>(k0 + npts);
-    uint64_t* k1 = c->scratch_b.as<uint64_t>();
-    int32_t* v1 = reinterpret_cast<int32_t*>(k1 + npts);
-    const dim3 g(grid_for(npts, 256, 8192)), b(256);
-    if (c->tree.n >= 2) {
-        hipLaunchKernelGGL(k_query_keys, g, b, 0, c->stream, x, y, z, 0, npts, c->nodes64.as<Node64>(), k0, v0);
-        bool alt = false;
-        PMX_TRY(radix_sort_pairs(c, k0, v0, npts, k1, v1, 0, 32, &alt));
-        *perm = alt ? v1 : v0;
-    } else {
-        hipLaunchKernelGGL(k_iota, g, b, 0, c->stream, k0, v0, npts);
-        *perm = v0;
-    }
-    PMX_HIP(c, hipGetLastError());
-    return PMX_OK;
-}
-
 unsigned tile_grid(int64_t nq) {
     const int64_t ntiles = (nq + kWave - 1) / kWave;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
@@ -880,6 +859,28 @@ void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const dou
 
 }  // namespace
 
+// Morton order of the samples (queries): permutation in c->scratch_a or c->scratch_b
+int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm) {
+    PMX_TRY(ensure(c, c->scratch_a, 8 * (size_t)npts + 4 * (size_t)npts + 64));
+    PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)npts + 4 * (size_t)npts + 64));
+    uint64_t* k0 = c->scratch_a.as<uint64_t>();
+    int32_t* v0 = reinterpret_cast<int32_t*>(k0 + npts);
+    uint64_t* k1 = c->scratch_b.as<uint64_t>();
+    int32_t* v1 = reinterpret_cast<int32_t*>(k1 + npts);
+    const dim3 g(grid_for(npts, 256, 8192)), b(256);
+    if (c->tree.n >= 2) {
+        hipLaunchKernelGGL(k_query_keys, g, b, 0, c->stream, x, y, z, 0, npts, c->nodes64.as<Node64>(), k0, v0);
+        bool alt = false;
+        PMX_TRY(radix_sort_pairs(c, k0, v0, npts, k1, v1, 0, 32, &alt));
+        *perm = alt ? v1 : v0;
+    } else {
+        hipLaunchKernelGGL(k_iota, g, b, 0, c->stream, k0, v0, npts);
+        *perm = v0;
+    }
+    PMX_HIP(c, hipGetLastError());
+    return PMX_OK;
+}
+
 int prepare_f32(pmx_ctx* c) {
     if (c->tree.f32_ready) return PMX_OK;
     const int64_t n = c->tree.n;
@@ -900,29 +901,26 @@ int prepare_f32(pmx_ctx* c) {
     return PMX_OK;
 }
 
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const double* qx, const double* qy,
-                       const double* qz, int64_t npts, float* out, int64_t ldo) {
-    const int ncol = c->tree.ncol;
-    int32_t* perm = nullptr;
-    PMX_TRY(sort_queries(c, qx, qy, qz, npts, &perm));
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32_t* perm, const double* qx,
+                       const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo) {
     // split the chunk into sub-chunks of <= 2 scalars, 1 gradient, 1 divergence, 1 curl
     const int nsub = std::max({(ch.ns + 1) / 2, ch.ng, ch.nd, ch.nc, 1});
     for (int k = 0; k < nsub; ++k) {
         PointCat cat{};
-        auto field = [&](int paycol) {
-            for (int f = 0; f < cat.nf; ++f)
-                if (cat.fcol[f] == paycol) return f;
-            cat.fcol[cat.nf] = paycol;
-            return cat.nf++;
+        auto dfield = [&](int col64) {
+            for (int f = 0; f < cat.nder; ++f)
+                if (cat.dcol[f] == col64) return f;
+            cat.dcol[cat.nder] = col64;
+            return cat.nder++;
         };
         cat.ns = std::max(0, std::min(2, ch.ns - 2 * k));
         cat.ng = k < ch.ng ? 1 : 0;
         cat.nd = k < ch.nd ? 1 : 0;
         cat.nc = k < ch.nc ? 1 : 0;
-        for (int j = 0; j < cat.ns; ++j) { cat.sf[j] = field(1 + ch.ss[2 * k + j]); cat.sh[j] = ch.sh[2 * k + j]; }
-        if (cat.ng) cat.gf = field(ch.gs[k] < 0 ? 1 + ncol : 1 + ch.gs[k]);
-        if (cat.nd) for (int j = 0; j < 3; ++j) cat.df[j] = field(1 + ch.ds[3 * k + j]);
-        if (cat.nc) for (int j = 0; j < 3; ++j) cat.cf[j] = field(1 + ch.cs[3 * k + j]);
+        for (int j = 0; j < cat.ns; ++j) { cat.scol[j] = 1 + ch.ss[2 * k + j]; cat.sh[j] = ch.sh[2 * k + j]; }
+        if (cat.ng) cat.gf = dfield(ch.gs[k] < 0 ? 5 : 6 + ch.gs[k]);
+        if (cat.nd) for (int j = 0; j < 3; ++j) cat.df[j] = dfield(6 + ch.ds[3 * k + j]);
+        if (cat.nc) for (int j = 0; j < 3; ++j) cat.cf[j] = dfield(6 + ch.cs[3 * k + j]);
         cat.os = ch.os + 2 * k;
         cat.og = ch.og + 3 * k;
         cat.od = ch.od + k;
@@ -941,10 +939,8 @@ int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const doubl
     return PMX_OK;
 }
 
-int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
-                      int64_t nl, float* out, int64_t ldo) {
-    int32_t* perm = nullptr;
-    PMX_TRY(sort_queries(c, o, o + nl, o + 2 * nl, nl, &perm));
+int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,
+                      const double* o, const double* d, int64_t nl, float* out, int64_t ldo) {
     for (int base = 0; base < ncols || (ncols == 0 && base == 0); base += 8) {
         LineCat32 cat{};
         cat.n = ncols - base < 8 ? ncols - base : 8;

@@ -198,7 +198,8 @@ int pmx_line_kernel_table(int32_t kernel, double *Q, double *Inorm);
 /* --- timing ------------------------------------------------------------------------ */
 /* Device time (ms, HIP events on the context stream) of the phases of the last build /
  * interpolation call: [0] build total, [1] encode+sort, [2] tree+refit, [3] column gather,
- * [4] nearest-h, [5] interpolation kernel, [6] last interpolation call total. */
+ * [4] nearest-h (unused: included in [5]), [5] interpolation phase (sample ordering and
+ * kernels), [6] last interpolation call total, [7] interpolation kernels only. */
 int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
 
 #ifdef __cplusplus
