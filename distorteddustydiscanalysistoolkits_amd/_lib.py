@@ -26,7 +26,7 @@ EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_bu
            "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
            "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
            "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings", "pmx_point_traversal_stats",
-           "pmx_line_traversal_stats")
+           "pmx_line_traversal_stats", "pmx_line_neighbor_digest")
 
 
 class PmxError(RuntimeError):
@@ -115,6 +115,7 @@ def lib():
         L.pmx_last_timings.argtypes = [vp, vp, i32]
         L.pmx_point_traversal_stats.argtypes = [vp, i32, C.POINTER(Points), vp]
         L.pmx_line_traversal_stats.argtypes = [vp, C.POINTER(Lines), vp]
+        L.pmx_line_neighbor_digest.argtypes = [vp, C.POINTER(Lines), vp, C.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -308,6 +309,18 @@ class Context:
         st = np.zeros(5, np.int64)
         self._check(lib().pmx_line_traversal_stats(self._h, C.byref(ln), st.ctypes.data))
         return st
+
+    def line_digest(self, origin, direction):
+        """(count, index sum, index-square sum) per line from the tiled line-map path, or None
+        when the lines do not take that path."""
+        ln = self._lines(origin, direction)
+        n = int(ln.n)
+        dg = np.zeros(3 * max(n, 1), np.uint64)
+        used = C.c_int32(0)
+        self._check(lib().pmx_line_neighbor_digest(self._h, C.byref(ln), dg.ctypes.data, C.byref(used)))
+        if not used.value:
+            return None
+        return dg[:n], dg[n:2 * n], dg[2 * n:3 * n]
 
     def kernel_eval(self, kernel, q):
         q = np.ascontiguousarray(q, np.float64)

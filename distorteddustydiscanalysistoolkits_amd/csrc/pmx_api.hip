@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -215,9 +216,21 @@ extern "C" {
 
 const char* pmx_version(void) { return "partia_mi355x 0.1 (gfx950)"; }
 
+// Launch-shape knobs (environment, read at context creation): PMX_LINES_TILED=0 sends every
+// line set through the union walk; PMX_BUCKET_POINTS / PMX_BUCKET_LINES set the leaf-run
+// size the sample-tile walks hand over at once (1..64); PMX_TILED_BUCKET the leaf-run size of
+// the line-map tile walk (1..65536).
+static void read_tuning() {
+    if (const char* v = std::getenv("PMX_LINES_TILED")) g_lines_tiled = std::atoi(v) != 0;
+    if (const char* v = std::getenv("PMX_BUCKET_POINTS")) g_bucket_points = std::max(1, std::min(64, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_BUCKET_LINES")) g_bucket_lines = std::max(1, std::min(64, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_TILED_BUCKET")) g_tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
+}
+
 int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
     if (!out) return PMX_ERR_ARGUMENT;
     *out = nullptr;
+    read_tuning();
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         (void)hipGetLastError();
@@ -235,8 +248,16 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
         if (hipEventCreate(&e) != hipSuccess) { delete c; return PMX_ERR_HIP; }
     std::vector<double> h64;
     upload_tables(c, h64);
-    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK ||
-        hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    std::vector<double> h2(2 * h64.size());   // (Inorm[i], Inorm[i+1] - Inorm[i])
+    for (int k = 0; k < 6; ++k)
+        for (int i = 0; i < kLutN; ++i) {
+            const double a = h64[k * kLutN + i], b = i + 1 < kLutN ? h64[k * kLutN + i + 1] : a;
+            h2[2 * (k * kLutN + i)] = a;
+            h2[2 * (k * kLutN + i) + 1] = b - a;
+        }
+    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK || ensure(c, c->lut2, 8 * h2.size()) != PMX_OK ||
+        hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->lut2.p, h2.data(), 8 * h2.size(), hipMemcpyHostToDevice) != hipSuccess) {
         pmx_destroy(c);
         return PMX_ERR_HIP;
     }
@@ -248,11 +269,11 @@ void pmx_destroy(pmx_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->lut64, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
+    DevBuf* bufs[] = {&c->lut64, &c->lut2, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
-                      &c->pinned};
+                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : c->ev)
@@ -345,10 +366,14 @@ int pmx_interpolate_lines(pmx_ctx* c, int32_t strategy, int32_t precision, const
     PMX_HIP(c, hipEventRecord(c->ev[5], c->stream));
     if (f32) {
         PMX_TRY(prepare_f32(c));
-        int32_t* perm = nullptr;
-        PMX_TRY(sort_queries(c, o, o + n, o + 2 * n, n, &perm));
-        PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
-        PMX_TRY(interp_lines_f32c(c, fc.ss.data(), fc.sh.data(), L, perm, o, d, n, c->outbuf.as<float>(), n));
+        bool tiled = false;
+        PMX_TRY(interp_lines_tiled(c, fc.ss.data(), fc.sh.data(), L, o, d, n, c->outbuf.as<float>(), n, &tiled));
+        if (!tiled) {
+            int32_t* perm = nullptr;
+            PMX_TRY(sort_queries(c, o, o + n, o + 2 * n, n, &perm));
+            PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
+            PMX_TRY(interp_lines_f32c(c, fc.ss.data(), fc.sh.data(), L, perm, o, d, n, c->outbuf.as<float>(), n));
+        }
     } else {
         PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         PMX_TRY(interp_lines_f64(c, strategy, fc.ss.data(), fc.sh.data(), L, o, d, n, c->outbuf.as<double>(), n));
@@ -557,6 +582,28 @@ int pmx_line_traversal_stats(pmx_ctx* c, const pmx_lines* lines, int64_t* stats)
     PMX_HIP(c, hipMemsetAsync(tot, 0, 40, c->stream));
     PMX_TRY(traversal_stats_f64(c, 1, kScatter, o, d, nullptr, lines->n, tot));
     PMX_HIP(c, hipMemcpyAsync(stats, tot, 40, hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP(c, hipStreamSynchronize(c->stream));
+    return PMX_OK;
+}
+
+int pmx_line_neighbor_digest(pmx_ctx* c, const pmx_lines* lines, uint64_t* digest, int32_t* used) {
+    PMX_TRY(require_tree(c));
+    PMX_HIP(c, hipSetDevice(c->device));
+    PMX_TRY(check_lines(c, lines));
+    if (!digest || !used) return set_error(c, PMX_ERR_ARGUMENT, "digest/used: null");
+    *used = 0;
+    const int64_t n = lines->n;
+    if (n == 0) return PMX_OK;
+    const double *o, *d;
+    PMX_TRY(stage_lines(c, lines, &o, &d));
+    PMX_TRY(prepare_f32(c));
+    PMX_TRY(ensure(c, c->outbuf, 24 * (size_t)n));
+    bool tiled = false;
+    PMX_TRY(line_digest_tiled(c, o, d, n, c->outbuf.as<uint64_t>(), &tiled));
+    if (tiled) {
+        PMX_HIP(c, hipMemcpyAsync(digest, c->outbuf.p, 24 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        *used = 1;
+    }
     PMX_HIP(c, hipStreamSynchronize(c->stream));
     return PMX_OK;
 }

@@ -392,13 +392,36 @@ struct ColPtrs {
     const void* p[24];
 };
 
+// The input columns are first interleaved into one record per particle (coalesced reads
+// and writes), so the permutation reads one record per particle instead of one element per
+// column from unrelated places.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_gather(ColPtrs src, int ncols, const int32_t* __restrict__ order,
-                                                   int64_t n, double* __restrict__ cols64) {
+__global__ void __launch_bounds__(kBlock) k_pack_records(ColPtrs src, int ncols, int64_t n, T* __restrict__ rec) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        for (int c = 0; c < ncols; ++c) rec[i * ncols + c] = static_cast<const T*>(src.p[c])[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__ rec, int ncols,
+                                                           const int32_t* __restrict__ order, int64_t n,
+                                                           double* __restrict__ cols64, Leaf64* __restrict__ leaf) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         int32_t o = order[i];
         o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;   // in-bounds guard (order is a permutation)
-        for (int c = 0; c < ncols; ++c) cols64[(int64_t)c * n + i] = (double)static_cast<const T*>(src.p[c])[o];
+        const T* r = rec + (int64_t)o * ncols;
+        double v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            v[c] = (double)r[c];
+            cols64[(int64_t)c * n + i] = v[c];
+        }
+        for (int c = 6; c < ncols; ++c) cols64[(int64_t)c * n + i] = (double)r[c];
+        Leaf64 L;
+        L.x = v[0];
+        L.y = v[1];
+        L.z = v[2];
+        L.h = v[4];
+        leaf[i] = L;
     }
 }
 
@@ -571,29 +594,72 @@ __device__ inline void wave_append(bool take, int32_t p, int32_t* __restrict__ l
     if (take) list[base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = p;
 }
 
-__global__ void __launch_bounds__(kBlock) k_refit_first(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
-                                                        const int32_t* __restrict__ parent,
-                                                        const int32_t* __restrict__ escape, int64_t n,
+// Nodes whose leaf run is short are computed directly from their leaves (min / max are
+// exact, so the result does not depend on the order).  Every other node starts its counter
+// at the number of children that are already complete; nodes with both children complete
+// form the first list of the bottom-up rounds.
+constexpr int kSmallRange = 16;
+
+__global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                                        const int32_t* __restrict__ escape,
+                                                        const int32_t* __restrict__ range_lo,
+                                                        const int32_t* __restrict__ range_hi, int64_t n,
                                                         uint32_t* __restrict__ visited, Node64* nodes,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    const int64_t iters = (n + stride - 1) / stride;   // uniform trip count keeps ballots convergent
+    const int64_t m = n - 1;                            // internal nodes 1 .. n-1
+    const int64_t iters = (m + stride - 1) / stride;    // uniform trip count keeps ballots convergent
     for (int64_t it = 0; it < iters; ++it) {
-        const int64_t li = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        const int64_t i = 1 + it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
         bool take = false;
-        int32_t p = 0;
-        if (li < n) {
-            p = parent[n + li];
-            if (p > 0 && p < n) {
-                const uint32_t old = atomicAdd(&visited[p], 1u);
-                if (old == 1u) {
-                    combine_node(leaf, lr, escape, n, nodes, p);
-                    take = true;
+        if (i < n) {
+            const int32_t lo = range_lo[i], hi = range_hi[i];
+            if (lo >= 1 && hi <= n && hi - lo + 1 <= kSmallRange) {
+                double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+                double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+                double hm = -__builtin_inf();
+                for (int32_t k = lo - 1; k < hi; ++k) {
+                    const Leaf64 L = leaf[k];
+                    mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
+                    mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
+                    mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];
+                    hm = L.h > hm ? L.h : hm;
                 }
+                Node64 out;
+                for (int k = 0; k < 3; ++k) { out.mn[k] = mn[k]; out.mx[k] = mx[k]; }
+                out.hmax = hm;
+                out.left = lr[2 * i];
+                out.escape = escape[i];
+                nodes[i] = out;
+            } else {
+                uint32_t done = 0;
+                for (int side = 0; side < 2; ++side) {
+                    const int32_t ch = lr[2 * i + side];
+                    if (ch >= n) done += 1;
+                    else if (ch > 0 && range_hi[ch] - range_lo[ch] + 1 <= kSmallRange) done += 1;
+                }
+                visited[i] = done;
+                take = done == 2;
             }
         }
-        wave_append(take, p, next, next_count);
+        wave_append(take, (int32_t)i, next, next_count);
     }
+}
+
+// One bottom-up round: complete every node of the list from its two complete children, then
+// count it at its parent; a parent counted twice is complete-able in the next round.
+__device__ inline void refit_step(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+                                  const int32_t* __restrict__ parent, const int32_t* __restrict__ escape, int64_t n,
+                                  uint32_t* __restrict__ visited, Node64* nodes, int32_t x, bool valid,
+                                  int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+    bool take = false;
+    int32_t p = 0;
+    if (valid && x > 0 && x < n) {
+        combine_node(leaf, lr, escape, n, nodes, x);
+        p = parent[x];
+        if (p > 0 && p < n) take = atomicAdd(&visited[p], 1u) == 1u;
+    }
+    wave_append(take, p, next, next_count);
 }
 
 __global__ void __launch_bounds__(kBlock) k_refit_round(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
@@ -607,20 +673,34 @@ __global__ void __launch_bounds__(kBlock) k_refit_round(const Leaf64* __restrict
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t k = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        bool take = false;
-        int32_t p = 0;
-        if (k < count) {
-            const int32_t x = cur[k];
-            p = (x > 0 && x < n) ? parent[x] : 0;
-            if (p > 0 && p < n) {
-                const uint32_t old = atomicAdd(&visited[p], 1u);
-                if (old == 1u) {
-                    combine_node(leaf, lr, escape, n, nodes, p);
-                    take = true;
-                }
-            }
+        refit_step(leaf, lr, parent, escape, n, visited, nodes, k < count ? cur[k] : 0, k < count, next, next_count);
+    }
+}
+
+// The remaining rounds (short lists near the root) in one workgroup: rounds are separated by
+// workgroup barriers instead of kernel launches.  Ends when a round appends nothing.
+constexpr int kTailBlock = 1024;
+constexpr int kMaxRounds = 4000;
+
+__global__ void __launch_bounds__(kTailBlock) k_refit_tail(const Leaf64* __restrict__ leaf,
+                                                           const int32_t* __restrict__ lr,
+                                                           const int32_t* __restrict__ parent,
+                                                           const int32_t* __restrict__ escape, int64_t n,
+                                                           uint32_t* __restrict__ visited, Node64* nodes,
+                                                           int32_t* listA, int32_t* listB, uint32_t* counts,
+                                                           int first_round) {
+    for (int r = first_round; r < kMaxRounds - 1; ++r) {
+        __syncthreads();
+        const int64_t count = __hip_atomic_load(&counts[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (count == 0) break;
+        const int32_t* cur = (r & 1) ? listB : listA;
+        int32_t* nxt = (r & 1) ? listA : listB;
+        const int64_t iters = (count + kTailBlock - 1) / kTailBlock;
+        for (int64_t it = 0; it < iters; ++it) {
+            const int64_t k = it * kTailBlock + threadIdx.x;
+            refit_step(leaf, lr, parent, escape, n, visited, nodes, k < count ? cur[k] : 0, k < count, nxt,
+                       &counts[r + 1]);
         }
-        wave_append(take, p, next, next_count);
     }
 }
 
@@ -662,37 +742,32 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
     return PMX_OK;
 }
 
-// Launch the refit rounds: the list of round r has at most n/2 entries, rounds continue
-// until a round appends nothing (the root was completed).  The tree height bounds the number
-// of rounds; batches of launches are issued before the host checks the last count.
+// Refit: short-range nodes directly, then kLaunchRounds grid-wide rounds (the long lists
+// near the leaves), then the tail kernel for the rest; no host synchronisation.
 int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     hipStream_t s = c->stream;
-    constexpr int kBatch = 48;
-    PMX_TRY(ensure(c, c->scratch_c, 4 * (size_t)(n + 2) + 4 * 4096));
+    constexpr int kLaunchRounds = 10;
+    PMX_TRY(ensure(c, c->scratch_c, 4 * (size_t)kMaxRounds + 64));
     int32_t* listA = c->keys_tmp.as<int32_t>();              // free after the sort (8n bytes)
     int32_t* listB = listA + (n / 2 + 64);
     uint32_t* counts = reinterpret_cast<uint32_t*>(c->scratch_c.p);
-    hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, 4096, 0u);
-    const unsigned gr = std::max(1u, std::min(g, 2048u));
-    hipLaunchKernelGGL(k_refit_first, dim3(gr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
-                       c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
-                       c->nodes64.as<Node64>(), listA, &counts[0]);
-    int round = 0;
-    for (;;) {
-        for (int b = 0; b < kBatch && round < 4000; ++b, ++round) {
-            int32_t* cur = (round & 1) ? listB : listA;
-            int32_t* nxt = (round & 1) ? listA : listB;
-            hipLaunchKernelGGL(k_refit_round, dim3(std::max(1u, std::min(g / 4, 1024u))), dim3(kBlock), 0, s,
-                               c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(), c->parent.as<int32_t>(),
-                               c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(), c->nodes64.as<Node64>(), cur,
-                               &counts[round], nxt, &counts[round + 1]);
-        }
-        uint32_t last = 0;
-        PMX_HIP(c, hipMemcpyAsync(&last, &counts[round], 4, hipMemcpyDeviceToHost, s));
-        PMX_HIP(c, hipStreamSynchronize(s));
-        if (last == 0) break;
-        if (round >= 4000) return set_error(c, PMX_ERR_HIP, "refit did not converge (tree deeper than 4000)");
+    hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, kMaxRounds, 0u);
+    const unsigned gr = std::max(1u, std::min(g, 4096u));
+    hipLaunchKernelGGL(k_refit_small, dim3(gr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
+                       c->escape.as<int32_t>(), c->range_lo.as<int32_t>(), c->range_hi.as<int32_t>(), n,
+                       c->visited.as<uint32_t>(), c->nodes64.as<Node64>(), listA, &counts[0]);
+    const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
+    for (int round = 0; round < kLaunchRounds; ++round) {
+        int32_t* cur = (round & 1) ? listB : listA;
+        int32_t* nxt = (round & 1) ? listA : listB;
+        hipLaunchKernelGGL(k_refit_round, dim3(grr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
+                           c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
+                           c->nodes64.as<Node64>(), cur, &counts[round], nxt, &counts[round + 1]);
     }
+    hipLaunchKernelGGL(k_refit_tail, dim3(1), dim3(kTailBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
+                       c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
+                       c->nodes64.as<Node64>(), listA, listB, counts, kLaunchRounds);
+    PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }
 
@@ -769,12 +844,17 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     ColPtrs cp;
     for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
     const unsigned g = grid_for(n, kBlock, 16384);
-    if (p->dtype == PMX_F64)
-        hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, order, n, cols64);
-    else
-        hipLaunchKernelGGL(k_gather<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, order, n, cols64);
     PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)n));
-    hipLaunchKernelGGL(k_pack_leaf, dim3(g), dim3(kBlock), 0, s, cols64, n, c->leaf64.as<Leaf64>());
+    PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
+    if (p->dtype == PMX_F64) {
+        hipLaunchKernelGGL(k_pack_records<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<double>());
+        hipLaunchKernelGGL(k_gather_records<double>, dim3(g), dim3(kBlock), 0, s, c->records.as<double>(), ncols,
+                           order, n, cols64, c->leaf64.as<Leaf64>());
+    } else {
+        hipLaunchKernelGGL(k_pack_records<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<float>());
+        hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
+                           order, n, cols64, c->leaf64.as<Leaf64>());
+    }
     PMX_HIP(c, hipEventRecord(c->ev[2], s));
     // ---- radix tree, escapes, refit ----
     PMX_TRY(ensure(c, c->lr, 8 * (size_t)(2 * n + 2)));

@@ -54,6 +54,7 @@ struct pmx_ctx {
     pmx::Tree tree;
     // kernel tables (device): Inorm per kernel, FP64 and FP32, and grid step dq
     pmx::DevBuf lut64;
+    pmx::DevBuf lut2;          // double2 (Inorm[i], Inorm[i+1] - Inorm[i]) per kernel
     double lut_dq[6] = {0, 0, 0, 0, 0, 0};
     // Morton-ordered particle data (device)
     pmx::DevBuf codes, order, keys_tmp, vals_tmp;
@@ -67,7 +68,8 @@ struct pmx_ctx {
     pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
     pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
-    pmx::DevBuf pinned;  // unused placeholder
+    pmx::DevBuf records;       // particle records (AoS) for the Morton-order permutation
+    pmx::DevBuf tl_st, tl_part, tl_cells;  // line-map binning (pmx_lines_tiled.hip)
     hipEvent_t ev[12] = {};
     double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -138,6 +140,13 @@ int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int3
 int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,
                       const double* o, const double* d, int64_t nl, float* out, int64_t ldo);
 int prepare_f32(pmx_ctx* c);
+// Parallel unit-direction line sets: particle-centric tiled path.  *used is false (and
+// nothing is written) when the lines do not qualify; the caller then uses the union walk.
+extern int g_lines_tiled;
+extern int g_bucket_points, g_bucket_lines, g_tiled_bucket;
+int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
+                       int64_t nl, float* out, int64_t ldo, bool* used);
+int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx);
 int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,
                           const double* ha, int64_t npts, int64_t* counts, const int64_t* offsets, int32_t* idx);

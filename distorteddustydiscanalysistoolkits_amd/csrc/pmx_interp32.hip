@@ -12,8 +12,8 @@
 //     so pruning is conservative: the accepted set is the FP64 predicate set.
 //   * Point weights, gradients and field terms are FP32 with compensated (TwoSum) sums per
 //     sample; derivative sums accumulate A_b - A_ref, formed in FP64 and rounded once, with
-//     A_ref one particle's value per tile, which leaves f - A~ b unchanged and removes the
-//     cancellation of large means.  Line maps test pairs in FP32; accepted pairs get their
+//     A_ref the sample's nearest particle (first accepted neighbour for scatter), which
+//     leaves f - A~ b unchanged and removes the cancellation of large means.  Line maps test pairs in FP32; accepted pairs get their
 //     perpendicular distance, weight and column sums in FP64 (the weight is a small share of
 //     the work), and all Shepard divisions are FP64.
 // Reference semantics: general_interpolation.jl:1-669 (points), line_integrated_scalar_
@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "pmx_internal.h"
+#include "pmx_tree32.h"
 
 namespace pmx {
 
@@ -35,6 +36,7 @@ constexpr int kWave = 64;
 constexpr int kMaxBucket = 64;
 constexpr float kEps = 4e-6f;     // guard band of FP32 distance / radius comparisons
 constexpr int kMaxF = 9;          // distinct field columns staged per leaf (points)
+constexpr int kMaxD = 7;          // distinct derivative columns staged per leaf (1 + 3 + 3)
 
 // neighbouring floats towards -inf / +inf (finite, non-NaN input)
 __device__ inline float next_down(float f) {
@@ -130,51 +132,55 @@ __global__ void __launch_bounds__(256) k_iota(uint64_t* __restrict__ keys, int32
 }
 
 // ---------------------------------------------------------------------------------------
-// FP32 kernel shapes (same piecewise polynomials as the FP64 path)
+// FP32 kernel shapes: f(q) and p(q) = f'(q) / q (kernels/M4_spline.jl .. C6_Wendland.jl).
+// p is written without the cancelling differences of the reference's derivative forms
+// (e.g. M4: -3/4 (2-q)^2 + 3 (1-q)^2 = q (-3 + 9/4 q)), so the FP32 gradient keeps its
+// relative accuracy for close neighbours, and grad W = sigma h^-5 p(q) Delta needs no 1/r.
 // ---------------------------------------------------------------------------------------
 template <int K>
-__device__ inline void shape_df32(float q, float& f, float& df) {
+__device__ inline void shape_fp32(float q, float e, float& f, float& p) {
+    // e = support - q, formed in FP64 by the caller: the edge factor (support - q) keeps its
+    // relative accuracy for neighbours near the support radius, where FP32 q would not
+    const float rq = q > 0.0f ? __builtin_amdgcn_rcpf(q) : 0.0f;
     if (K == kM4) {
-        const float a = 2.0f - q, b = 1.0f - q;
-        if (q < 1.0f) { f = 0.25f * (a * a * a) - b * b * b; df = -0.75f * (a * a) + 3.0f * (b * b); }
-        else if (q < 2.0f) { f = 0.25f * (a * a * a); df = -0.75f * (a * a); }
-        else { f = 0.0f; df = 0.0f; }
+        const float a = e, b = 1.0f - q;
+        if (q < 1.0f) { f = 0.25f * (a * a * a) - b * b * b; p = -3.0f + 2.25f * q; }
+        else if (q < 2.0f) { f = 0.25f * (a * a * a); p = (-0.75f * (a * a)) * rq; }
+        else { f = 0.0f; p = 0.0f; }
     } else if (K == kM5) {
-        const float a = 2.5f - q, b = 1.5f - q, c = 0.5f - q;
+        const float a = e, b = 1.5f - q, c = 0.5f - q;
         const float a3 = a * a * a, b3 = b * b * b, c3 = c * c * c;
-        if (q < 0.5f) { f = (a3 * a - 5.0f * (b3 * b)) + 10.0f * (c3 * c); df = (-4.0f * a3 + 20.0f * b3) - 40.0f * c3; }
-        else if (q < 1.5f) { f = a3 * a - 5.0f * (b3 * b); df = -4.0f * a3 + 20.0f * b3; }
-        else if (q < 2.5f) { f = a3 * a; df = -4.0f * a3; }
-        else { f = 0.0f; df = 0.0f; }
+        if (q < 0.5f) { f = (a3 * a - 5.0f * (b3 * b)) + 10.0f * (c3 * c); p = -30.0f + 24.0f * (q * q); }
+        else if (q < 1.5f) { f = a3 * a - 5.0f * (b3 * b); p = (-4.0f * a3 + 20.0f * b3) * rq; }
+        else if (q < 2.5f) { f = a3 * a; p = (-4.0f * a3) * rq; }
+        else { f = 0.0f; p = 0.0f; }
     } else if (K == kM6) {
-        const float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+        const float a = e, b = 2.0f - q, c = 1.0f - q;
         const float a4 = (a * a) * (a * a), b4 = (b * b) * (b * b), c4 = (c * c) * (c * c);
-        if (q < 1.0f) { f = (a4 * a - 6.0f * (b4 * b)) + 15.0f * (c4 * c); df = (-5.0f * a4 + 30.0f * b4) - 75.0f * c4; }
-        else if (q < 2.0f) { f = a4 * a - 6.0f * (b4 * b); df = -5.0f * a4 + 30.0f * b4; }
-        else if (q < 3.0f) { f = a4 * a; df = -5.0f * a4; }
-        else { f = 0.0f; df = 0.0f; }
+        if (q < 1.0f) { f = (a4 * a - 6.0f * (b4 * b)) + 15.0f * (c4 * c); p = -120.0f + (q * q) * (120.0f - 50.0f * q); }
+        else if (q < 2.0f) { f = a4 * a - 6.0f * (b4 * b); p = (-5.0f * a4 + 30.0f * b4) * rq; }
+        else if (q < 3.0f) { f = a4 * a; p = (-5.0f * a4) * rq; }
+        else { f = 0.0f; p = 0.0f; }
     } else if (K == kC2) {
         if (q < 2.0f) {
-            const float u = 1.0f - 0.5f * q, u3 = u * u * u;
+            const float u = 0.5f * e, u3 = u * u * u;
             f = (u3 * u) * (2.0f * q + 1.0f);
-            df = (u3 * u) * 2.0f - u3 * (4.0f * q + 2.0f);
-        } else { f = 0.0f; df = 0.0f; }
+            p = -5.0f * u3;
+        } else { f = 0.0f; p = 0.0f; }
     } else if (K == kC4) {
         if (q < 2.0f) {
-            const float u = 1.0f - 0.5f * q, u5 = (u * u) * (u * u) * u;
+            const float u = 0.5f * e, u5 = (u * u) * (u * u) * u;
             f = (u5 * u) * (((35.0f / 12.0f) * (q * q) + 3.0f * q) + 1.0f);
-            df = (u5 * u) * ((35.0f / 6.0f) * q + 3.0f) - u5 * (((35.0f / 4.0f) * (q * q) + 9.0f * q) + 3.0f);
-        } else { f = 0.0f; df = 0.0f; }
+            p = (-14.0f / 3.0f) * u5 * (1.0f + 2.5f * q);
+        } else { f = 0.0f; p = 0.0f; }
     } else {
         if (q < 2.0f) {
-            const float u = 1.0f - 0.5f * q, u7 = (u * u) * (u * u) * (u * u) * u;
+            const float u = 0.5f * e, u7 = (u * u) * (u * u) * (u * u) * u;
             f = (u7 * u) * (((4.0f * (q * q * q) + 6.25f * (q * q)) + 4.0f * q) + 1.0f);
-            df = (u7 * u) * ((12.0f * (q * q) + 12.5f * q) + 4.0f) -
-                 u7 * (((16.0f * (q * q * q) + 25.0f * (q * q)) + 16.0f * q) + 4.0f);
-        } else { f = 0.0f; df = 0.0f; }
+            p = -5.5f * u7 * ((4.0f * q + 3.5f) * q + 1.0f);
+        } else { f = 0.0f; p = 0.0f; }
     }
 }
-
 
 // Compensated accumulator (TwoSum; -ffp-contract=off keeps it exact).
 struct CSum {
@@ -187,56 +193,6 @@ struct CSum {
     }
     __device__ inline double value() const { return (double)s + (double)c; }
 };
-
-struct T32 {
-    const Node32* __restrict__ nodes;
-    const float4* __restrict__ leaf;
-    const float4* __restrict__ leaflo;
-    const int32_t* __restrict__ leaf_esc;
-    const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant
-    const float* __restrict__ pay;         // m/rho | quant... | rho
-    const double* __restrict__ w64;        // m/rho in FP64
-    int64_t n;
-    int32_t root;
-    int32_t bucket;
-};
-
-__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
-__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Union walk of the tile over left / escape links.  need(nd) is each lane's conservative
-// node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
-template <class Need, class Bucket>
-__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket) {
-    const int64_t n = T.n;
-    if (!any_valid) return;
-    if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
-        bucket(0, (int)n);
-        return;
-    }
-    int node = T.root;
-    int steps = 0;
-    while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
-        if (node >= n) {
-            const int li = node - (int)n;
-            bucket(li, 1);
-            node = uniform(T.leaf_esc[li]);
-            continue;
-        }
-        const Node32 nd = T.nodes[node];
-        if (!ballot(need(nd))) {
-            node = uniform(nd.escape);
-            continue;
-        }
-        const int lo = uniform(nd.lo), cnt = uniform(nd.cnt);
-        if (cnt <= T.bucket && cnt >= 1 && lo >= 0 && lo + cnt <= n) {
-            bucket(lo, cnt);
-            node = uniform(nd.escape);
-            continue;
-        }
-        node = uniform(nd.left);
-    }
-}
 
 // ---------------------------------------------------------------------------------------
 // Points
@@ -360,10 +316,10 @@ __device__ inline bool node_need_point(const PointLane& q, const Node32& nd, flo
     return q.valid && !(d2_box32(q, nd) > r2 * (1.0f + kEps));
 }
 
-// Scalars are staged from the FP32 payload; the fields under a derivative are staged as
-// FP32 differences A_b - A_ref computed in FP64 from the FP64 columns, with A_ref one
-// particle's value per tile (the shift leaves every derivative estimate unchanged and keeps
-// the FP32 rounding relative to the local variation instead of the field's magnitude).
+// Scalars are staged from the FP32 payload; the fields under a derivative are staged in FP64
+// and each sample rounds A_b - A_ref to FP32 with its own FP64 reference A_ref (the shift
+// leaves every derivative estimate unchanged and keeps the FP32 rounding relative to the
+// local variation instead of the field's magnitude).
 struct PointCat {
     int ns, ng, nd, nc;
     int scol[2], sh[2];   // payload column of each scalar (1.. = quant, 1+ncol = rho)
@@ -377,8 +333,9 @@ struct PointCat {
 
 struct BPay {
     float w[kMaxBucket];
+    double ih64[kMaxBucket];   // 1 / h in FP64
     float s[2][kMaxBucket];
-    float d[kMaxF][kMaxBucket];
+    double d[kMaxD][kMaxBucket];
 };
 
 template <int K, int S, bool DERIV>
@@ -401,7 +358,7 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
         const int64_t qi = valid ? perm[slot] : 0;
         const PointLane q = point_lane(qx, qy, qz, valid, qi);
 
-        double radius2_64 = 0.0;
+        double radius2_64 = 0.0, invha64 = 0.0;
         float radius2_32 = 0.0f, invha = 0.0f;
         int64_t nearest = -1;
         if (S != kScatter) {
@@ -410,29 +367,37 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
             const double radius = Kv * ha64;
             radius2_64 = radius * radius;
             radius2_32 = (float)radius2_64;
-            invha = (float)(1.0 / ha64);
+            invha64 = 1.0 / ha64;
+            invha = (float)invha64;
         }
         const float invha3 = invha * invha * invha;
-        const float invha4 = (invha * invha) * (invha * invha);
+        const float invha5 = (invha * invha) * (invha * invha) * invha;
 
         CSum S1{0, 0}, sc[2] = {}, b[3] = {}, gf[3] = {}, gs{0, 0}, df{0, 0}, ds[3] = {}, cf[3] = {}, cs[3] = {};
-        double ref64[kMaxF];
+        // per-sample FP64 reference values of the derivative fields: the nearest particle
+        // (gather / symmetric) or the first accepted neighbour (scatter)
+        double refg = 0.0, refd[3] = {0.0, 0.0, 0.0}, refc[3] = {0.0, 0.0, 0.0};
         bool have_ref = false;
+        if (DERIV && nearest >= 0) {
+            if (cat.ng) refg = T.cols64[(int64_t)cat.dcol[cat.gf] * n + nearest];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (cat.nd) refd[k] = T.cols64[(int64_t)cat.dcol[cat.df[k]] * n + nearest];
+                if (cat.nc) refc[k] = T.cols64[(int64_t)cat.dcol[cat.cf[k]] * n + nearest];
+            }
+            have_ref = true;
+        }
 
         auto need = [&](const Node32& nd) { return node_need_point<S>(q, nd, Kvf, radius2_32); };
         auto bucket = [&](int lo, int cnt) {
-            if (DERIV && !have_ref) {      // tile reference: first leaf of the first bucket (uniform)
-                for (int f = 0; f < cat.nder; ++f) ref64[f] = T.cols64[(int64_t)cat.dcol[f] * n + lo];
-                have_ref = true;
-            }
             stage_pos(T, P, lo, cnt);
             if (lane < cnt) {
                 const int64_t li = lo + lane;
                 Y.w[lane] = T.pay[li];
+                Y.ih64[lane] = 1.0 / T.cols64[4 * n + li];
                 for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
                 if (DERIV)
-                    for (int f = 0; f < cat.nder; ++f)
-                        Y.d[f][lane] = (float)(T.cols64[(int64_t)cat.dcol[f] * n + li] - ref64[f]);
+                    for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
             }
             __syncthreads();
             for (int j = 0; j < cnt; ++j) {
@@ -440,22 +405,27 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                 const float d2 = d2_pos32(q, P, j, dx, dy, dz);
                 if (!accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64)) continue;
                 const float invh = P.invh[j];
-                const float rinv = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
-                const float r = d2 * rinv;
-                float W, G;
+                // separation in FP64 from the hi / lo parts (both differences exact in FP64)
+                const double ex = ((double)q.ph[0] - (double)P.x[j]) + ((double)q.pl[0] - (double)P.xl[j]);
+                const double ey = ((double)q.ph[1] - (double)P.y[j]) + ((double)q.pl[1] - (double)P.yl[j]);
+                const double ez = ((double)q.ph[2] - (double)P.z[j]) + ((double)q.pl[2] - (double)P.zl[j]);
+                const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+                float W, G;       // W and sigma h^-5 f'(q)/q
                 {
-                    float fb, dfb;
-                    shape_df32<K>(r * invh, fb, dfb);
+                    const double qb = rd * Y.ih64[j];
+                    float fb, pb;
+                    shape_fp32<K>((float)qb, (float)(Kv - qb), fb, pb);
                     const float ib2 = invh * invh;
                     const float Wb = (ib2 * invh) * (sig * fb);
-                    const float Gb = (ib2 * ib2) * (sig * dfb);
+                    const float Gb = ((ib2 * ib2) * invh) * (sig * pb);
                     if (S == kScatter) {
                         W = Wb; G = Gb;
                     } else {
-                        float fa, dfa;
-                        shape_df32<K>(r * invha, fa, dfa);
+                        const double qa = rd * invha64;
+                        float fa, pa;
+                        shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
                         const float Wa = invha3 * (sig * fa);
-                        const float Ga = invha4 * (sig * dfa);
+                        const float Ga = invha5 * (sig * pa);
                         if (S == kGather) { W = Wa; G = Ga; }
                         else { W = 0.5f * (Wa + Wb); G = 0.5f * (Ga + Gb); }
                     }
@@ -466,22 +436,33 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                 if (cat.ns > 0) sc[0].add(Y.s[0][j] * w);
                 if (cat.ns > 1) sc[1].add(Y.s[1][j] * w);
                 if (!DERIV) continue;
-                // (m/rho) grad W = (m/rho) f'(q) sigma h^-4 Delta / r  (zero at r == 0)
-                const float cg = wb * G * rinv;
+                if (!have_ref) {
+                    if (cat.ng) refg = Y.d[cat.gf][j];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (cat.nd) refd[k] = Y.d[cat.df[k]][j];
+                        if (cat.nc) refc[k] = Y.d[cat.cf[k]][j];
+                    }
+                    have_ref = true;
+                }
+                // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
+                const float cg = wb * G;
                 const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
                 b[0].add(gx); b[1].add(gy); b[2].add(gz);
                 if (cat.ng) {
-                    const float A = Y.d[cat.gf][j];
+                    const float A = (float)(Y.d[cat.gf][j] - refg);
                     gf[0].add(A * gx); gf[1].add(A * gy); gf[2].add(A * gz);
                     gs.add(A * w);
                 }
                 if (cat.nd) {
-                    const float Ax = Y.d[cat.df[0]][j], Ay = Y.d[cat.df[1]][j], Az = Y.d[cat.df[2]][j];
+                    const float Ax = (float)(Y.d[cat.df[0]][j] - refd[0]), Ay = (float)(Y.d[cat.df[1]][j] - refd[1]),
+                                Az = (float)(Y.d[cat.df[2]][j] - refd[2]);
                     df.add(Ax * gx + Ay * gy + Az * gz);
                     ds[0].add(Ax * w); ds[1].add(Ay * w); ds[2].add(Az * w);
                 }
                 if (cat.nc) {
-                    const float Ax = Y.d[cat.cf[0]][j], Ay = Y.d[cat.cf[1]][j], Az = Y.d[cat.cf[2]][j];
+                    const float Ax = (float)(Y.d[cat.cf[0]][j] - refc[0]), Ay = (float)(Y.d[cat.cf[1]][j] - refc[1]),
+                                Az = (float)(Y.d[cat.cf[2]][j] - refc[2]);
                     cf[0].add(Ay * gz - Az * gy);
                     cf[1].add(Az * gx - Ax * gz);
                     cf[2].add(Ax * gy - Ay * gx);
@@ -808,22 +789,6 @@ __global__ void __launch_bounds__(kWave) k_lines32_nb(T32 T, double Kv, const do
     }
 }
 
-T32 tree32(pmx_ctx* c, int bucket) {
-    T32 T;
-    T.nodes = c->nodes32.as<Node32>();
-    T.leaf = c->leaf32.as<float4>();
-    T.leaflo = c->leaf32.as<float4>() + c->tree.n;
-    T.leaf_esc = c->leaf_esc.as<int32_t>();
-    T.cols64 = c->cols64.as<double>();
-    T.pay = c->pay32.as<float>();
-    T.w64 = reinterpret_cast<const double*>(c->pay32.as<float>() + (size_t)c->tree.n * (2 + c->tree.ncol) +
-                                            ((size_t)c->tree.n * (2 + c->tree.ncol) & 1));
-    T.n = c->tree.n;
-    T.root = c->tree.root;
-    T.bucket = std::max(1, std::min(bucket, kMaxBucket));
-    return T;
-}
-
 unsigned tile_grid(int64_t nq) {
     const int64_t ntiles = (nq + kWave - 1) / kWave;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
@@ -858,6 +823,22 @@ void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const dou
 }
 
 }  // namespace
+
+T32 tree32(pmx_ctx* c, int bucket) {
+    T32 T;
+    const size_t payf = (size_t)c->tree.n * (2 + c->tree.ncol);
+    T.nodes = c->nodes32.as<Node32>();
+    T.leaf = c->leaf32.as<float4>();
+    T.leaflo = c->leaf32.as<float4>() + c->tree.n;
+    T.leaf_esc = c->leaf_esc.as<int32_t>();
+    T.cols64 = c->cols64.as<double>();
+    T.pay = c->pay32.as<float>();
+    T.w64 = reinterpret_cast<const double*>(c->pay32.as<float>() + payf + (payf & 1));
+    T.n = c->tree.n;
+    T.root = c->tree.root;
+    T.bucket = std::max(1, std::min(bucket, kMaxBucket));
+    return T;
+}
 
 // Morton order of the samples (queries): permutation in c->scratch_a or c->scratch_b
 int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm) {

@@ -1,0 +1,65 @@
+// pmx_tree32.h — FP32 view of the LBVH and the wave-uniform walk shared by the
+// FP32-compensated kernels (sample tiles and line-map tiles).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pmx_internal.h"
+
+namespace pmx {
+
+struct T32 {
+    const Node32* __restrict__ nodes;
+    const float4* __restrict__ leaf;
+    const float4* __restrict__ leaflo;
+    const int32_t* __restrict__ leaf_esc;
+    const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant
+    const float* __restrict__ pay;         // m/rho | quant... | rho
+    const double* __restrict__ w64;        // m/rho in FP64
+    int64_t n;
+    int32_t root;
+    int32_t bucket;
+};
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Union walk of the tile over left / escape links.  need(nd) is each lane's conservative
+// node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
+template <class Need, class Bucket>
+__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket) {
+    const int64_t n = T.n;
+    if (!any_valid) return;
+    if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
+        bucket(0, (int)n);
+        return;
+    }
+    int node = T.root;
+    int steps = 0;
+    while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
+        if (node >= n) {
+            const int li = node - (int)n;
+            bucket(li, 1);
+            node = uniform(T.leaf_esc[li]);
+            continue;
+        }
+        const Node32 nd = T.nodes[node];
+        if (!ballot(need(nd))) {
+            node = uniform(nd.escape);
+            continue;
+        }
+        const int lo = uniform(nd.lo), cnt = uniform(nd.cnt);
+        if (cnt <= T.bucket && cnt >= 1 && lo >= 0 && lo + cnt <= n) {
+            bucket(lo, cnt);
+            node = uniform(nd.escape);
+            continue;
+        }
+        node = uniform(nd.left);
+    }
+}
+
+// FP32 tree view of the context's current LBVH (prepare_f32 must have run); `bucket` is the
+// largest leaf run a walk hands over in one piece (clamped to 1..64).
+T32 tree32(pmx_ctx* c, int bucket);
+
+}  // namespace pmx

@@ -187,6 +187,15 @@ int pmx_point_traversal_stats(struct pmx_ctx *ctx, int32_t strategy, const pmx_p
                               int64_t *stats);
 int pmx_line_traversal_stats(struct pmx_ctx *ctx, const pmx_lines *lines, int64_t *stats);
 
+/* Neighbour-set digest of the tiled line-map path (parallel unit-direction lines, the path
+ * pmx_interpolate_lines takes for PMX_PREC_F32C on such lines): for each line i (host
+ * arrays of 3 n uint64) digest[i] = accepted count, digest[n + i] = sum of accepted sorted
+ * particle indices, digest[2n + i] = sum of their squares (mod 2^64).  *used = 0 (digest
+ * untouched) when the lines do not qualify.  No reference counterpart: a check that the
+ * tiled path accepts exactly the reference's pairs (LineSamples neighbour sets,
+ * line_integrated_scalar_interpolation.jl:123-166) at sizes where full lists are too big. */
+int pmx_line_neighbor_digest(struct pmx_ctx *ctx, const pmx_lines *lines, uint64_t *digest, int32_t *used);
+
 /* --- kernel functions (host-callable, evaluated on the device) --------------------- */
 /* f(q), f'(q) of a kernel on the device in FP64 (kernels/{M4..C6}.jl, (::K)(q) and (::_dK)(q)). */
 int pmx_kernel_eval(struct pmx_ctx *ctx, int32_t kernel, int64_t n, const double *q,

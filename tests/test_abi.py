@@ -31,6 +31,16 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.EXPORTS) <= set(syms)
 
 
+def test_binding_declares_every_signature():
+    """Every declared entry point is bound with explicit argument types: an undeclared
+    ctypes signature passes 64-bit host pointers as C int and truncates them."""
+    lib = _lib.lib()
+    assert set(declared_symbols()) == set(_lib.EXPORTS)
+    no_args = {"pmx_version"}
+    missing = [s for s in _lib.EXPORTS if s not in no_args and getattr(lib, s).argtypes is None]
+    assert not missing, missing
+
+
 def test_version_and_error_without_context():
     lib = _lib.lib()
     assert b"gfx950" in lib.pmx_version()

@@ -209,3 +209,72 @@ def test_f32c_lines_fields(built):
         linf, l2 = rel_norms(g, r)
         print(f"[f32c lines] rel Linf {linf:.3e} L2 {l2:.3e}")
         assert linf <= F32C_TOL and l2 <= F32C_TOL
+
+
+# ---------------------------------------------------------------------------------------
+# Tiled line-map path (parallel unit-direction lines): accepted pairs digested per line
+# (count, index sum, index-square sum) against the oracle's neighbour lists, and fields.
+# ---------------------------------------------------------------------------------------
+def _tilted_lines(m, direction, seed=7):
+    d = np.asarray(direction, np.float64)
+    d = d / np.sqrt(d @ d)
+    rng = np.random.default_rng(seed)
+    o = [rng.uniform(-60.0, 60.0, m * m), rng.uniform(-60.0, 60.0, m * m), rng.uniform(-3.0, 3.0, m * m)]
+    return o, [np.full(m * m, d[0]), np.full(m * m, d[1]), np.full(m * m, d[2])]
+
+
+def _digest_from_csr(off, idx):
+    n = len(off) - 1
+    cnt = np.diff(off).astype(np.uint64)
+    ii = idx.astype(np.uint64)
+    seg = np.repeat(np.arange(n), np.diff(off))
+    s1 = np.zeros(n, np.uint64)
+    s2 = np.zeros(n, np.uint64)
+    np.add.at(s1, seg, ii)
+    np.add.at(s2, seg, ii * ii)
+    return cnt, s1, s2
+
+
+@pytest.mark.parametrize("geometry", ["faceon", "tilted", "edgeon"])
+def test_tiled_line_digest_matches_oracle(built, geometry):
+    if geometry == "faceon":
+        o, d = disc.faceon_lines(40)
+    elif geometry == "tilted":
+        o, d = _tilted_lines(32, (0.3, -0.2, 0.9))
+    else:
+        o, d = _tilted_lines(32, (0.0, 1.0, 0.0))
+    dg = built["ctx"].line_digest(o, d)
+    assert dg is not None, "parallel lines must take the tiled path"
+    roff, ridx = built["tree"].line_neighbors("M4", "scatter", o, d)
+    ref = _digest_from_csr(roff, ridx)
+    for g, r in zip(dg, ref):
+        assert np.array_equal(g, r)
+    assert ref[0].sum() > 0
+
+
+def test_tiled_line_digest_declines_nonparallel(built):
+    o, d = _tilted_lines(8, (0.0, 0.0, 1.0))
+    d[0] = d[0].copy()
+    d[0][3] = 1e-3
+    assert built["ctx"].line_digest(o, d) is None
+
+
+@pytest.mark.parametrize("geometry", ["tilted", "nonparallel"])
+def test_f32c_lines_other_geometries(built, geometry):
+    if geometry == "tilted":
+        o, d = _tilted_lines(24, (0.3, -0.2, 0.9))
+    else:
+        o, d = _tilted_lines(24, (0.0, 0.0, 1.0))
+        rng = np.random.default_rng(5)
+        tilt = rng.uniform(-0.2, 0.2, (2, o[0].size))
+        nrm = np.sqrt(1.0 + tilt[0] ** 2 + tilt[1] ** 2)
+        d = [tilt[0] / nrm, tilt[1] / nrm, 1.0 / nrm]
+    cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
+    outs = [np.empty(o[0].size) for _ in range(2)]
+    built["ctx"].interpolate_lines(_lib.PMX_SCATTER, _lib.PMX_PREC_F32C, cat, o, d, outs)
+    ref = built["tree"].line_samples("M4", "scatter", [0, 3], [0, 1], o, d)
+    for g, r in zip(outs, ref):
+        assert np.array_equal(np.isnan(g), np.isnan(r))
+        linf, l2 = rel_norms(g, r)
+        print(f"[f32c lines {geometry}] rel Linf {linf:.3e} L2 {l2:.3e}")
+        assert linf <= F32C_TOL and l2 <= F32C_TOL
