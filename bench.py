@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--grid", default="256,256,64", help="points workload: ns,nphi,nz")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-samples", type=int, default=16384, help="samples in the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -129,6 +131,13 @@ def main():
     from distorteddustydiscanalysistoolkits_amd import _lib
 
     dev = torch.device("cuda:0")
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
     w = Workload(a, torch, dev)
     ctx = _lib.Context(0)
 
@@ -176,33 +185,30 @@ def main():
     main_key = "f32c" if "f32c" in results else "f64"
     r = results[main_key]
 
-    # Algorithmic bytes of the interpolation kernels per launch: traversal records (FP32 node
-    # 36 B, leaf 20 B; FP64 node 64 B, leaf 36 B), one particle record per accepted pair
-    # ((6 + k) columns x 4 B, x 8 B on the FP64 path), sample input and outputs.
+    # Algorithmic bytes of the interpolation kernels per launch (pair-stream model): one
+    # particle record of (6 + k) FP32 columns per accepted (sample, particle) pair, 8 B columns
+    # on the FP64 path.  The per-sample traversal model (FP32 node 36 B, leaf 20 B per visit,
+    # FP64 64 B / 36 B) is reported beside it; HBM traffic is the measured FETCH + WRITE bytes
+    # per launch (profiles/pmc_traffic.json) when present.
     st = w.stats(ctx)
     n_int, n_leaf, n_acc = int(st[2]), int(st[3]), int(st[4])
     n_near_int, n_near_leaf = int(st[0]), int(st[1])
     k = len(w.qnames)
-    if w.kind == "lines":
-        sample_bytes = 48
-    else:
-        sample_bytes = 24
     if main_key == "f64":
-        rec, node, leaf, outb = 8 * (6 + k), 64, 36, 8
+        rec, node, leaf = 8 * (6 + k), 64, 36
     else:
-        rec, node, leaf, outb = 4 * (6 + k), 36, 20, 4
+        rec, node, leaf = 4 * (6 + k), 36, 20
     trav_bytes = node * (n_int + n_near_int) + leaf * (n_leaf + n_near_leaf)
     pair_bytes = rec * n_acc
-    bytes_launch = trav_bytes + pair_bytes + w.P * (sample_bytes + outb * w.L)
     t_kernel = r["kernel_ms"] * 1e-3
-    achieved = bytes_launch / t_kernel / 1e9
+    achieved = pair_bytes / t_kernel / 1e9
     flops_pair = 103 if w.kind == "points" else 30
     flops = flops_pair * n_acc
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"{w.kind}_{main_key}")
+            traffic = json.load(open(tfile)).get(f"{w.kind}_{main_key}", {}).get("bytes")
         except Exception:
             traffic = None
 
@@ -249,9 +255,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"{w.kind} interpolation ({main_key})",
-                     "algorithmic_bytes_per_launch": bytes_launch,
-                     "pair_stream_GBs": pair_bytes / t_kernel / 1e9,
-                     "pair_stream_frac": pair_bytes / t_kernel / 1e9 / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_launch": pair_bytes, "bytes_per_unit": rec, "units": n_acc,
+                     "unit_name": "accepted (sample, particle) pairs",
+                     "measured_GBs": (traffic / t_kernel / 1e9) if traffic else None,
+                     "traversal_model_bytes": trav_bytes,
                      "valu_TFLOPs": flops / t_kernel / 1e12,
                      "valu_frac": flops / t_kernel / 1e12 / FP32_PEAK_TFLOPS},
         "cpu_baseline": cpu,

@@ -1,5 +1,5 @@
 # Partia package extension binding libpartia_mi355x (include/partia_mi355x.h) with ccall.
-# Activated by the weak dependency that ships the shared library (see INTEGRATION.md).
+# Activated by the weak dependency PartiaMI355X_jll, which ships the shared library.
 # Adds MI355XComputeBackend methods to Partia.PointSamples_interpolation and
 # Partia.LineSamples_interpolation, the two methods the CUDA extension adds
 # (Partia.jl/ext/CUDAExt/KernelInterpolation/grid_interpolation/PointSamples_interpolation.jl:12,

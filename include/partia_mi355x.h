@@ -8,7 +8,7 @@
  * and LineSamples_interpolation.jl:11) and the CPU LBVH build they call
  * (Partia.jl/src/julia/KernelInterpolation/grid_interpolation/initialize_interpolation.jl:49,
  * LinearBVH! at Partia.jl/src/julia/KernelInterpolation/interpolation_setup/InterpolationInput.jl:203).
- * Julia binds these symbols with `ccall`; the Python package binds
+ * Julia binds these symbols with `ccall` (julia/MI355XExt.jl of the package); the Python package binds
  * them with ctypes.
  *
  * Conventions

@@ -108,8 +108,8 @@ def test_line_table_against_reference_table(k):
     nm = NAMES[k]
     assert np.array_equal(Q[idx], f[nm + "_Q"])
     ref = f[nm + "_Inorm"]
-    # the reference's table carries its own quadrature error (up to 5e-11 relative, see
-    # the mpmath comparison in DESIGN.md); the regenerated table must agree to that level
+    # the reference's table carries its own quadrature error (up to 5e-11 relative against a
+    # high-precision quadrature of the same integral); the regenerated table must agree to that level
     assert np.all(np.abs(In[idx] - ref) <= 1e-10 * np.abs(ref).max())
     mask = ref > 1e-6 * ref.max()
     assert np.max(np.abs(In[idx][mask] / ref[mask] - 1)) < 1e-10
