@@ -219,12 +219,14 @@ const char* pmx_version(void) { return "partia_mi355x 0.1 (gfx950)"; }
 // Launch-shape knobs (environment, read at context creation): PMX_LINES_TILED=0 sends every
 // line set through the union walk; PMX_BUCKET_POINTS / PMX_BUCKET_LINES set the leaf-run
 // size the sample-tile walks hand over at once (1..64); PMX_TILED_BUCKET the leaf-run size of
-// the line-map tile walk (1..65536).
+// the line-map tile walk (1..65536); PMX_TILED_NEAR how close (in cells) a node's box must be to
+// the tile for the walk to read it whole.
 static void read_tuning() {
     if (const char* v = std::getenv("PMX_LINES_TILED")) g_lines_tiled = std::atoi(v) != 0;
     if (const char* v = std::getenv("PMX_BUCKET_POINTS")) g_bucket_points = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_BUCKET_LINES")) g_bucket_lines = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_BUCKET")) g_tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_TILED_NEAR")) g_tiled_near = std::max(0, std::min(1 << 20, std::atoi(v)));
 }
 
 int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
@@ -248,16 +250,11 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
         if (hipEventCreate(&e) != hipSuccess) { delete c; return PMX_ERR_HIP; }
     std::vector<double> h64;
     upload_tables(c, h64);
-    std::vector<double> h2(2 * h64.size());   // (Inorm[i], Inorm[i+1] - Inorm[i])
-    for (int k = 0; k < 6; ++k)
-        for (int i = 0; i < kLutN; ++i) {
-            const double a = h64[k * kLutN + i], b = i + 1 < kLutN ? h64[k * kLutN + i + 1] : a;
-            h2[2 * (k * kLutN + i)] = a;
-            h2[2 * (k * kLutN + i) + 1] = b - a;
-        }
-    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK || ensure(c, c->lut2, 8 * h2.size()) != PMX_OK ||
+    std::vector<float> hf(h64.size());
+    for (size_t i = 0; i < h64.size(); ++i) hf[i] = (float)h64[i];
+    if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK || ensure(c, c->lutf, 4 * hf.size()) != PMX_OK ||
         hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->lut2.p, h2.data(), 8 * h2.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->lutf.p, hf.data(), 4 * hf.size(), hipMemcpyHostToDevice) != hipSuccess) {
         pmx_destroy(c);
         return PMX_ERR_HIP;
     }
@@ -269,7 +266,7 @@ void pmx_destroy(pmx_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->lut64, &c->lut2, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
+    DevBuf* bufs[] = {&c->lut64, &c->lutf, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,

@@ -54,7 +54,7 @@ struct pmx_ctx {
     pmx::Tree tree;
     // kernel tables (device): Inorm per kernel, FP64 and FP32, and grid step dq
     pmx::DevBuf lut64;
-    pmx::DevBuf lut2;          // double2 (Inorm[i], Inorm[i+1] - Inorm[i]) per kernel
+    pmx::DevBuf lutf;          // Inorm per kernel rounded to FP32 (16 KB each: stays in L1)
     double lut_dq[6] = {0, 0, 0, 0, 0, 0};
     // Morton-ordered particle data (device)
     pmx::DevBuf codes, order, keys_tmp, vals_tmp;
@@ -143,7 +143,7 @@ int prepare_f32(pmx_ctx* c);
 // Parallel unit-direction line sets: particle-centric tiled path.  *used is false (and
 // nothing is written) when the lines do not qualify; the caller then uses the union walk.
 extern int g_lines_tiled;
-extern int g_bucket_points, g_bucket_lines, g_tiled_bucket;
+extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near;
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used);
 int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);

@@ -35,7 +35,7 @@ namespace {
 
 constexpr int kTS = 8;                 // cells per tile side
 constexpr int kCells = kTS * kTS;      // one wavefront per tile: lane = cell when loading
-constexpr int kCap = 128;              // lines per LDS chunk of a tile
+constexpr int kCap = 64;               // lines per LDS chunk of a tile (a lattice tile holds 64)
 constexpr int kMaxCols = 8;            // field columns per launch
 
 struct Frame {
@@ -44,6 +44,7 @@ struct Frame {
     int gx, gy, tx, ty;                // cells and tiles along s and t
     double band;                       // absolute band on d^2 around (K h)^2 -> exact re-check
     double pad;                        // absolute padding of footprint radii
+    double near;                       // cells: nodes whose box is this close are read whole
 };
 
 // (s, t) of each line origin, parallel check against line 0, per-block bounds.
@@ -94,31 +95,21 @@ __global__ void __launch_bounds__(256) k_line_keys(const double* __restrict__ st
     }
 }
 
-// cell -> [lo, hi) run of sorted lines (cells zero-filled beforehand), sorted (s, t)
+// CSR start of every cell over the sorted lines: start[c] = first sorted position with key >= c
+// (empty cells get the start of the next non-empty one; start[ncell] = nl), and the sorted
+// (s, t) of each line.
 __global__ void __launch_bounds__(256) k_line_cells(const uint64_t* __restrict__ keys, const int32_t* __restrict__ perm,
-                                                    const double* __restrict__ st, int64_t nl,
-                                                    int2* __restrict__ cells, double2* __restrict__ st_sorted) {
+                                                    const double* __restrict__ st, int64_t nl, int64_t ncell,
+                                                    int32_t* __restrict__ start, double2* __restrict__ st_sorted) {
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < nl; j += (int64_t)gridDim.x * 256) {
-        const uint64_t k = keys[j];
-        if (j == 0 || keys[j - 1] != k) cells[k].x = (int)j;
-        if (j == nl - 1 || keys[j + 1] != k) cells[k].y = (int)(j + 1);
+        const int64_t k = (int64_t)keys[j];
+        const int64_t kp = j == 0 ? -1 : (int64_t)keys[j - 1];
+        for (int64_t c = kp + 1; c <= k && c <= ncell; ++c) start[c] = (int32_t)j;
+        if (j == nl - 1)
+            for (int64_t c = k + 1; c <= ncell; ++c) start[c] = (int32_t)nl;
         const int32_t i = perm[j];
         st_sorted[j] = make_double2(st[i], st[nl + i]);
     }
-}
-
-__global__ void __launch_bounds__(256) k_zero_int2(int2* __restrict__ p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        p[i] = make_int2(0, 0);
-}
-
-__device__ inline int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ inline int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
 }
 
 struct TileCat {
@@ -143,38 +134,37 @@ __device__ inline void project_box(const Node32& b, const double a[3], double& l
 // MODE 0: interpolation (FP64 weight sums).  MODE 1: neighbour digest per line (count, sum
 // of sorted particle indices, sum of their squares; unsigned 64-bit, wrapping).
 template <int K, int MODE>
-__global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat, const int2* __restrict__ cells,
+__global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat, const int32_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ perm,
                                                     const double2* __restrict__ st_sorted,
                                                     const double* __restrict__ o, int64_t nl,
-                                                    const double2* __restrict__ lut, double inv_dq,
+                                                    const float* __restrict__ lut, double inv_dq,
                                                     float* __restrict__ out, int64_t ldo,
                                                     uint64_t* __restrict__ digest) {
+    // dynamic LDS: line s, t [kCap]; accumulators [nacc][kCap]; particle fields [ncol][64]
     extern __shared__ double dyn[];
-    __shared__ int clo[kCells], chi[kCells];
-    // sweep state: particle data per lane and the owner lane of each (particle, cell) item
-    __shared__ double w_sb[64], w_tb[64], w_r2[64], w_wb[64];
-    __shared__ double w_ih[64];
-    __shared__ int w_p[64], w_cx0[64], w_cy0[64], w_ncx[64], w_start[64];
-    __shared__ float w_A[kMaxCols][64];
-    __shared__ unsigned char w_own[64 * kCells];
-    double* S = dyn;                       // [kCap] line s
-    double* Tt = dyn + kCap;               // [kCap] line t
-    double* acc = dyn + 2 * kCap;          // [(1 + ncol)][kCap] FP64 sums (MODE 0)
-    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);  // [3][kCap] (MODE 1)
+    __shared__ int32_t cs[kCells + 1];
+    // sweep state: one particle per lane, and the owner lane of each (particle, row) item
+    __shared__ double w_sb[64], w_tb[64], w_r2[64], w_wb[64], w_ih[64];
+    __shared__ int32_t w_p[64], w_cx0[64], w_cx1[64], w_cy0[64], w_start[64];
+    __shared__ unsigned char w_own[64 * kTS];
     const int lane = threadIdx.x;
+    const int nacc = MODE == 0 ? 1 + cat.n : 3;
+    double* S = dyn;
+    double* Tt = dyn + kCap;
+    double* acc = dyn + 2 * kCap;
+    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);
+    float* w_A = reinterpret_cast<float*>(dyn + (2 + nacc) * kCap);     // [ncol][64]
     const double Kv = support(K);
     const int64_t n = T.n;
-    const int nacc = MODE == 0 ? 1 + cat.n : 3;
     const int64_t ntiles = (int64_t)f.tx * f.ty;
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int2 cr = cells[tile * kCells + lane];
-        clo[lane] = cr.x;
-        chi[lane] = cr.y;
-        const int L0 = wave_min(cr.y > cr.x ? cr.x : 0x7fffffff);
-        const int L1 = wave_max(cr.y > cr.x ? cr.y : -1);
         __syncthreads();
+        cs[lane] = cstart[tile * kCells + lane];
+        if (lane == 0) cs[kCells] = cstart[(tile + 1) * kCells];
+        __syncthreads();
+        const int L0 = cs[0], L1 = cs[kCells];
         if (L0 >= L1) continue;
         const int txi = (int)(tile % f.tx), tyi = (int)(tile / f.tx);
         const int cx_base = txi * kTS, cy_base = tyi * kTS;
@@ -183,6 +173,7 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
         const double rs1 = cx_base + kTS >= f.gx ? __builtin_inf() : f.so + (double)(cx_base + kTS) * f.cs;
         const double rt0 = cy_base == 0 ? -__builtin_inf() : f.to + (double)cy_base * f.ct;
         const double rt1 = cy_base + kTS >= f.gy ? __builtin_inf() : f.to + (double)(cy_base + kTS) * f.ct;
+        const double near_s = f.near * f.cs, near_t = f.near * f.ct;
 
         for (int C0 = L0; C0 < L1; C0 += kCap) {
             const int m = min(kCap, L1 - C0);
@@ -192,10 +183,10 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                     S[k] = v.x;
                     Tt[k] = v.y;
                 }
-                if (MODE == 0)
-                    for (int a = 0; a < nacc; ++a) acc[a * kCap + k] = 0.0;
-                else
-                    for (int a = 0; a < 3; ++a) dg[a * kCap + k] = 0ull;
+                for (int a = 0; a < nacc; ++a) {
+                    if (MODE == 0) acc[a * kCap + k] = 0.0;
+                    else dg[a * kCap + k] = 0ull;
+                }
             }
             __syncthreads();
 
@@ -210,11 +201,11 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                 return gs * gs + gt * gt <= Rp * Rp;
             };
             // One sweep = up to 64 particles of a leaf run, one per lane.  Each lane finds the
-            // cells of this tile its footprint covers; the (particle, cell) items of the sweep
-            // are then dealt round-robin to the lanes, so a few particles with large h do not
-            // hold the other lanes idle, and neighbouring lanes add into different lines.
+            // rows and columns of this tile its footprint covers; the (particle, row) items of the
+            // sweep are dealt round-robin to the lanes (particles with large h do not hold the
+            // other lanes idle), and a row's lines are one contiguous run of sorted lines.
             auto sweep = [&](int64_t p, bool valid) {
-                int ncell = 0;
+                int nrow = 0;
                 if (valid) {
                     const float4 H = T.leaf[p];
                     const float4 Lo = T.leaflo[p];
@@ -238,50 +229,51 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                         const int cx0 = max(gx0 - cx_base, 0), cx1 = min(gx1 - cx_base, kTS - 1);
                         const int cy0 = max(gy0 - cy_base, 0), cy1 = min(gy1 - cy_base, kTS - 1);
                         if (cx0 <= cx1 && cy0 <= cy1) {
-                            ncell = (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+                            nrow = cy1 - cy0 + 1;
                             w_sb[lane] = sb;
                             w_tb[lane] = tb;
                             w_r2[lane] = r2;
-                            w_p[lane] = (int)p;
+                            w_p[lane] = (int32_t)p;
                             w_cx0[lane] = cx0;
+                            w_cx1[lane] = cx1;
                             w_cy0[lane] = cy0;
-                            w_ncx[lane] = cx1 - cx0 + 1;
                             if (MODE == 0) {
                                 w_wb[lane] = T.w64[p];
                                 w_ih[lane] = 1.0 / hb;
-                                for (int c = 0; c < cat.n; ++c) w_A[c][lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
+                                for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                             }
                         }
                     }
                 }
                 // exclusive prefix of the item counts across the wave
-                int incl = ncell;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int u = __shfl_up(incl, o);
-                    if (lane >= o) incl += u;
+                int incl = nrow;
+                for (int o2 = 1; o2 < 64; o2 <<= 1) {
+                    const int u = __shfl_up(incl, o2);
+                    if (lane >= o2) incl += u;
                 }
                 const int total = __shfl(incl, 63);
                 if (total == 0) return;
-                const int excl = incl - ncell;
+                const int excl = incl - nrow;
                 w_start[lane] = excl;
-                for (int i = 0; i < ncell; ++i) w_own[excl + i] = (unsigned char)lane;
+                for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
                 __syncthreads();
                 for (int t = lane; t < total; t += 64) {
                     const int j = w_own[t];
-                    const int c = t - w_start[j];
-                    const int ncx = w_ncx[j];
-                    const int qy = (int)(((float)c + 0.5f) * __builtin_amdgcn_rcpf((float)ncx));
-                    const int cell = (w_cy0[j] + qy) * kTS + w_cx0[j] + (c - qy * ncx);
-                    const int a = max(clo[cell], C0), b = min(chi[cell], C0 + m);
+                    const int row = w_cy0[j] + (t - w_start[j]);
+                    const int a = max(cs[row * kTS + w_cx0[j]], C0);
+                    const int b = min(cs[row * kTS + w_cx1[j] + 1], C0 + m);
                     if (a >= b) continue;
                     const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
-                    for (int k = a; k < b; ++k) {
+                    const int64_t pj = w_p[j];
+                    // lanes on the same row start at different columns (fewer LDS bank conflicts)
+                    const int len = b - a;
+                    int k = a + (lane % len);
+                    for (int it = 0; it < len; ++it, k = (k + 1 == b) ? a : k + 1) {
                         const int kk = k - C0;
                         const double ds = sb - S[kk], dt = tb - Tt[kk];
                         const double d2 = ds * ds + dt * dt;
                         if (d2 > r2 + f.band) continue;
                         bool ok = d2 < r2 - f.band;
-                        const int64_t pj = w_p[j];
                         if (!ok) {
                             // reference formula on the FP64 columns (_squared_distance_line_coords)
                             const int64_t li = perm[k];
@@ -293,22 +285,27 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                         }
                         if (!ok) continue;
                         if (MODE == 0) {
-                            // line_integrated_kernel_function: invh * Inorm(r invh) in FP64, the
-                            // table lerp from (value, slope) pairs
+                            // line_integrated_kernel_function: invh * Inorm(r invh).  r from an FP32
+                            // reciprocal square root and one FP64 correction step (relative error
+                            // ~1e-14), q and the table interpolation in FP64 on FP32 table values
                             const double invh = w_ih[j];
-                            const double q = sqrt(d2) * invh;
+                            const float d2f = (float)d2;
+                            const float rs = d2f > 0.0f ? __builtin_amdgcn_rsqf(d2f) : 0.0f;
+                            const double r0 = (double)(d2f * rs);
+                            const double r = r0 + (d2 - r0 * r0) * (0.5 * (double)rs);
+                            const double q = r * invh;
                             double I = 0.0;
                             if (q < Kv) {
                                 const double idxf = q * inv_dq + 1.0;
                                 int i = (int)idxf;
                                 i = i < 1 ? 1 : (i > kLutN - 1 ? kLutN - 1 : i);
-                                const double2 L = lut[i - 1];
-                                I = L.x + (idxf - (double)i) * L.y;
+                                const double t = idxf - (double)i;
+                                I = (double)lut[i - 1] * (1.0 - t) + (double)lut[i] * t;
                             }
                             const double w = w_wb[j] * (invh * I);
                             atomicAdd(&acc[kk], w);
                             for (int cc = 0; cc < cat.n; ++cc)
-                                atomicAdd(&acc[(1 + cc) * kCap + kk], (double)w_A[cc][j] * w);
+                                atomicAdd(&acc[(1 + cc) * kCap + kk], (double)w_A[cc * 64 + j] * w);
                         } else {
                             const unsigned long long ip = (unsigned long long)pj;
                             atomicAdd(&dg[kk], 1ull);
@@ -319,12 +316,25 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                 }
                 __syncthreads();
             };
-            // runs of up to T.bucket leaves (contiguous in Morton order), 64 per sweep
+            // runs of leaves (contiguous in Morton order), 64 per sweep
             auto bucket = [&](int lo, int cnt) {
                 for (int base = 0; base < cnt; base += 64)
                     sweep((int64_t)lo + base + lane, base + lane < cnt);
             };
-            walk(T, true, need, bucket);
+            // A node is taken whole (up to T.bucket leaves) when its box itself lies near the
+            // tile; a node that reaches the tile only through a large hmax is descended down to
+            // one sweep of 64, so that only the subtrees holding the large-h particles are read.
+            auto take = [&](const Node32& nd, int cnt) {
+                if (cnt <= 64) return true;
+                if (cnt > T.bucket) return false;
+                double s0, s1, t0, t1;
+                project_box(nd, f.u, s0, s1);
+                project_box(nd, f.v, t0, t1);
+                const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
+                const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
+                return gs <= near_s && gt <= near_t;
+            };
+            walk(T, true, need, bucket, take);
             __syncthreads();
             for (int k = lane; k < m; k += 64) {
                 const int64_t li = perm[C0 + k];
@@ -342,26 +352,26 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                     digest[2 * nl + li] = dg[2 * kCap + k];
                 }
             }
-            __syncthreads();
         }
     }
 }
 
 template <int K, int MODE>
-void launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int2* cells, const int32_t* perm,
+void launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
                   const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest) {
     const int64_t ntiles = (int64_t)f.tx * f.ty;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
-    const size_t lds = sizeof(double) * (size_t)kCap * (2 + (MODE == 0 ? 1 + cat.n : 3));
+    const size_t lds = sizeof(double) * (size_t)kCap * (2 + (MODE == 0 ? 1 + cat.n : 3)) +
+                       sizeof(float) * 64 * (size_t)(MODE == 0 ? cat.n : 0);
     T32 T = tree32(c, 64);
     T.bucket = g_tiled_bucket;
     hipLaunchKernelGGL((k_lines_tiled<K, MODE>), dim3(grid), dim3(64), lds, c->stream, T, f, cat, cells,
-                       perm, sts, o, nl, c->lut2.as<double2>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out, ldo,
+                       perm, sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out, ldo,
                        digest);
 }
 
 template <int MODE>
-void launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int2* cells, const int32_t* perm,
+void launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
                     const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest) {
     switch (c->tree.kernel) {
     case kM4: launch_tiled<kM4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
@@ -377,7 +387,7 @@ void launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int2* 
 // lines are not a parallel set of unit-length lines the tiled path handles.
 struct Binned {
     Frame f;
-    const int2* cells;
+    const int32_t* cells;
     const int32_t* perm;
     const double2* sts;
 };
@@ -468,6 +478,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     f.cs = gx > 1 ? es / (double)(gx - 1) : std::max(es, 1.0);
     f.ct = gy > 1 ? et / (double)(gy - 1) : std::max(et, 1.0);
     if (!(f.cs > 0.0) || !(f.ct > 0.0)) return PMX_OK;
+    f.near = (double)g_tiled_near;
     f.so = smin - 0.5 * f.cs;
     f.to = tmin - 0.5 * f.ct;
     f.gx = (int)gx;
@@ -480,7 +491,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
 
     PMX_TRY(ensure(c, c->scratch_a, 12 * (size_t)nl + 64));
     PMX_TRY(ensure(c, c->scratch_b, 12 * (size_t)nl + 64));
-    PMX_TRY(ensure(c, c->tl_cells, sizeof(int2) * (size_t)ncell + sizeof(double2) * (size_t)nl + 64));
+    PMX_TRY(ensure(c, c->tl_cells, sizeof(int32_t) * (size_t)(ncell + 1) + sizeof(double2) * (size_t)nl + 64));
     uint64_t* k0 = c->scratch_a.as<uint64_t>();
     int32_t* v0 = reinterpret_cast<int32_t*>(k0 + nl);
     uint64_t* k1 = c->scratch_b.as<uint64_t>();
@@ -491,10 +502,9 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     PMX_TRY(radix_sort_pairs(c, k0, v0, nl, k1, v1, 0, bits, &alt));
     const uint64_t* ks = alt ? k1 : k0;
     const int32_t* perm = alt ? v1 : v0;
-    int2* cells = c->tl_cells.as<int2>();
-    double2* sts = reinterpret_cast<double2*>(cells + ncell + (ncell & 1));
-    hipLaunchKernelGGL(k_zero_int2, dim3(grid_for(ncell, 256, 8192)), b, 0, c->stream, cells, ncell);
-    hipLaunchKernelGGL(k_line_cells, g, b, 0, c->stream, ks, perm, c->tl_st.as<double>(), nl, cells, sts);
+    int32_t* cells = c->tl_cells.as<int32_t>();
+    double2* sts = reinterpret_cast<double2*>(c->tl_cells.as<char>() + ((sizeof(int32_t) * (size_t)(ncell + 1) + 15) & ~(size_t)15));
+    hipLaunchKernelGGL(k_line_cells, g, b, 0, c->stream, ks, perm, c->tl_st.as<double>(), nl, ncell, cells, sts);
     PMX_HIP(c, hipGetLastError());
     out->f = f;
     out->cells = cells;
@@ -508,6 +518,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
 
 int g_lines_tiled = 1;
 int g_tiled_bucket = 512;
+int g_tiled_near = 4;
 
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used) {

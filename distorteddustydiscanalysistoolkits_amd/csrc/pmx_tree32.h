@@ -26,8 +26,10 @@ __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v);
 
 // Union walk of the tile over left / escape links.  need(nd) is each lane's conservative
 // node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
-template <class Need, class Bucket>
-__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket) {
+// take(nd, cnt) says whether a needed node is handed over as one run instead of descended
+// (the default takes every run of at most T.bucket leaves).
+template <class Need, class Bucket, class Take>
+__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket, Take&& take) {
     const int64_t n = T.n;
     if (!any_valid) return;
     if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
@@ -49,13 +51,18 @@ __device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& 
             continue;
         }
         const int lo = uniform(nd.lo), cnt = uniform(nd.cnt);
-        if (cnt <= T.bucket && cnt >= 1 && lo >= 0 && lo + cnt <= n) {
+        if (cnt >= 1 && lo >= 0 && lo + cnt <= n && ballot(take(nd, cnt))) {
             bucket(lo, cnt);
             node = uniform(nd.escape);
             continue;
         }
         node = uniform(nd.left);
     }
+}
+
+template <class Need, class Bucket>
+__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket) {
+    walk(T, any_valid, need, bucket, [&](const Node32&, int cnt) { return cnt <= T.bucket; });
 }
 
 // FP32 tree view of the context's current LBVH (prepare_f32 must have run); `bucket` is the
