@@ -220,13 +220,15 @@ const char* pmx_version(void) { return "partia_mi355x 0.1 (gfx950)"; }
 // line set through the union walk; PMX_BUCKET_POINTS / PMX_BUCKET_LINES set the leaf-run
 // size the sample-tile walks hand over at once (1..64); PMX_TILED_BUCKET the leaf-run size of
 // the line-map tile walk (1..65536); PMX_TILED_NEAR how close (in cells) a node's box must be to
-// the tile for the walk to read it whole.
+// the tile for the walk to read it whole; PMX_REFIT_SMALL the leaf-run length up to which
+// the refit computes a node directly from its leaves.
 static void read_tuning() {
     if (const char* v = std::getenv("PMX_LINES_TILED")) g_lines_tiled = std::atoi(v) != 0;
     if (const char* v = std::getenv("PMX_BUCKET_POINTS")) g_bucket_points = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_BUCKET_LINES")) g_bucket_lines = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_BUCKET")) g_tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_NEAR")) g_tiled_near = std::max(0, std::min(1 << 20, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_REFIT_SMALL")) g_refit_small = std::max(1, std::min(1024, std::atoi(v)));
 }
 
 int pmx_create(pmx_ctx** out, int32_t device_ordinal) {

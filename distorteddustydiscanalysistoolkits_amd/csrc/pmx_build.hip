@@ -401,10 +401,23 @@ __global__ void __launch_bounds__(kBlock) k_pack_records(ColPtrs src, int ncols,
         for (int c = 0; c < ncols; ++c) rec[i * ncols + c] = static_cast<const T*>(src.p[c])[i];
 }
 
+// Also writes the FP32 leaf data of the compensated path: {x_hi, y_hi, z_hi, h} and
+// {x_lo, y_lo, z_lo, 1/h} (hi + lo = the FP64 coordinate to 2^-48), and the payload columns
+// m/rho | quant... | rho in FP32 followed by m/rho in FP64.
+struct Leaf32Out {
+    float4* leaf;
+    float4* leaflo;
+    float* pay;
+    double* w64;
+    double* ih64;
+};
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__ rec, int ncols,
                                                            const int32_t* __restrict__ order, int64_t n,
-                                                           double* __restrict__ cols64, Leaf64* __restrict__ leaf) {
+                                                           double* __restrict__ cols64, Leaf64* __restrict__ leaf,
+                                                           Leaf32Out o32) {
+    const int ncol = ncols - 6;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         int32_t o = order[i];
         o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;   // in-bounds guard (order is a permutation)
@@ -415,13 +428,26 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
             v[c] = (double)r[c];
             cols64[(int64_t)c * n + i] = v[c];
         }
-        for (int c = 6; c < ncols; ++c) cols64[(int64_t)c * n + i] = (double)r[c];
+        for (int c = 6; c < ncols; ++c) {
+            const double q = (double)r[c];
+            cols64[(int64_t)c * n + i] = q;
+            o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
+        }
         Leaf64 L;
         L.x = v[0];
         L.y = v[1];
         L.z = v[2];
         L.h = v[4];
         leaf[i] = L;
+        const float xh = (float)v[0], yh = (float)v[1], zh = (float)v[2];
+        o32.leaf[i] = make_float4(xh, yh, zh, (float)v[4]);
+        o32.leaflo[i] = make_float4((float)(v[0] - (double)xh), (float)(v[1] - (double)yh),
+                                    (float)(v[2] - (double)zh), (float)(1.0 / v[4]));
+        const double w = v[3] / v[5];
+        o32.pay[i] = (float)w;
+        o32.pay[(int64_t)(1 + ncol) * n + i] = (float)v[5];
+        o32.w64[i] = w;
+        o32.ih64[i] = 1.0 / v[4];
     }
 }
 
@@ -551,8 +577,27 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, i
 // in rounds < r (or leaves), and consecutive rounds are separate kernel launches on one
 // stream, so node data needs no fences; the arrival counter is the only shared atomic.
 // Round 0 starts from every leaf; later rounds from the list the previous round appended.
+// The FP32 copy of a node used by the FP32-compensated traversals: box rounded outward,
+// hmax rounded up (every FP32 node test stays conservative), plus its 0-based leaf run.
+__device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __restrict__ rlo,
+                                  const int32_t* __restrict__ rhi, Node64* nodes, Node32* __restrict__ n32) {
+    nodes[i] = a;
+    Node32 b;
+    b.mnx = f32_down(a.mn[0]); b.mny = f32_down(a.mn[1]); b.mnz = f32_down(a.mn[2]);
+    b.mxx = f32_up(a.mx[0]); b.mxy = f32_up(a.mx[1]); b.mxz = f32_up(a.mx[2]);
+    b.hmax = f32_up(a.hmax);
+    b.left = a.left;
+    b.escape = a.escape;
+    b.lo = rlo[i] - 1;
+    b.cnt = rhi[i] - rlo[i] + 1;
+    b.pad = 0;
+    n32[i] = b;
+}
+
 __device__ inline void combine_node(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
-                                    const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p) {
+                                    const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p,
+                                    const int32_t* __restrict__ rlo, const int32_t* __restrict__ rhi,
+                                    Node32* __restrict__ n32) {
     const int32_t l = lr[2 * p], r = lr[2 * p + 1];
     double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
     if (l >= n) {
@@ -579,7 +624,7 @@ __device__ inline void combine_node(const Leaf64* __restrict__ leaf, const int32
     out.hmax = hl > hr ? hl : hr;
     out.left = l;
     out.escape = escape[p];
-    nodes[p] = out;
+    store_node(out, p, rlo, rhi, nodes, n32);
 }
 
 // Append `p` (when `take`) to the next round's list with one atomic per wavefront.
@@ -598,13 +643,34 @@ __device__ inline void wave_append(bool take, int32_t p, int32_t* __restrict__ l
 // exact, so the result does not depend on the order).  Every other node starts its counter
 // at the number of children that are already complete; nodes with both children complete
 // form the first list of the bottom-up rounds.
-constexpr int kSmallRange = 16;
+
+// Block-wide version of wave_append: one atomic per workgroup of kBlock threads (the list
+// counter is a single address that every workgroup increments).  Call from every thread.
+__device__ inline void block_append(bool take, int32_t p, int32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_cnt[kBlock / 64];
+    __shared__ uint32_t base_all;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(take);
+    if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wave_cnt[w];
+        base_all = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = base_all;
+    for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+    if (take) list[off + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = p;
+    __syncthreads();
+}
 
 __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
                                                         const int32_t* __restrict__ range_lo,
                                                         const int32_t* __restrict__ range_hi, int64_t n,
                                                         uint32_t* __restrict__ visited, Node64* nodes,
+                                                        Node32* __restrict__ n32, int small_range,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                            // internal nodes 1 .. n-1
@@ -614,7 +680,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
         bool take = false;
         if (i < n) {
             const int32_t lo = range_lo[i], hi = range_hi[i];
-            if (lo >= 1 && hi <= n && hi - lo + 1 <= kSmallRange) {
+            if (lo >= 1 && hi <= n && hi - lo + 1 <= small_range) {
                 double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
                 double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
                 double hm = -__builtin_inf();
@@ -630,50 +696,63 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
                 out.hmax = hm;
                 out.left = lr[2 * i];
                 out.escape = escape[i];
-                nodes[i] = out;
+                store_node(out, i, range_lo, range_hi, nodes, n32);
             } else {
                 uint32_t done = 0;
                 for (int side = 0; side < 2; ++side) {
                     const int32_t ch = lr[2 * i + side];
                     if (ch >= n) done += 1;
-                    else if (ch > 0 && range_hi[ch] - range_lo[ch] + 1 <= kSmallRange) done += 1;
+                    else if (ch > 0 && range_hi[ch] - range_lo[ch] + 1 <= small_range) done += 1;
                 }
                 visited[i] = done;
                 take = done == 2;
             }
         }
-        wave_append(take, (int32_t)i, next, next_count);
+        block_append(take, (int32_t)i, next, next_count);
     }
 }
 
 // One bottom-up round: complete every node of the list from its two complete children, then
 // count it at its parent; a parent counted twice is complete-able in the next round.
-__device__ inline void refit_step(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
-                                  const int32_t* __restrict__ parent, const int32_t* __restrict__ escape, int64_t n,
-                                  uint32_t* __restrict__ visited, Node64* nodes, int32_t x, bool valid,
-                                  int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+struct RefitTree {
+    const Leaf64* leaf;
+    const int32_t* lr;
+    const int32_t* parent;
+    const int32_t* escape;
+    const int32_t* rlo;
+    const int32_t* rhi;
+    int64_t n;
+    uint32_t* visited;
+    Node64* nodes;
+    Node32* n32;
+};
+
+template <bool BLOCK>
+__device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int32_t* __restrict__ next,
+                                  uint32_t* __restrict__ next_count) {
+    const int64_t n = R.n;
     bool take = false;
     int32_t p = 0;
     if (valid && x > 0 && x < n) {
-        combine_node(leaf, lr, escape, n, nodes, x);
+        combine_node(R.leaf, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
+        const int32_t* parent = R.parent;
+        uint32_t* visited = R.visited;
         p = parent[x];
         if (p > 0 && p < n) take = atomicAdd(&visited[p], 1u) == 1u;
     }
-    wave_append(take, p, next, next_count);
+    if (BLOCK) block_append(take, p, next, next_count);
+    else wave_append(take, p, next, next_count);
 }
 
-__global__ void __launch_bounds__(kBlock) k_refit_round(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
-                                                        const int32_t* __restrict__ parent,
-                                                        const int32_t* __restrict__ escape, int64_t n,
-                                                        uint32_t* __restrict__ visited, Node64* nodes,
-                                                        const int32_t* __restrict__ cur, const uint32_t* __restrict__ cur_count,
+__global__ void __launch_bounds__(kBlock) k_refit_round(RefitTree R, const int32_t* __restrict__ cur,
+                                                        const uint32_t* __restrict__ cur_count,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
     const int64_t count = *cur_count;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t k = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        refit_step(leaf, lr, parent, escape, n, visited, nodes, k < count ? cur[k] : 0, k < count, next, next_count);
+        refit_step<true>(R, k < count ? cur[k] : 0, k < count, next, next_count);
     }
 }
 
@@ -682,13 +761,8 @@ __global__ void __launch_bounds__(kBlock) k_refit_round(const Leaf64* __restrict
 constexpr int kTailBlock = 1024;
 constexpr int kMaxRounds = 4000;
 
-__global__ void __launch_bounds__(kTailBlock) k_refit_tail(const Leaf64* __restrict__ leaf,
-                                                           const int32_t* __restrict__ lr,
-                                                           const int32_t* __restrict__ parent,
-                                                           const int32_t* __restrict__ escape, int64_t n,
-                                                           uint32_t* __restrict__ visited, Node64* nodes,
-                                                           int32_t* listA, int32_t* listB, uint32_t* counts,
-                                                           int first_round) {
+__global__ void __launch_bounds__(kTailBlock) k_refit_tail(RefitTree R, int32_t* listA, int32_t* listB,
+                                                           uint32_t* counts, int first_round) {
     for (int r = first_round; r < kMaxRounds - 1; ++r) {
         __syncthreads();
         const int64_t count = __hip_atomic_load(&counts[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -698,8 +772,7 @@ __global__ void __launch_bounds__(kTailBlock) k_refit_tail(const Leaf64* __restr
         const int64_t iters = (count + kTailBlock - 1) / kTailBlock;
         for (int64_t it = 0; it < iters; ++it) {
             const int64_t k = it * kTailBlock + threadIdx.x;
-            refit_step(leaf, lr, parent, escape, n, visited, nodes, k < count ? cur[k] : 0, k < count, nxt,
-                       &counts[r + 1]);
+            refit_step<false>(R, k < count ? cur[k] : 0, k < count, nxt, &counts[r + 1]);
         }
     }
 }
@@ -744,6 +817,8 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
 
 // Refit: short-range nodes directly, then kLaunchRounds grid-wide rounds (the long lists
 // near the leaves), then the tail kernel for the rest; no host synchronisation.
+int g_refit_small = 32;
+
 int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     hipStream_t s = c->stream;
     constexpr int kLaunchRounds = 10;
@@ -752,21 +827,28 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     int32_t* listB = listA + (n / 2 + 64);
     uint32_t* counts = reinterpret_cast<uint32_t*>(c->scratch_c.p);
     hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, kMaxRounds, 0u);
-    const unsigned gr = std::max(1u, std::min(g, 4096u));
-    hipLaunchKernelGGL(k_refit_small, dim3(gr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
-                       c->escape.as<int32_t>(), c->range_lo.as<int32_t>(), c->range_hi.as<int32_t>(), n,
-                       c->visited.as<uint32_t>(), c->nodes64.as<Node64>(), listA, &counts[0]);
+    RefitTree R;
+    R.leaf = c->leaf64.as<Leaf64>();
+    R.lr = c->lr.as<int32_t>();
+    R.parent = c->parent.as<int32_t>();
+    R.escape = c->escape.as<int32_t>();
+    R.rlo = c->range_lo.as<int32_t>();
+    R.rhi = c->range_hi.as<int32_t>();
+    R.n = n;
+    R.visited = c->visited.as<uint32_t>();
+    R.nodes = c->nodes64.as<Node64>();
+    R.n32 = c->nodes32.as<Node32>();
+    const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
+    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.leaf, R.lr, R.escape, R.rlo, R.rhi, n,
+                       R.visited, R.nodes, R.n32, g_refit_small, listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
         int32_t* cur = (round & 1) ? listB : listA;
         int32_t* nxt = (round & 1) ? listA : listB;
-        hipLaunchKernelGGL(k_refit_round, dim3(grr), dim3(kBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
-                           c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
-                           c->nodes64.as<Node64>(), cur, &counts[round], nxt, &counts[round + 1]);
+        hipLaunchKernelGGL(k_refit_round, dim3(grr), dim3(kBlock), 0, s, R, cur, &counts[round], nxt,
+                           &counts[round + 1]);
     }
-    hipLaunchKernelGGL(k_refit_tail, dim3(1), dim3(kTailBlock), 0, s, c->leaf64.as<Leaf64>(), c->lr.as<int32_t>(),
-                       c->parent.as<int32_t>(), c->escape.as<int32_t>(), n, c->visited.as<uint32_t>(),
-                       c->nodes64.as<Node64>(), listA, listB, counts, kLaunchRounds);
+    hipLaunchKernelGGL(k_refit_tail, dim3(1), dim3(kTailBlock), 0, s, R, listA, listB, counts, kLaunchRounds);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }
@@ -846,14 +928,25 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     const unsigned g = grid_for(n, kBlock, 16384);
     PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)n));
     PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
+    // FP32 data of the compensated path (layout shared with tree32 / prepare_f32)
+    const size_t payf = (size_t)n * (2 + p->ncol);
+    PMX_TRY(ensure(c, c->nodes32, sizeof(Node32) * (size_t)(n + 1)));
+    PMX_TRY(ensure(c, c->leaf32, 2 * sizeof(float4) * (size_t)n));
+    PMX_TRY(ensure(c, c->pay32, 4 * (payf + (payf & 1)) + 16 * (size_t)n));
+    Leaf32Out o32;
+    o32.leaf = c->leaf32.as<float4>();
+    o32.leaflo = c->leaf32.as<float4>() + n;
+    o32.pay = c->pay32.as<float>();
+    o32.w64 = reinterpret_cast<double*>(c->pay32.as<float>() + payf + (payf & 1));
+    o32.ih64 = o32.w64 + n;
     if (p->dtype == PMX_F64) {
         hipLaunchKernelGGL(k_pack_records<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<double>());
         hipLaunchKernelGGL(k_gather_records<double>, dim3(g), dim3(kBlock), 0, s, c->records.as<double>(), ncols,
-                           order, n, cols64, c->leaf64.as<Leaf64>());
+                           order, n, cols64, c->leaf64.as<Leaf64>(), o32);
     } else {
         hipLaunchKernelGGL(k_pack_records<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<float>());
         hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
-                           order, n, cols64, c->leaf64.as<Leaf64>());
+                           order, n, cols64, c->leaf64.as<Leaf64>(), o32);
     }
     PMX_HIP(c, hipEventRecord(c->ev[2], s));
     // ---- radix tree, escapes, refit ----
@@ -924,7 +1017,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     c->tree.ncol = p->ncol;
     c->tree.kernel = p->kernel;
     c->tree.built = true;
-    c->tree.f32_ready = false;   // the FP32 tree is derived lazily by prepare_f32
+    c->tree.f32_ready = true;    // the refit and the record gather also wrote the FP32 tree
     return PMX_OK;
 }
 

@@ -276,4 +276,23 @@ __device__ inline double radius2_rule(double Kv, double radius2, double h) {
     return jl_max(radius2, r * r);
 }
 
+
+// neighbouring floats towards -inf / +inf (finite, non-NaN input)
+__device__ inline float next_down(float f) {
+    if (f == 0.0f) return -__uint_as_float(1u);
+    const uint32_t u = __float_as_uint(f);
+    return __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
+}
+__device__ inline float next_up(float f) { return -next_down(-f); }
+__device__ inline float f32_down(double d) {
+    float f = (float)d;
+    if ((double)f > d) f = next_down(f);
+    return f;
+}
+__device__ inline float f32_up(double d) {
+    float f = (float)d;
+    if ((double)f < d) f = next_up(f);
+    return f;
+}
+
 }  // namespace pmx

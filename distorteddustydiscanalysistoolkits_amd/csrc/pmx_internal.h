@@ -143,7 +143,7 @@ int prepare_f32(pmx_ctx* c);
 // Parallel unit-direction line sets: particle-centric tiled path.  *used is false (and
 // nothing is written) when the lines do not qualify; the caller then uses the union walk.
 extern int g_lines_tiled;
-extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near;
+extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near, g_refit_small;
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used);
 int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);

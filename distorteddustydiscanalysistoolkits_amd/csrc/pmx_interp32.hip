@@ -38,63 +38,6 @@ constexpr float kEps = 4e-6f;     // guard band of FP32 distance / radius compar
 constexpr int kMaxF = 9;          // distinct field columns staged per leaf (points)
 constexpr int kMaxD = 7;          // distinct derivative columns staged per leaf (1 + 3 + 3)
 
-// neighbouring floats towards -inf / +inf (finite, non-NaN input)
-__device__ inline float next_down(float f) {
-    if (f == 0.0f) return -__uint_as_float(1u);
-    const uint32_t u = __float_as_uint(f);
-    return __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
-}
-__device__ inline float next_up(float f) { return -next_down(-f); }
-__device__ inline float f32_down(double d) {
-    float f = (float)d;
-    if ((double)f > d) f = next_down(f);
-    return f;
-}
-__device__ inline float f32_up(double d) {
-    float f = (float)d;
-    if ((double)f < d) f = next_up(f);
-    return f;
-}
-
-// FP32 tree: boxes rounded outwards, hmax rounded up, leaf range of each node.
-__global__ void __launch_bounds__(256) k_prep_nodes(const Node64* __restrict__ n64, const int32_t* __restrict__ rlo,
-                                                    const int32_t* __restrict__ rhi, int64_t n,
-                                                    Node32* __restrict__ n32) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x + 1; i < n; i += (int64_t)gridDim.x * 256) {
-        const Node64 a = n64[i];
-        Node32 b;
-        b.mnx = f32_down(a.mn[0]); b.mny = f32_down(a.mn[1]); b.mnz = f32_down(a.mn[2]);
-        b.mxx = f32_up(a.mx[0]); b.mxy = f32_up(a.mx[1]); b.mxz = f32_up(a.mx[2]);
-        b.hmax = f32_up(a.hmax);
-        b.left = a.left;
-        b.escape = a.escape;
-        b.lo = rlo[i] - 1;
-        b.cnt = rhi[i] - rlo[i] + 1;
-        b.pad = 0;
-        n32[i] = b;
-    }
-}
-
-// leaf: {x_hi, y_hi, z_hi, h}; leaflo: {x_lo, y_lo, z_lo, 1/h}; pay column 0 = m/rho,
-// columns 1.. = quant columns, column 1+ncol = rho; w64 = m/rho in FP64.
-__global__ void __launch_bounds__(256) k_prep_leaves(const double* __restrict__ cols, int64_t n, int ncol,
-                                                     float4* __restrict__ leaf, float4* __restrict__ leaflo,
-                                                     float* __restrict__ pay, double* __restrict__ w64) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const double x = cols[i], y = cols[n + i], z = cols[2 * n + i];
-        const double m = cols[3 * n + i], h = cols[4 * n + i], rho = cols[5 * n + i];
-        const float xh = (float)x, yh = (float)y, zh = (float)z;
-        leaf[i] = make_float4(xh, yh, zh, (float)h);
-        leaflo[i] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
-                                (float)(1.0 / h));
-        const double w = m / rho;
-        pay[i] = (float)w;
-        w64[i] = w;
-        for (int c = 0; c < ncol; ++c) pay[(int64_t)(1 + c) * n + i] = (float)cols[(int64_t)(6 + c) * n + i];
-        pay[(int64_t)(1 + ncol) * n + i] = (float)rho;
-    }
-}
-
 // 30-bit Morton key of a sample position inside the particle box (clamped), for tiling.
 __device__ inline uint64_t spread10(uint32_t v) {
     uint64_t x = v & 0x3ff;
@@ -834,6 +777,7 @@ T32 tree32(pmx_ctx* c, int bucket) {
     T.cols64 = c->cols64.as<double>();
     T.pay = c->pay32.as<float>();
     T.w64 = reinterpret_cast<const double*>(c->pay32.as<float>() + payf + (payf & 1));
+    T.ih64 = T.w64 + c->tree.n;
     T.n = c->tree.n;
     T.root = c->tree.root;
     T.bucket = std::max(1, std::min(bucket, kMaxBucket));
@@ -862,23 +806,10 @@ int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, 
     return PMX_OK;
 }
 
+// The FP32 tree (nodes32, leaf32, pay32) is written by the build itself (refit and column
+// gather); this only checks that it belongs to the current tree.
 int prepare_f32(pmx_ctx* c) {
-    if (c->tree.f32_ready) return PMX_OK;
-    const int64_t n = c->tree.n;
-    const int ncol = c->tree.ncol;
-    const size_t payf = (size_t)n * (2 + ncol);
-    PMX_TRY(ensure(c, c->nodes32, sizeof(Node32) * (size_t)(n + 1)));
-    PMX_TRY(ensure(c, c->leaf32, 2 * sizeof(float4) * (size_t)n));
-    PMX_TRY(ensure(c, c->pay32, 4 * (payf + (payf & 1)) + 8 * (size_t)n));
-    const dim3 g(grid_for(n, 256, 16384)), b(256);
-    if (n >= 2)
-        hipLaunchKernelGGL(k_prep_nodes, g, b, 0, c->stream, c->nodes64.as<Node64>(), c->range_lo.as<int32_t>(),
-                           c->range_hi.as<int32_t>(), n, c->nodes32.as<Node32>());
-    hipLaunchKernelGGL(k_prep_leaves, g, b, 0, c->stream, c->cols64.as<double>(), n, ncol, c->leaf32.as<float4>(),
-                       c->leaf32.as<float4>() + n, c->pay32.as<float>(),
-                       reinterpret_cast<double*>(c->pay32.as<float>() + payf + (payf & 1)));
-    PMX_HIP(c, hipGetLastError());
-    c->tree.f32_ready = true;
+    if (!c->tree.f32_ready) return set_error(c, PMX_ERR_STATE, "FP32 tree data missing: rebuild the LBVH");
     return PMX_OK;
 }
 

@@ -44,6 +44,8 @@ struct Frame {
     int gx, gy, tx, ty;                // cells and tiles along s and t
     double band;                       // absolute band on d^2 around (K h)^2 -> exact re-check
     double pad;                        // absolute padding of footprint radii
+    double sqrt_band;                  // sqrt(band): R + sqrt(band) >= sqrt(R^2 + band)
+    double inv_cs, inv_ct;             // 1 / cell size (binning and footprints use the same products)
     double near;                       // cells: nodes whose box is this close are read whole
 };
 
@@ -81,14 +83,14 @@ __global__ void __launch_bounds__(256) k_line_project(const double* __restrict__
 }
 
 __device__ inline int clamp_cell(double c, int g) { return c < 0.0 ? 0 : (c > (double)(g - 1) ? g - 1 : (int)c); }
-__device__ inline int cell_of(double v, double org, double size, int g) { return clamp_cell(floor((v - org) / size), g); }
+__device__ inline int cell_of(double v, double org, double inv_size, int g) { return clamp_cell(floor((v - org) * inv_size), g); }
 
 // sort key of a line: tile-major, then cell inside the tile
 __global__ void __launch_bounds__(256) k_line_keys(const double* __restrict__ st, int64_t nl, Frame f,
                                                    uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nl; i += (int64_t)gridDim.x * 256) {
-        const int cx = cell_of(st[i], f.so, f.cs, f.gx);
-        const int cy = cell_of(st[nl + i], f.to, f.ct, f.gy);
+        const int cx = cell_of(st[i], f.so, f.inv_cs, f.gx);
+        const int cy = cell_of(st[nl + i], f.to, f.inv_ct, f.gy);
         const uint64_t tile = (uint64_t)(cy / kTS) * (uint64_t)f.tx + (uint64_t)(cx / kTS);
         keys[i] = tile * kCells + (uint64_t)((cy % kTS) * kTS + (cx % kTS));
         vals[i] = (int32_t)i;
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                 const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
                 const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
                 const double R = Kv * (double)nd.hmax;
-                const double Rp = sqrt(R * R + f.band) + f.pad;
+                const double Rp = R + f.sqrt_band + f.pad;
                 return gs * gs + gt * gt <= Rp * Rp;
             };
             // One sweep = up to 64 particles of a leaf run, one per lane.  Each lane finds the
@@ -215,17 +217,17 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                     const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
                     // reject on the FP32 h (round-to-nearest, widened by 1e-6) before more loads
                     const double Ru = Kv * (double)H.w * (1.0 + 1e-6);
-                    const double Rpu = sqrt(Ru * Ru + f.band) + f.pad;
+                    const double Rpu = Ru + f.sqrt_band + f.pad;
                     if (!(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1)) {
                         const double hb = T.cols64[4 * n + p];
                         const double rr = Kv * hb;
                         const double r2 = rr * rr;
-                        const double Rp = sqrt(r2 + f.band) + f.pad;
+                        const double Rp = rr + f.sqrt_band + f.pad;
                         // global cell range of the footprint, clamped like the binning, then tile-local
-                        const int gx0 = clamp_cell(floor((sb - Rp - f.so) / f.cs), f.gx);
-                        const int gx1 = clamp_cell(floor((sb + Rp - f.so) / f.cs), f.gx);
-                        const int gy0 = clamp_cell(floor((tb - Rp - f.to) / f.ct), f.gy);
-                        const int gy1 = clamp_cell(floor((tb + Rp - f.to) / f.ct), f.gy);
+                        const int gx0 = clamp_cell(floor((sb - Rp - f.so) * f.inv_cs), f.gx);
+                        const int gx1 = clamp_cell(floor((sb + Rp - f.so) * f.inv_cs), f.gx);
+                        const int gy0 = clamp_cell(floor((tb - Rp - f.to) * f.inv_ct), f.gy);
+                        const int gy1 = clamp_cell(floor((tb + Rp - f.to) * f.inv_ct), f.gy);
                         const int cx0 = max(gx0 - cx_base, 0), cx1 = min(gx1 - cx_base, kTS - 1);
                         const int cy0 = max(gy0 - cy_base, 0), cy1 = min(gy1 - cy_base, kTS - 1);
                         if (cx0 <= cx1 && cy0 <= cy1) {
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                             w_cy0[lane] = cy0;
                             if (MODE == 0) {
                                 w_wb[lane] = T.w64[p];
-                                w_ih[lane] = 1.0 / hb;
+                                w_ih[lane] = T.ih64[p];
                                 for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                             }
                         }
@@ -287,7 +289,8 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                         if (MODE == 0) {
                             // line_integrated_kernel_function: invh * Inorm(r invh).  r from an FP32
                             // reciprocal square root and one FP64 correction step (relative error
-                            // ~1e-14), q and the table interpolation in FP64 on FP32 table values
+                            // ~1e-14), q and the table interpolation in FP64 on FP32 table values:
+                            // FP32 q would carry ~1e-7 relative error, amplified by dlnI/dlnq
                             const double invh = w_ih[j];
                             const float d2f = (float)d2;
                             const float rs = d2f > 0.0f ? __builtin_amdgcn_rsqf(d2f) : 0.0f;
@@ -296,11 +299,12 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                             const double q = r * invh;
                             double I = 0.0;
                             if (q < Kv) {
-                                const double idxf = q * inv_dq + 1.0;
-                                int i = (int)idxf;
-                                i = i < 1 ? 1 : (i > kLutN - 1 ? kLutN - 1 : i);
-                                const double t = idxf - (double)i;
-                                I = (double)lut[i - 1] * (1.0 - t) + (double)lut[i] * t;
+                                const double x = q * inv_dq;
+                                int i = (int)x;
+                                i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
+                                const double t = x - (double)i;
+                                const double L0 = (double)lut[i];
+                                I = L0 + t * ((double)lut[i + 1] - L0);
                             }
                             const double w = w_wb[j] * (invh * I);
                             atomicAdd(&acc[kk], w);
@@ -460,6 +464,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     // normalisation within 1e-12); band 1e-11 E^2 leaves a wide margin
     f.band = 1e-11 * E * E;
     f.pad = 1e-12 * E;
+    f.sqrt_band = std::sqrt(f.band);
     // cells: about one line per cell, lines at cell centres for a regular lattice
     const double es = smax - smin, et = tmax - tmin;
     int64_t gx, gy;
@@ -479,6 +484,8 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     f.ct = gy > 1 ? et / (double)(gy - 1) : std::max(et, 1.0);
     if (!(f.cs > 0.0) || !(f.ct > 0.0)) return PMX_OK;
     f.near = (double)g_tiled_near;
+    f.inv_cs = 1.0 / f.cs;
+    f.inv_ct = 1.0 / f.ct;
     f.so = smin - 0.5 * f.cs;
     f.to = tmin - 0.5 * f.ct;
     f.gx = (int)gx;

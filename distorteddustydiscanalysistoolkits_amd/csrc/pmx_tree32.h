@@ -16,6 +16,7 @@ struct T32 {
     const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant
     const float* __restrict__ pay;         // m/rho | quant... | rho
     const double* __restrict__ w64;        // m/rho in FP64
+    const double* __restrict__ ih64;       // 1/h in FP64
     int64_t n;
     int32_t root;
     int32_t bucket;
