@@ -26,7 +26,10 @@ EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_bu
            "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
            "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
            "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings", "pmx_point_traversal_stats",
-           "pmx_line_traversal_stats", "pmx_line_neighbor_digest")
+           "pmx_line_traversal_stats", "pmx_line_neighbor_digest", "pmx_batch_create", "pmx_batch_run",
+           "pmx_batch_interpolate", "pmx_batch_tune", "pmx_batch_info", "pmx_batch_last_error",
+           "pmx_batch_destroy")
+PMX_JOB_POINTS, PMX_JOB_LINES = 0, 1
 
 
 class PmxError(RuntimeError):
@@ -79,6 +82,11 @@ class Output(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("mem", C.c_int32), ("cols", C.POINTER(C.c_void_p))]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("particles", Particles), ("kind", C.c_int32), ("strategy", C.c_int32), ("points", Points),
+                ("lines", Lines), ("catalog", Catalog), ("out", Output)]
+
+
 _lib = None
 
 
@@ -116,6 +124,15 @@ def lib():
         L.pmx_point_traversal_stats.argtypes = [vp, i32, C.POINTER(Points), vp]
         L.pmx_line_traversal_stats.argtypes = [vp, C.POINTER(Lines), vp]
         L.pmx_line_neighbor_digest.argtypes = [vp, C.POINTER(Lines), vp, C.POINTER(i32)]
+        L.pmx_batch_create.argtypes = [C.POINTER(vp), i32, i32, C.POINTER(BatchJob)]
+        L.pmx_batch_run.argtypes = [vp, i32]
+        L.pmx_batch_interpolate.argtypes = [vp]
+        L.pmx_batch_tune.argtypes = [vp]
+        L.pmx_batch_info.argtypes = [vp, vp, i32]
+        L.pmx_batch_last_error.argtypes = [vp]
+        L.pmx_batch_last_error.restype = C.c_char_p
+        L.pmx_batch_destroy.argtypes = [vp]
+        L.pmx_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -333,3 +350,79 @@ class Context:
         t = np.zeros(8)
         lib().pmx_last_timings(self._h, t.ctypes.data, 8)
         return t
+
+
+class Batch:
+    """pmx_batch: many snapshots (device particle columns + device samples), one resident LBVH
+    each, interpolated by one persistent kernel (optionally replayed as a hipGraph).
+
+    jobs: list of dicts with keys
+      cols (x, y, z, m, h, rho), quant (list), kernel, kind ("points" | "lines"),
+      strategy, samples ((qx, qy, qz) or ((ox, oy, oz), (dx, dy, dz))), scalar_slots,
+      scalar_shepard, out (list of FP32 device tensors, one per scalar)."""
+
+    def __init__(self, jobs, device: int = 0):
+        self._keep = []
+        arr = (BatchJob * len(jobs))()
+        for j, J in enumerate(jobs):
+            cols = list(J["cols"])
+            quant = list(J.get("quant", ()))
+            qp = (C.c_void_p * max(1, len(quant)))(*[_ptr(q) for q in quant])
+            part = Particles(_dtype_code(cols[0]), _mem_of(cols + quant), int(cols[0].shape[0]),
+                             *[_ptr(c) for c in cols], len(quant), C.cast(qp, C.POINTER(C.c_void_p)),
+                             int(J.get("kernel", 0)))
+            cat = catalog_struct(scalar_slots=J["scalar_slots"], scalar_shepard=J.get("scalar_shepard"))
+            outs = list(J["out"])
+            op = (C.c_void_p * max(1, len(outs)))(*[_ptr(o) for o in outs])
+            out = Output(PMX_F32, PMX_MEM_DEVICE, C.cast(op, C.POINTER(C.c_void_p)))
+            job = arr[j]
+            job.particles = part
+            job.catalog = cat
+            job.out = out
+            if J["kind"] == "points":
+                qx, qy, qz = J["samples"]
+                job.kind = PMX_JOB_POINTS
+                job.strategy = int(J.get("strategy", PMX_SYMMETRIC))
+                job.points = Points(_mem_of([qx, qy, qz]), int(qx.shape[0]), _ptr(qx), _ptr(qy), _ptr(qz))
+            else:
+                o, d = J["samples"]
+                job.kind = PMX_JOB_LINES
+                job.strategy = PMX_SCATTER
+                job.lines = Lines(_mem_of(list(o) + list(d)), int(o[0].shape[0]), *[_ptr(a) for a in list(o) + list(d)])
+            self._keep.append((qp, cat, op, J))
+        self._h = C.c_void_p()
+        rc = lib().pmx_batch_create(C.byref(self._h), int(device), len(jobs), arr)
+        if rc != PMX_OK:
+            msg = lib().pmx_batch_last_error(self._h).decode() if self._h else "pmx_batch_create"
+            self.close()
+            _raise(rc, msg)
+
+    def _check(self, rc):
+        if rc != PMX_OK:
+            _raise(rc, lib().pmx_batch_last_error(self._h).decode())
+
+    def run(self, graph: bool = False):
+        self._check(lib().pmx_batch_run(self._h, 1 if graph else 0))
+
+    def interpolate(self):
+        self._check(lib().pmx_batch_interpolate(self._h))
+
+    def tune(self):
+        self._check(lib().pmx_batch_tune(self._h))
+
+    def info(self):
+        v = np.zeros(8)
+        lib().pmx_batch_info(self._h, v.ctypes.data, 8)
+        return dict(run_ms=v[0], build_ms=v[1], interp_ms=v[2], tiles=int(v[3]), waves_per_cu=int(v[4]),
+                    bucket_points=int(v[5]), bucket_lines=int(v[6]), heavy_first=int(v[7]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -871,8 +871,13 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     const size_t esz = p->dtype == PMX_F64 ? 8 : 4;
     hipStream_t s = c->stream;
     c->tree.built = false;
+    // kBuildAsync: no events, no device-to-host copies, no host synchronisation (the build can
+    // be captured into a graph; the NaN count stays on the device, see nan_count_device)
+    const bool async = (flags & kBuildAsync) != 0;
+    if (async && (p->mem == PMX_MEM_HOST || codes_out || order_out || (flags & PMX_BUILD_WRITEBACK)))
+        return set_error(c, PMX_ERR_ARGUMENT, "asynchronous build needs device columns and no host outputs");
 
-    PMX_HIP(c, hipEventRecord(c->ev[0], s));
+    if (!async) PMX_HIP(c, hipEventRecord(c->ev[0], s));
     // ---- inputs on the device (host columns are staged) ----
     const void* src[24];
     const void* in[24] = {p->x, p->y, p->z, p->m, p->h, p->rho};
@@ -919,7 +924,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_dedup_tilemax, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
     hipLaunchKernelGGL(k_dedup_scan, dim3(1), dim3(kBlock), 0, s, tmax, ntiles);
     hipLaunchKernelGGL(k_dedup_apply, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
-    PMX_HIP(c, hipEventRecord(c->ev[1], s));
+    if (!async) PMX_HIP(c, hipEventRecord(c->ev[1], s));
     // ---- columns into Morton order ----
     PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
     double* cols64 = c->cols64.as<double>();
@@ -948,7 +953,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
                            order, n, cols64, c->leaf64.as<Leaf64>(), o32);
     }
-    PMX_HIP(c, hipEventRecord(c->ev[2], s));
+    if (!async) PMX_HIP(c, hipEventRecord(c->ev[2], s));
     // ---- radix tree, escapes, refit ----
     PMX_TRY(ensure(c, c->lr, 8 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->parent, 4 * (size_t)(2 * n + 2)));
@@ -977,6 +982,15 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
     if (n >= 2) PMX_TRY(refit_rounds(c, n, g));
     PMX_HIP(c, hipGetLastError());
+    c->tree.n = n;
+    c->tree.root = n >= 2 ? 1 : 0;
+    c->tree.ncol = p->ncol;
+    c->tree.kernel = p->kernel;
+    if (async) {
+        c->tree.built = true;
+        c->tree.f32_ready = true;
+        return PMX_OK;
+    }
     PMX_HIP(c, hipEventRecord(c->ev[3], s));
     // ---- outputs ----
     int nan_host = 0;
@@ -1021,4 +1035,9 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     return PMX_OK;
 }
 
+}  // namespace pmx
+
+namespace pmx {
+// Device address of the NaN-coordinate counter of the context's last build.
+int* nan_count_device(pmx_ctx* c) { return reinterpret_cast<int*>(c->misc.as<double>() + 6 * 1024 + 6); }
 }  // namespace pmx

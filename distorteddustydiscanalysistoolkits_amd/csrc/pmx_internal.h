@@ -100,6 +100,9 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20) {
 }
 
 // build (pmx_build.hip)
+// build_lbvh flag (internal): capture-safe build, see pmx_build.hip
+constexpr int kBuildAsync = 1 << 30;
+int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
                int64_t* order_out);
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,

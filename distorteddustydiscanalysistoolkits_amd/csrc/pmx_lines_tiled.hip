@@ -28,26 +28,14 @@
 
 #include "pmx_internal.h"
 #include "pmx_tree32.h"
+#include "pmx_lines_tile.h"
 
 namespace pmx {
 
 namespace {
 
-constexpr int kTS = 8;                 // cells per tile side
-constexpr int kCells = kTS * kTS;      // one wavefront per tile: lane = cell when loading
-constexpr int kCap = 64;               // lines per LDS chunk of a tile (a lattice tile holds 64)
-constexpr int kMaxCols = 8;            // field columns per launch
+using namespace lmap;
 
-struct Frame {
-    double u[3], v[3], d[3];
-    double so, to, cs, ct;             // cell origin and size along s and t
-    int gx, gy, tx, ty;                // cells and tiles along s and t
-    double band;                       // absolute band on d^2 around (K h)^2 -> exact re-check
-    double pad;                        // absolute padding of footprint radii
-    double sqrt_band;                  // sqrt(band): R + sqrt(band) >= sqrt(R^2 + band)
-    double inv_cs, inv_ct;             // 1 / cell size (binning and footprints use the same products)
-    double near;                       // cells: nodes whose box is this close are read whole
-};
 
 // (s, t) of each line origin, parallel check against line 0, per-block bounds.
 __global__ void __launch_bounds__(256) k_line_project(const double* __restrict__ o, const double* __restrict__ d,
@@ -82,9 +70,6 @@ __global__ void __launch_bounds__(256) k_line_project(const double* __restrict__
     if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__device__ inline int clamp_cell(double c, int g) { return c < 0.0 ? 0 : (c > (double)(g - 1) ? g - 1 : (int)c); }
-__device__ inline int cell_of(double v, double org, double inv_size, int g) { return clamp_cell(floor((v - org) * inv_size), g); }
-
 // sort key of a line: tile-major, then cell inside the tile
 __global__ void __launch_bounds__(256) k_line_keys(const double* __restrict__ st, int64_t nl, Frame f,
                                                    uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
@@ -114,27 +99,6 @@ __global__ void __launch_bounds__(256) k_line_cells(const uint64_t* __restrict__
     }
 }
 
-struct TileCat {
-    int n;
-    int fcol[kMaxCols];   // payload column of each output (1.. = quant)
-    int shep[kMaxCols];
-};
-
-// Extent of an outward-rounded FP32 box projected on a unit axis a.
-__device__ inline void project_box(const Node32& b, const double a[3], double& lo, double& hi) {
-    const double mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
-    lo = 0.0;
-    hi = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double p = a[k] * mn[k], q = a[k] * mx[k];
-        lo += fmin(p, q);
-        hi += fmax(p, q);
-    }
-}
-
-// MODE 0: interpolation (FP64 weight sums).  MODE 1: neighbour digest per line (count, sum
-// of sorted particle indices, sum of their squares; unsigned 64-bit, wrapping).
 template <int K, int MODE>
 __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat, const int32_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ perm,
@@ -143,221 +107,11 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                                                     const float* __restrict__ lut, double inv_dq,
                                                     float* __restrict__ out, int64_t ldo,
                                                     uint64_t* __restrict__ digest) {
-    // dynamic LDS: line s, t [kCap]; accumulators [nacc][kCap]; particle fields [ncol][64]
     extern __shared__ double dyn[];
-    __shared__ int32_t cs[kCells + 1];
-    // sweep state: one particle per lane, and the owner lane of each (particle, row) item
-    __shared__ double w_sb[64], w_tb[64], w_r2[64], w_wb[64], w_ih[64];
-    __shared__ int32_t w_p[64], w_cx0[64], w_cx1[64], w_cy0[64], w_start[64];
-    __shared__ unsigned char w_own[64 * kTS];
-    const int lane = threadIdx.x;
-    const int nacc = MODE == 0 ? 1 + cat.n : 3;
-    double* S = dyn;
-    double* Tt = dyn + kCap;
-    double* acc = dyn + 2 * kCap;
-    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);
-    float* w_A = reinterpret_cast<float*>(dyn + (2 + nacc) * kCap);     // [ncol][64]
-    const double Kv = support(K);
-    const int64_t n = T.n;
+    __shared__ LineTileLDS L;
     const int64_t ntiles = (int64_t)f.tx * f.ty;
-
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        __syncthreads();
-        cs[lane] = cstart[tile * kCells + lane];
-        if (lane == 0) cs[kCells] = cstart[(tile + 1) * kCells];
-        __syncthreads();
-        const int L0 = cs[0], L1 = cs[kCells];
-        if (L0 >= L1) continue;
-        const int txi = (int)(tile % f.tx), tyi = (int)(tile / f.tx);
-        const int cx_base = txi * kTS, cy_base = tyi * kTS;
-        // tile rectangle in the plane; border tiles also hold lines clamped into the grid
-        const double rs0 = cx_base == 0 ? -__builtin_inf() : f.so + (double)cx_base * f.cs;
-        const double rs1 = cx_base + kTS >= f.gx ? __builtin_inf() : f.so + (double)(cx_base + kTS) * f.cs;
-        const double rt0 = cy_base == 0 ? -__builtin_inf() : f.to + (double)cy_base * f.ct;
-        const double rt1 = cy_base + kTS >= f.gy ? __builtin_inf() : f.to + (double)(cy_base + kTS) * f.ct;
-        const double near_s = f.near * f.cs, near_t = f.near * f.ct;
-
-        for (int C0 = L0; C0 < L1; C0 += kCap) {
-            const int m = min(kCap, L1 - C0);
-            for (int k = lane; k < kCap; k += 64) {
-                if (k < m) {
-                    const double2 v = st_sorted[C0 + k];
-                    S[k] = v.x;
-                    Tt[k] = v.y;
-                }
-                for (int a = 0; a < nacc; ++a) {
-                    if (MODE == 0) acc[a * kCap + k] = 0.0;
-                    else dg[a * kCap + k] = 0ull;
-                }
-            }
-            __syncthreads();
-
-            auto need = [&](const Node32& nd) {
-                double s0, s1, t0, t1;
-                project_box(nd, f.u, s0, s1);
-                project_box(nd, f.v, t0, t1);
-                const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
-                const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
-                const double R = Kv * (double)nd.hmax;
-                const double Rp = R + f.sqrt_band + f.pad;
-                return gs * gs + gt * gt <= Rp * Rp;
-            };
-            // One sweep = up to 64 particles of a leaf run, one per lane.  Each lane finds the
-            // rows and columns of this tile its footprint covers; the (particle, row) items of the
-            // sweep are dealt round-robin to the lanes (particles with large h do not hold the
-            // other lanes idle), and a row's lines are one contiguous run of sorted lines.
-            auto sweep = [&](int64_t p, bool valid) {
-                int nrow = 0;
-                if (valid) {
-                    const float4 H = T.leaf[p];
-                    const float4 Lo = T.leaflo[p];
-                    const double x = (double)H.x + (double)Lo.x, y = (double)H.y + (double)Lo.y,
-                                 z = (double)H.z + (double)Lo.z;
-                    const double sb = (x * f.u[0] + y * f.u[1]) + z * f.u[2];
-                    const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
-                    // reject on the FP32 h (round-to-nearest, widened by 1e-6) before more loads
-                    const double Ru = Kv * (double)H.w * (1.0 + 1e-6);
-                    const double Rpu = Ru + f.sqrt_band + f.pad;
-                    if (!(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1)) {
-                        const double hb = T.cols64[4 * n + p];
-                        const double rr = Kv * hb;
-                        const double r2 = rr * rr;
-                        const double Rp = rr + f.sqrt_band + f.pad;
-                        // global cell range of the footprint, clamped like the binning, then tile-local
-                        const int gx0 = clamp_cell(floor((sb - Rp - f.so) * f.inv_cs), f.gx);
-                        const int gx1 = clamp_cell(floor((sb + Rp - f.so) * f.inv_cs), f.gx);
-                        const int gy0 = clamp_cell(floor((tb - Rp - f.to) * f.inv_ct), f.gy);
-                        const int gy1 = clamp_cell(floor((tb + Rp - f.to) * f.inv_ct), f.gy);
-                        const int cx0 = max(gx0 - cx_base, 0), cx1 = min(gx1 - cx_base, kTS - 1);
-                        const int cy0 = max(gy0 - cy_base, 0), cy1 = min(gy1 - cy_base, kTS - 1);
-                        if (cx0 <= cx1 && cy0 <= cy1) {
-                            nrow = cy1 - cy0 + 1;
-                            w_sb[lane] = sb;
-                            w_tb[lane] = tb;
-                            w_r2[lane] = r2;
-                            w_p[lane] = (int32_t)p;
-                            w_cx0[lane] = cx0;
-                            w_cx1[lane] = cx1;
-                            w_cy0[lane] = cy0;
-                            if (MODE == 0) {
-                                w_wb[lane] = T.w64[p];
-                                w_ih[lane] = T.ih64[p];
-                                for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
-                            }
-                        }
-                    }
-                }
-                // exclusive prefix of the item counts across the wave
-                int incl = nrow;
-                for (int o2 = 1; o2 < 64; o2 <<= 1) {
-                    const int u = __shfl_up(incl, o2);
-                    if (lane >= o2) incl += u;
-                }
-                const int total = __shfl(incl, 63);
-                if (total == 0) return;
-                const int excl = incl - nrow;
-                w_start[lane] = excl;
-                for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
-                __syncthreads();
-                for (int t = lane; t < total; t += 64) {
-                    const int j = w_own[t];
-                    const int row = w_cy0[j] + (t - w_start[j]);
-                    const int a = max(cs[row * kTS + w_cx0[j]], C0);
-                    const int b = min(cs[row * kTS + w_cx1[j] + 1], C0 + m);
-                    if (a >= b) continue;
-                    const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
-                    const int64_t pj = w_p[j];
-                    // lanes on the same row start at different columns (fewer LDS bank conflicts)
-                    const int len = b - a;
-                    int k = a + (lane % len);
-                    for (int it = 0; it < len; ++it, k = (k + 1 == b) ? a : k + 1) {
-                        const int kk = k - C0;
-                        const double ds = sb - S[kk], dt = tb - Tt[kk];
-                        const double d2 = ds * ds + dt * dt;
-                        if (d2 > r2 + f.band) continue;
-                        bool ok = d2 < r2 - f.band;
-                        if (!ok) {
-                            // reference formula on the FP64 columns (_squared_distance_line_coords)
-                            const int64_t li = perm[k];
-                            double D2 = 0.0, Dm = 0.0, e;
-                            e = T.cols64[pj] - o[li]; D2 += e * e; Dm += e * f.d[0];
-                            e = T.cols64[n + pj] - o[nl + li]; D2 += e * e; Dm += e * f.d[1];
-                            e = T.cols64[2 * n + pj] - o[2 * nl + li]; D2 += e * e; Dm += e * f.d[2];
-                            ok = D2 - Dm * Dm <= r2;
-                        }
-                        if (!ok) continue;
-                        if (MODE == 0) {
-                            // line_integrated_kernel_function: invh * Inorm(r invh).  r from an FP32
-                            // reciprocal square root and one FP64 correction step (relative error
-                            // ~1e-14), q and the table interpolation in FP64 on FP32 table values:
-                            // FP32 q would carry ~1e-7 relative error, amplified by dlnI/dlnq
-                            const double invh = w_ih[j];
-                            const float d2f = (float)d2;
-                            const float rs = d2f > 0.0f ? __builtin_amdgcn_rsqf(d2f) : 0.0f;
-                            const double r0 = (double)(d2f * rs);
-                            const double r = r0 + (d2 - r0 * r0) * (0.5 * (double)rs);
-                            const double q = r * invh;
-                            double I = 0.0;
-                            if (q < Kv) {
-                                const double x = q * inv_dq;
-                                int i = (int)x;
-                                i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
-                                const double t = x - (double)i;
-                                const double L0 = (double)lut[i];
-                                I = L0 + t * ((double)lut[i + 1] - L0);
-                            }
-                            const double w = w_wb[j] * (invh * I);
-                            atomicAdd(&acc[kk], w);
-                            for (int cc = 0; cc < cat.n; ++cc)
-                                atomicAdd(&acc[(1 + cc) * kCap + kk], (double)w_A[cc * 64 + j] * w);
-                        } else {
-                            const unsigned long long ip = (unsigned long long)pj;
-                            atomicAdd(&dg[kk], 1ull);
-                            atomicAdd(&dg[kCap + kk], ip);
-                            atomicAdd(&dg[2 * kCap + kk], ip * ip);
-                        }
-                    }
-                }
-                __syncthreads();
-            };
-            // runs of leaves (contiguous in Morton order), 64 per sweep
-            auto bucket = [&](int lo, int cnt) {
-                for (int base = 0; base < cnt; base += 64)
-                    sweep((int64_t)lo + base + lane, base + lane < cnt);
-            };
-            // A node is taken whole (up to T.bucket leaves) when its box itself lies near the
-            // tile; a node that reaches the tile only through a large hmax is descended down to
-            // one sweep of 64, so that only the subtrees holding the large-h particles are read.
-            auto take = [&](const Node32& nd, int cnt) {
-                if (cnt <= 64) return true;
-                if (cnt > T.bucket) return false;
-                double s0, s1, t0, t1;
-                project_box(nd, f.u, s0, s1);
-                project_box(nd, f.v, t0, t1);
-                const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
-                const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
-                return gs <= near_s && gt <= near_t;
-            };
-            walk(T, true, need, bucket, take);
-            __syncthreads();
-            for (int k = lane; k < m; k += 64) {
-                const int64_t li = perm[C0 + k];
-                if (MODE == 0) {
-                    const double S1 = acc[k];
-                    for (int c = 0; c < cat.n; ++c) {
-                        float v;
-                        if (S1 == 0.0) v = __builtin_nanf("");
-                        else v = (float)(cat.shep[c] ? acc[(1 + c) * kCap + k] / S1 : acc[(1 + c) * kCap + k]);
-                        out[(int64_t)c * ldo + li] = v;
-                    }
-                } else {
-                    digest[li] = dg[k];
-                    digest[nl + li] = dg[kCap + k];
-                    digest[2 * nl + li] = dg[2 * kCap + k];
-                }
-            }
-        }
-    }
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        lines_tile<K, MODE>(T, f, cat, cstart, perm, st_sorted, o, nl, lut, inv_dq, out, ldo, digest, tile, L, dyn);
 }
 
 template <int K, int MODE>
@@ -365,8 +119,7 @@ void launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t*
                   const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest) {
     const int64_t ntiles = (int64_t)f.tx * f.ty;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
-    const size_t lds = sizeof(double) * (size_t)kCap * (2 + (MODE == 0 ? 1 + cat.n : 3)) +
-                       sizeof(float) * 64 * (size_t)(MODE == 0 ? cat.n : 0);
+    const size_t lds = lines_tile_lds(cat.n, MODE);
     T32 T = tree32(c, 64);
     T.bucket = g_tiled_bucket;
     hipLaunchKernelGGL((k_lines_tiled<K, MODE>), dim3(grid), dim3(64), lds, c->stream, T, f, cat, cells,
@@ -545,6 +298,18 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
     }
     PMX_HIP(c, hipGetLastError());
     *used = true;
+    return PMX_OK;
+}
+
+int lines_plan(pmx_ctx* c, const double* o, const double* d, int64_t nl, lmap::LinePlan* plan, bool* ok) {
+    Binned B;
+    PMX_TRY(bin_lines(c, o, d, nl, ok, &B));
+    if (*ok) {
+        plan->f = B.f;
+        plan->cstart = B.cells;
+        plan->perm = B.perm;
+        plan->sts = B.sts;
+    }
     return PMX_OK;
 }
 

@@ -211,6 +211,42 @@ int pmx_line_kernel_table(int32_t kernel, double *Q, double *Inorm);
  * kernels), [6] last interpolation call total, [7] interpolation kernels only. */
 int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
 
+/* --- batches of snapshots ------------------------------------------------------------ */
+/* No counterpart in the reference, which interpolates one snapshot per call and rebuilds the
+ * LBVH each time (initialize_interpolation.jl:49-65).  For callers with many independent
+ * snapshots and samples of uneven size (cylindrical grids, slices, column maps): a batch keeps
+ * one LBVH per job resident, builds all of them, then runs every job's sample tiles in one
+ * persistent kernel fed by a device work queue; builds and kernel can be captured into one
+ * hipGraph and replayed, and pmx_batch_tune picks launch shapes by timing them.
+ * Restrictions: the FP32-compensated path, scalar catalogs, one SPH kernel for all jobs,
+ * particle columns / samples / outputs in device memory (FP32 outputs), line jobs must be one
+ * parallel unit-direction line set each.  Job buffers are borrowed for the batch's lifetime. */
+typedef struct pmx_batch pmx_batch;
+enum { PMX_JOB_POINTS = 0, PMX_JOB_LINES = 1 };
+typedef struct pmx_batch_job {
+    pmx_particles particles;   /* mem = PMX_MEM_DEVICE */
+    int32_t kind;              /* PMX_JOB_POINTS or PMX_JOB_LINES */
+    int32_t strategy;          /* points: PMX_GATHER / PMX_SCATTER / PMX_SYMMETRIC; lines: PMX_SCATTER */
+    pmx_points points;         /* kind PMX_JOB_POINTS; mem = PMX_MEM_DEVICE */
+    pmx_lines lines;           /* kind PMX_JOB_LINES; mem = PMX_MEM_DEVICE */
+    pmx_catalog catalog;       /* scalars only */
+    pmx_output out;            /* dtype PMX_F32, mem = PMX_MEM_DEVICE, one column per scalar */
+} pmx_batch_job;
+/* Plans the batch: one build per job (sizes every buffer), sample ordering, line binning. */
+int pmx_batch_create(pmx_batch **out, int32_t device_ordinal, int32_t njobs, const pmx_batch_job *jobs);
+/* Rebuilds every job's LBVH from its (possibly updated) columns and interpolates all jobs;
+ * use_graph = 1 replays one captured hipGraph of the whole sequence (captured on first use). */
+int pmx_batch_run(pmx_batch *b, int32_t use_graph);
+/* The persistent interpolation kernel alone, on the current trees. */
+int pmx_batch_interpolate(pmx_batch *b);
+/* Times candidate launch shapes (waves per CU, leaf-run sizes, job order) and keeps the fastest. */
+int pmx_batch_tune(pmx_batch *b);
+/* [0] last run, [1] builds of the last eager run, [2] last interpolation, [3] tiles; then the
+ * chosen waves per CU, point leaf run, line leaf run, heavy-first order. */
+int pmx_batch_info(const pmx_batch *b, double *vals, int32_t n);
+const char *pmx_batch_last_error(const pmx_batch *b);
+void pmx_batch_destroy(pmx_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
