@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--workload", default="lines", choices=["lines", "points", "batch"])
     ap.add_argument("--snapshots", type=int, default=64, help="batch workload: number of snapshots")
     ap.add_argument("--no-tune", action="store_true", help="batch workload: skip the autotuner")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch workload: skip the per-call FP32c / FP64 comparison")
     ap.add_argument("--precision", default="best", choices=["best", "f32c", "f64", "both"])
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--map", type=int, default=1024, help="lines workload: map side")
@@ -337,6 +339,8 @@ def run_batch(a, torch, dev):
     ms = 1e3 * (time.perf_counter() - t) / a.steps
     info = b.info()
     log(f"[bench] graph replay {ms:.1f} ms/step")
+    b.close()
+    compare = None if a.no_compare else per_call_compare(jobs, torch, dev)
     line = {
         "metric": "interp grid points/s + LBVH build particles/s at 16M parts, FP32c vs FP64",
         "value": P_total / (ms * 1e-3),
@@ -352,12 +356,46 @@ def run_batch(a, torch, dev):
         "build_particles_per_s": N_total / (eager["build_ms"] * 1e-3),
         "interp_points_per_s": P_total / (eager["interp_ms"] * 1e-3),
         "eager_ms": eager["run_ms"], "eager_build_ms": eager["build_ms"], "eager_interp_ms": eager["interp_ms"],
-        "tuning": {k: info[k] for k in ("waves_per_cu", "bucket_points", "bucket_lines", "heavy_first")},
+        "tuning": {k: info[k] for k in ("waves_per_cu", "bucket_points", "bucket_lines", "heavy_first",
+                                        "split_budget")},
+        "per_call": compare,
+        "speedup_vs_f64_per_call": None if compare is None else (P_total / (ms * 1e-3)) / compare["f64"]["grid_points_per_s"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))
-    b.close()
     return 0
+
+
+def per_call_compare(jobs, torch, dev):
+    """The same snapshots one call at a time on one context (largest first, after one warm-up
+    call): build + interpolation device time per job, summed, for the FP32c path and for the
+    plain FP64 path (FP64 outputs)."""
+    from distorteddustydiscanalysistoolkits_amd import _lib
+    order = sorted(range(len(jobs)), key=lambda j: -jobs[j]["cols"][0].shape[0])
+    P_total = sum(jobs[j]["out"][0].shape[0] for j in order)
+    res = {}
+    for name, prec in (("f32c", _lib.PMX_PREC_F32C), ("f64", _lib.PMX_PREC_F64)):
+        ctx = _lib.Context(0)
+        tb = ti = 0.0
+        t0 = time.time()
+        for k, j in enumerate([order[0]] + order):
+            job = jobs[j]
+            ctx.build(*job["cols"], quant=job["quant"], kernel=KERNEL_M4)
+            b_ms = ctx.timings()[0]
+            cat = _lib.catalog_struct(scalar_slots=job["scalar_slots"], scalar_shepard=job["scalar_shepard"])
+            P = job["out"][0].shape[0]
+            out = [torch.empty(P, dtype=torch.float32 if name == "f32c" else torch.float64, device=dev)]
+            if job["kind"] == "points":
+                ctx.interpolate_points(job["strategy"], prec, cat, *job["samples"], out)
+            else:
+                ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat, job["samples"][0], job["samples"][1], out)
+            if k > 0:                      # the first call is the warm-up
+                tb += b_ms
+                ti += ctx.timings()[6]
+        ctx.close()
+        log(f"[bench] per-call {name}: builds {tb:.1f} ms, interpolation {ti:.1f} ms ({time.time() - t0:.1f} s)")
+        res[name] = {"build_ms": tb, "interp_ms": ti, "grid_points_per_s": P_total / ((tb + ti) * 1e-3)}
+    return res
 
 
 def cpu_baseline(w, nsample):

@@ -347,8 +347,10 @@ class Context:
         return f, df
 
     def timings(self):
-        t = np.zeros(8)
-        lib().pmx_last_timings(self._h, t.ctypes.data, 8)
+        """Phase times (ms) of the last calls, then the split counts of the last FP32c
+        gather / symmetric point pass ([8] tiles split, [9] items queued)."""
+        t = np.zeros(10)
+        lib().pmx_last_timings(self._h, t.ctypes.data, 10)
         return t
 
 
@@ -411,10 +413,11 @@ class Batch:
         self._check(lib().pmx_batch_tune(self._h))
 
     def info(self):
-        v = np.zeros(8)
-        lib().pmx_batch_info(self._h, v.ctypes.data, 8)
+        v = np.zeros(9)
+        lib().pmx_batch_info(self._h, v.ctypes.data, 9)
         return dict(run_ms=v[0], build_ms=v[1], interp_ms=v[2], tiles=int(v[3]), waves_per_cu=int(v[4]),
-                    bucket_points=int(v[5]), bucket_lines=int(v[6]), heavy_first=int(v[7]))
+                    bucket_points=int(v[5]), bucket_lines=int(v[6]), heavy_first=int(v[7]),
+                    split_budget=int(v[8]))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -228,6 +228,7 @@ static void read_tuning() {
     if (const char* v = std::getenv("PMX_BUCKET_LINES")) g_bucket_lines = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_BUCKET")) g_tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_NEAR")) g_tiled_near = std::max(0, std::min(1 << 20, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_POINT_ORDER")) g_point_order = std::max(0, std::min(4, std::atoi(v)));
     if (const char* v = std::getenv("PMX_REFIT_SMALL")) g_refit_small = std::max(1, std::min(1024, std::atoi(v)));
 }
 
@@ -244,6 +245,12 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
     pmx_ctx* c = new (std::nothrow) pmx_ctx();
     if (!c) return PMX_ERR_NOMEM;
     c->device = device_ordinal;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) c->ncu = std::max(1, prop.multiProcessorCount);
+    }
+    if (const char* v = std::getenv("PMX_SPLIT_BUDGET")) c->split_budget = std::max(0, std::atoi(v));
+    if (const char* v = std::getenv("PMX_SPLIT_ITEM")) c->split_item = std::max(64, std::atoi(v));
     if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return PMX_ERR_HIP;
@@ -272,7 +279,7 @@ void pmx_destroy(pmx_ctx* c) {
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
-                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records};
+                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records, &c->nidx, &c->split};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : c->ev)
@@ -312,9 +319,20 @@ int pmx_interpolate_points(pmx_ctx* c, int32_t strategy, int32_t precision, cons
         PMX_TRY(prepare_f32(c));
         int32_t* perm = nullptr;
         PMX_TRY(sort_queries(c, qx, qy, qz, n, &perm));
+        const int32_t* nidx = nullptr;
+        if (strategy != kScatter && g_point_order > 0) PMX_TRY(nearest_pass_f32c(c, qx, qy, qz, n, perm, &nidx, &perm));
         PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         for (const CatChunk& ch : chunks)
-            PMX_TRY(interp_points_f32c(c, strategy, ch, perm, qx, qy, qz, n, c->outbuf.as<float>(), n));
+            PMX_TRY(interp_points_f32c(c, strategy, ch, perm, nidx, qx, qy, qz, n, c->outbuf.as<float>(), n));
+        c->timings[8] = c->timings[9] = 0.0;
+        if (strategy != kScatter && c->split_budget > 0 && c->split.p) {
+            // split counters of the last pass (queued items, slots taken), capped as the kernels cap them
+            unsigned int cnt[4] = {0, 0, 0, 0};
+            PMX_HIP(c, hipMemcpyAsync(cnt, c->split.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+            PMX_HIP(c, hipStreamSynchronize(c->stream));
+            c->timings[8] = std::min<double>(cnt[1], std::min<int64_t>(kSplitSlots, (n + 63) / 64));
+            c->timings[9] = std::min<double>(cnt[0], kSplitItems);
+        }
     } else {
         PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         // ha from the first chunk is reused by later chunks (same nearest-h search result)
@@ -631,7 +649,7 @@ int pmx_line_kernel_table(int32_t kernel, double* Q, double* Inorm) {
 
 int pmx_last_timings(const pmx_ctx* c, double* ms, int32_t n) {
     if (!c || !ms) return PMX_ERR_ARGUMENT;
-    for (int i = 0; i < n && i < 8; ++i) ms[i] = c->timings[i];
+    for (int i = 0; i < n && i < 10; ++i) ms[i] = c->timings[i];
     return PMX_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,12 +52,12 @@ struct BatchUnit {
 
 template <int K>
 __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ units, int nunits, int64_t total,
-                                              unsigned long long* __restrict__ counter) {
-    // dynamic LDS shared by the two tile kinds: points {BPos, BPay}, lines {LineTileLDS, dyn}
+                                              unsigned long long* __restrict__ counter, pts::PointSplit split) {
+    // dynamic LDS shared by the two tile kinds: points {BPos, BTile}, lines {LineTileLDS, dyn}
     extern __shared__ double lds[];
     __shared__ long long item_s;
     pts::BPos& P = *reinterpret_cast<pts::BPos*>(lds);
-    pts::BPay& Y = *reinterpret_cast<pts::BPay*>(reinterpret_cast<char*>(lds) + sizeof(pts::BPos));
+    pts::BTile<false>& Y = *reinterpret_cast<pts::BTile<false>*>(reinterpret_cast<char*>(lds) + sizeof(pts::BPos));
     lmap::LineTileLDS& L = *reinterpret_cast<lmap::LineTileLDS*>(lds);
     double* ldyn = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + sizeof(lmap::LineTileLDS));
     for (;;) {
@@ -74,13 +75,15 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
         const BatchUnit& U = units[lo];
         const int64_t t = item - U.tile0;
         if (U.kind == PMX_JOB_POINTS) {
+            pts::PointSplit sp = split;
+            sp.unit = lo;
             if (U.strategy == kGather)
-                pts::points_tile<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y);
+                pts::points_tile<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t, P, Y, sp);
             else if (U.strategy == kScatter)
-                pts::points_tile<K, kScatter, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y);
+                pts::points_tile<K, kScatter, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t, P, Y, sp);
             else
-                pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P,
-                                                       Y);
+                pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t,
+                                                       P, Y, sp);
         } else {
             lmap::lines_tile<K, 0>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut, U.inv_dq, U.out, U.ldo,
                                    nullptr, t, L, ldyn);
@@ -88,7 +91,36 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
     }
 }
 
+// Items of split point tiles of every job (after k_batch), then the split tiles' outputs.
+template <int K>
+__global__ void __launch_bounds__(64) k_batch_items(const BatchUnit* __restrict__ units, pts::PointSplit sp) {
+    __shared__ pts::BPos P;
+    __shared__ pts::BTile<false> Y;
+    const unsigned int nitems = min(sp.counters[0], (unsigned int)sp.cap_items);
+    for (;;) {
+        const unsigned int item = pts::lane0_atomic_add(&sp.counters[2], 1u);
+        if (item >= nitems) break;
+        const BatchUnit& U = units[uniform(sp.slot_unit[uniform(sp.items[item].x)])];
+        if (U.strategy == kGather)
+            pts::points_item<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
+        else
+            pts::points_item<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_batch_finish(const BatchUnit* __restrict__ units, pts::PointSplit sp) {
+    const int nslots = (int)min(sp.counters[1], (unsigned int)sp.cap_slots);
+    for (int split = blockIdx.x; split < nslots; split += gridDim.x) {
+        const BatchUnit& U = units[sp.slot_unit[split]];
+        pts::points_finish<false>(U.pcat, U.qperm, U.npts, U.out, U.ldo, sp, split);
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+__global__ void k_zero_split(unsigned int* p) {
+    if (threadIdx.x < 4) p[threadIdx.x] = 0u;
+}
 
 }  // namespace
 }  // namespace pmx
@@ -107,7 +139,9 @@ struct pmx_batch {
     std::vector<lmap::LinePlan> plan;             // per job: binned lines
     std::vector<const double*> lines_o;           // per job: packed line origins
     std::vector<BatchUnit> units;
-    DevBuf units_dev, counter;
+    DevBuf units_dev, counter, split_buf;
+    pts::PointSplit split{};                      // heavy point tiles (gather / symmetric)
+    int split_budget = 8192, split_item = 4096;
     int64_t total_tiles = 0;
     size_t lds_bytes = 0;
     int ncu = 256;
@@ -136,10 +170,47 @@ int ctx_fail(pmx_batch* b, pmx_ctx* c, int rc, int job) {
     return bfail(b, rc, "job " + std::to_string(job) + ": " + std::string(pmx_last_error(c)));
 }
 
+// Split-tile storage shared by the point jobs (gather / symmetric); budget 0 when no job
+// needs it or splitting is off.
+int make_split(pmx_batch* b) {
+    pts::PointSplit& sp = b->split;
+    sp = pts::PointSplit{};
+    bool any = false;
+    for (const pmx_batch_job& J : b->jobs) any = any || (J.kind == PMX_JOB_POINTS && J.strategy != PMX_SCATTER);
+    if (!any) return PMX_OK;
+    sp.budget = b->split_budget;
+    sp.item_max = std::max(b->split_item, 2 * b->bucket_points);
+    sp.cap_items = kSplitItems;
+    sp.cap_slots = kSplitSlots;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t b_cnt = 256, b_items = up(8 * (size_t)sp.cap_items), b_tile = up(4 * (size_t)sp.cap_slots),
+                 b_unit = up(4 * (size_t)sp.cap_slots), b_near = up(4 * (size_t)sp.cap_slots * 64),
+                 b_acc = 8 * (size_t)sp.cap_slots * pts::kAccMax * 64;
+    if (ensure(b->ctx[0], b->split_buf, b_cnt + b_items + b_tile + b_unit + b_near + b_acc) != PMX_OK)
+        return bfail(b, PMX_ERR_NOMEM, "batch: split storage allocation failed");
+    char* p = b->split_buf.as<char>();
+    sp.counters = reinterpret_cast<unsigned int*>(p);
+    p += b_cnt;
+    sp.items = reinterpret_cast<int2*>(p);
+    p += b_items;
+    sp.slot_tile = reinterpret_cast<int*>(p);
+    p += b_tile;
+    sp.slot_unit = reinterpret_cast<int*>(p);
+    p += b_unit;
+    sp.slot_near = reinterpret_cast<int*>(p);
+    p += b_near;
+    sp.slot_acc = reinterpret_cast<double*>(p);
+    return PMX_OK;
+}
+
 // Work units from the jobs with the current launch shape; tile ranges in queue order.
 int make_units(pmx_batch* b) {
     b->units.clear();
-    size_t lds = std::max<size_t>(sizeof(pts::BPos) + sizeof(pts::BPay), sizeof(lmap::LineTileLDS) + 64);
+    if (b->split.counters) {
+        b->split.budget = b->split_budget;
+        b->split.item_max = std::max(b->split_item, 2 * b->bucket_points);
+    }
+    size_t lds = std::max<size_t>(sizeof(pts::BPos) + sizeof(pts::BTile<false>), sizeof(lmap::LineTileLDS) + 64);
     std::vector<std::pair<double, BatchUnit>> list;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const pmx_batch_job& J = b->jobs[j];
@@ -212,21 +283,32 @@ int make_units(pmx_batch* b) {
     return PMX_OK;
 }
 
-// Persistent kernel plus the copies into the callers' columns (stream-ordered, capturable).
+template <int K>
+void launch_batch(pmx_batch* b, int64_t grid) {
+    hipStream_t s = b->stream;
+    const BatchUnit* U = b->units_dev.as<BatchUnit>();
+    const pts::PointSplit& sp = b->split;
+    hipLaunchKernelGGL(k_batch<K>, dim3(grid), dim3(64), b->lds_bytes, s, U, (int)b->units.size(), b->total_tiles,
+                       b->counter.as<unsigned long long>(), sp);
+    if (sp.budget <= 0) return;
+    hipLaunchKernelGGL(k_batch_items<K>, dim3(b->ncu * 32), dim3(64), 0, s, U, sp);
+    hipLaunchKernelGGL(k_batch_finish, dim3(std::min(sp.cap_slots, 4096)), dim3(64), 0, s, U, sp);
+}
+
+// Persistent kernel (then the split tiles' items and outputs) plus the copies into the
+// callers' columns (stream-ordered, capturable).
 int enqueue_interpolation(pmx_batch* b) {
     hipStream_t s = b->stream;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, b->counter.as<unsigned long long>());
+    if (b->split.budget > 0) hipLaunchKernelGGL(k_zero_split, dim3(1), dim3(64), 0, s, b->split.counters);
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(b->total_tiles, (int64_t)b->ncu * b->waves_per_cu));
-    const BatchUnit* U = b->units_dev.as<BatchUnit>();
-    const int nu = (int)b->units.size();
-    unsigned long long* ctr = b->counter.as<unsigned long long>();
     switch (b->kernel) {
-    case kM4: hipLaunchKernelGGL(k_batch<kM4>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
-    case kM5: hipLaunchKernelGGL(k_batch<kM5>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
-    case kM6: hipLaunchKernelGGL(k_batch<kM6>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
-    case kC2: hipLaunchKernelGGL(k_batch<kC2>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
-    case kC4: hipLaunchKernelGGL(k_batch<kC4>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
-    default: hipLaunchKernelGGL(k_batch<kC6>, dim3(grid), dim3(64), b->lds_bytes, s, U, nu, b->total_tiles, ctr); break;
+    case kM4: launch_batch<kM4>(b, grid); break;
+    case kM5: launch_batch<kM5>(b, grid); break;
+    case kM6: launch_batch<kM6>(b, grid); break;
+    case kC2: launch_batch<kC2>(b, grid); break;
+    case kC4: launch_batch<kC4>(b, grid); break;
+    default: launch_batch<kC6>(b, grid); break;
     }
     PMXB_HIP(b, hipGetLastError());
     for (size_t j = 0; j < b->jobs.size(); ++j) {
@@ -361,6 +443,9 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
         PMXB_HIP(b, hipStreamSynchronize(c->stream));
     }
     if (ensure(b->ctx[0], b->counter, 64) != PMX_OK) return bfail(b, PMX_ERR_NOMEM, "batch: counter allocation failed");
+    if (const char* v = std::getenv("PMX_SPLIT_BUDGET")) b->split_budget = std::max(0, std::atoi(v));
+    if (const char* v = std::getenv("PMX_SPLIT_ITEM")) b->split_item = std::max(64, std::atoi(v));
+    PMX_TRY(make_split(b));
     PMX_TRY(make_units(b));
     b->info[3] = (double)b->total_tiles;
     return PMX_OK;
@@ -429,7 +514,8 @@ int pmx_batch_tune(pmx_batch* b) {
     Knob knobs[] = {{&b->waves_per_cu, {4, 8, 16, 32}},
                     {&b->bucket_lines, {128, 256, 512, 1024}},
                     {&b->bucket_points, {8, 16, 32, 64}},
-                    {&b->heavy_first, {0, 1}}};
+                    {&b->heavy_first, {0, 1}},
+                    {&b->split_budget, {2048, 8192, 32768}}};
     double best_ms = 0;
     PMX_TRY(make_units(b));
     PMX_TRY(time_interpolation(b, &best_ms));
@@ -458,9 +544,10 @@ int pmx_batch_tune(pmx_batch* b) {
 
 int pmx_batch_info(const pmx_batch* b, double* vals, int32_t n) {
     if (!b || !vals) return PMX_ERR_ARGUMENT;
-    const double all[8] = {b->info[0], b->info[1], b->info[2], b->info[3], (double)b->waves_per_cu,
-                           (double)b->bucket_points, (double)b->bucket_lines, (double)b->heavy_first};
-    for (int i = 0; i < n && i < 8; ++i) vals[i] = all[i];
+    const double all[9] = {b->info[0], b->info[1], b->info[2], b->info[3], (double)b->waves_per_cu,
+                           (double)b->bucket_points, (double)b->bucket_lines, (double)b->heavy_first,
+                           (double)b->split_budget};
+    for (int i = 0; i < n && i < 9; ++i) vals[i] = all[i];
     return PMX_OK;
 }
 
@@ -473,6 +560,7 @@ void pmx_batch_destroy(pmx_batch* b) {
     drop_graph(b);
     if (b->units_dev.p) hipFree(b->units_dev.p);
     if (b->counter.p) hipFree(b->counter.p);
+    if (b->split_buf.p) hipFree(b->split_buf.p);
     for (pmx_ctx* c : b->ctx) pmx_destroy(c);
     for (auto& e : b->ev)
         if (e) hipEventDestroy(e);

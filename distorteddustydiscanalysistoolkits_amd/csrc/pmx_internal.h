@@ -68,10 +68,15 @@ struct pmx_ctx {
     pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
     pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
+    pmx::DevBuf nidx;          // nearest particle per sample (gather / symmetric, FP32c path)
+    int ncu = 256;             // compute units of the device
+    int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
+    int split_item = 4096;
+    pmx::DevBuf split;         // split-tile counters, item queue and slots (FP32c points)
     pmx::DevBuf records;       // particle records (AoS) for the Morton-order permutation
     pmx::DevBuf tl_st, tl_part, tl_cells;  // line-map binning (pmx_lines_tiled.hip)
     hipEvent_t ev[12] = {};
-    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double timings[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace pmx {
@@ -101,6 +106,8 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20) {
 
 // build (pmx_build.hip)
 // build_lbvh flag (internal): capture-safe build, see pmx_build.hip
+// capacities of the split-tile storage (slots of FP64 sums, queued subtree items)
+constexpr int kSplitSlots = 8192, kSplitItems = 1 << 21;
 constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
@@ -138,8 +145,10 @@ int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, con
 // Morton order of the samples (FP32-compensated path tiles); the permutation lives in the
 // context's scratch buffers until the next call that uses them.
 int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm);
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm, const double* qx,
-                       const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
+int nearest_pass_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts,
+                      const int32_t* perm, const int32_t** nidx_out, int32_t** perm_out);
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm, const int32_t* nidx,
+                       const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
 int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,
                       const double* o, const double* d, int64_t nl, float* out, int64_t ldo);
 int prepare_f32(pmx_ctx* c);
@@ -147,6 +156,7 @@ int prepare_f32(pmx_ctx* c);
 // nothing is written) when the lines do not qualify; the caller then uses the union walk.
 extern int g_lines_tiled;
 extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near, g_refit_small;
+extern int g_point_order;
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used);
 int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);

@@ -18,6 +18,7 @@ constexpr int kMaxBucket = 64;
 constexpr float kEps = 4e-6f;     // guard band of FP32 distance / radius comparisons
 constexpr int kMaxF = 9;          // distinct field columns staged per leaf (points)
 constexpr int kMaxD = 7;          // distinct derivative columns staged per leaf (1 + 3 + 3)
+constexpr int kQueue = 128;       // accepted-pair queue of a sample tile (two chunks of 64)
 
 // ---------------------------------------------------------------------------------------
 // FP32 kernel shapes: f(q) and p(q) = f'(q) / q (kernels/M4_spline.jl .. C6_Wendland.jl).
@@ -219,84 +220,113 @@ struct PointCat {
     int os, og, od, oc;
 };
 
-struct BPay {
+// LDS of one sample tile: the staged payload of a leaf run, the tile's samples (read by the
+// lanes that process their pairs), the queue of accepted (sample, particle) pairs and the
+// FP64 accumulators of every sample.
+template <bool DERIV>
+struct BTile {
+    static constexpr int kAcc = DERIV ? 20 : 3;
+    static constexpr int kRef = DERIV ? kMaxD : 1;
     float w[kMaxBucket];
     double ih64[kMaxBucket];   // 1 / h in FP64
     float s[2][kMaxBucket];
-    double d[kMaxD][kMaxBucket];
+    double d[kRef][kMaxBucket];
+    float ph[3][kWave], pl[3][kWave];
+    double invha[kWave];
+    double ref[kRef][kWave];
+    double acc[kAcc][kWave];
+    uint16_t pair[kQueue];
 };
 
-// One tile of 64 Morton-ordered samples (a wavefront): nearest-h (gather / symmetric), the
-// union walk, FP32 compensated accumulation and the FP64 finalisation.  P and Y are the
-// wavefront's LDS staging areas.
+// accumulator slots: 0 = sum W (Shepard), 1.. = scalars, then the derivative sums
+enum : int { kA_S1 = 0, kA_SC = 1, kA_B = 3, kA_GF = 6, kA_GS = 9, kA_DF = 10, kA_DS = 11, kA_CF = 14, kA_CS = 17 };
+constexpr int kAccMax = 20;
+
+// Splitting of heavy tiles (gather / symmetric).  A sample whose nearest particle has a large
+// h gathers from a huge sphere (a halo sample above a dense midplane can take 10^5 - 10^6
+// pairs), and one wavefront would walk it alone.  A tile that has tested `budget` leaves takes
+// a split slot and hands every further needed subtree of at most `item_max` particles to an
+// item queue instead of walking it; the items are walked by all wavefronts of a second
+// kernel, which add their FP64 sums into the slot, and a third kernel finalises the slot's
+// samples.  Slots or items beyond the capacities are walked in place.
+struct PointSplit {
+    int budget;                 // leaves a tile tests before it splits (0 = never split)
+    int item_max;               // largest subtree handed over as one item
+    int cap_items, cap_slots;
+    unsigned int* counters;     // [0] items queued, [1] slots taken, [2] items started
+    int2* items;                // {slot, subtree root}
+    int* slot_tile;             // slot -> tile
+    int* slot_unit;             // slot -> work unit (batches; may be null)
+    int unit;                   // work unit of the tile being walked
+    int* slot_near;             // [slot][64] nearest particle of each lane
+    double* slot_acc;           // [slot][kAccMax][64] FP64 sums
+};
+
+// The per-lane state of a tile walk and the two steps of a staged leaf run: the accept test
+// with the pair queue (run) and the kernel evaluation of 64 queued pairs (process).
 template <int K, int S, bool DERIV>
-__device__ inline void points_tile(const T32& T, const PointCat& cat, const double* __restrict__ qx,
-                                   const double* __restrict__ qy, const double* __restrict__ qz,
-                                   const int32_t* __restrict__ perm, int64_t npts, float* __restrict__ out,
-                                   int64_t ldo, int64_t tile, BPos& P, BPay& Y) {
-    const int lane = threadIdx.x;
-    const double Kv = support(K);
-    const float Kvf = (float)Kv;
-    const float sig = norm3<float>(K);
-    const int64_t n = T.n;
+struct PointWalk {
+    const T32& T;
+    const PointCat& cat;
+    BPos& P;
+    BTile<DERIV>& Y;
+    PointLane q;
+    int lane;
+    double Kv, radius2_64;
+    float Kvf, sig, radius2_32;
+    bool have_ref;
+    int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
+    int spent;                  // leaves tested so far (wave-uniform)
 
-    const int64_t slot = tile * kWave + lane;
-    const bool valid = slot < npts;
-    const int64_t qi = valid ? perm[slot] : 0;
-    const PointLane q = point_lane(qx, qy, qz, valid, qi);
+    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<DERIV>& Y_, const PointLane& q_)
+        : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
+          Kvf((float)support(K)), sig(norm3<float>(K)), radius2_32(0.0f), have_ref(false), qhead(0), qtail(0),
+          spent(0) {}
 
-    double radius2_64 = 0.0, invha64 = 0.0;
-    float radius2_32 = 0.0f, invha = 0.0f;
-    int64_t nearest = -1;
-    if (S != kScatter) {
-        nearest = nearest32(T, q, P);
-        const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * n + nearest];
-        const double radius = Kv * ha64;
-        radius2_64 = radius * radius;
-        radius2_32 = (float)radius2_64;
-        invha64 = 1.0 / ha64;
-        invha = (float)invha64;
-    }
-    const float invha3 = invha * invha * invha;
-    const float invha5 = (invha * invha) * (invha * invha) * invha;
-
-    CSum S1{0, 0}, sc[2] = {}, b[3] = {}, gf[3] = {}, gs{0, 0}, df{0, 0}, ds[3] = {}, cf[3] = {}, cs[3] = {};
-    // per-sample FP64 reference values of the derivative fields: the nearest particle
-    // (gather / symmetric) or the first accepted neighbour (scatter)
-    double refg = 0.0, refd[3] = {0.0, 0.0, 0.0}, refc[3] = {0.0, 0.0, 0.0};
-    bool have_ref = false;
-    if (DERIV && nearest >= 0) {
-        if (cat.ng) refg = T.cols64[(int64_t)cat.dcol[cat.gf] * n + nearest];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (cat.nd) refd[k] = T.cols64[(int64_t)cat.dcol[cat.df[k]] * n + nearest];
-            if (cat.nc) refc[k] = T.cols64[(int64_t)cat.dcol[cat.cf[k]] * n + nearest];
-        }
-        have_ref = true;
-    }
-
-    auto need = [&](const Node32& nd) { return node_need_point<S>(q, nd, Kvf, radius2_32); };
-    auto bucket = [&](int lo, int cnt) {
-        stage_pos(T, P, lo, cnt);
-        if (lane < cnt) {
-            const int64_t li = lo + lane;
-            Y.w[lane] = T.pay[li];
-            Y.ih64[lane] = 1.0 / T.cols64[4 * n + li];
-            for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
-            if (DERIV)
-                for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
+    // lane state in LDS; nearest < 0 for scatter (or no particle)
+    __device__ void setup(int64_t nearest) {
+        const int64_t n = T.n;
+        double invha64 = 0.0;
+        if (S != kScatter) {
+            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * n + nearest];
+            const double radius = Kv * ha64;
+            radius2_64 = radius * radius;
+            radius2_32 = (float)radius2_64;
+            invha64 = 1.0 / ha64;
         }
         __syncthreads();
-        for (int j = 0; j < cnt; ++j) {
-            float dx, dy, dz;
-            const float d2 = d2_pos32(q, P, j, dx, dy, dz);
-            if (!accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64)) continue;
-            const float invh = P.invh[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { Y.ph[k][lane] = q.ph[k]; Y.pl[k][lane] = q.pl[k]; }
+        Y.invha[lane] = invha64;
+#pragma unroll
+        for (int a = 0; a < BTile<DERIV>::kAcc; ++a) Y.acc[a][lane] = 0.0;
+        // per-sample FP64 reference values of the derivative fields: the nearest particle
+        // (gather / symmetric) or the first accepted neighbour (scatter)
+        if (DERIV && nearest >= 0) {
+            for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + nearest];
+            have_ref = true;
+        }
+    }
+
+    __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
+
+    // one pair per lane: pairs qhead .. qhead + cnt - 1 of the queue
+    __device__ void process(int cnt) {
+        __syncthreads();
+        if (lane < cnt) {
+            const int pr = Y.pair[(qhead + lane) & (kQueue - 1)];
+            const int sa = pr & (kWave - 1), j = pr >> 6;
+            const float qh0 = Y.ph[0][sa], qh1 = Y.ph[1][sa], qh2 = Y.ph[2][sa];
+            const float ql0 = Y.pl[0][sa], ql1 = Y.pl[1][sa], ql2 = Y.pl[2][sa];
+            const float dx = (qh0 - P.x[j]) + (ql0 - P.xl[j]);
+            const float dy = (qh1 - P.y[j]) + (ql1 - P.yl[j]);
+            const float dz = (qh2 - P.z[j]) + (ql2 - P.zl[j]);
             // separation in FP64 from the hi / lo parts (both differences exact in FP64)
-            const double ex = ((double)q.ph[0] - (double)P.x[j]) + ((double)q.pl[0] - (double)P.xl[j]);
-            const double ey = ((double)q.ph[1] - (double)P.y[j]) + ((double)q.pl[1] - (double)P.yl[j]);
-            const double ez = ((double)q.ph[2] - (double)P.z[j]) + ((double)q.pl[2] - (double)P.zl[j]);
+            const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
+            const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
+            const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
             const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+            const float invh = P.invh[j];
             float W, G;       // W and sigma h^-5 f'(q)/q
             {
                 const double qb = rd * Y.ih64[j];
@@ -308,94 +338,248 @@ __device__ inline void points_tile(const T32& T, const PointCat& cat, const doub
                 if (S == kScatter) {
                     W = Wb; G = Gb;
                 } else {
-                    const double qa = rd * invha64;
+                    const double ia64 = Y.invha[sa];
+                    const float ia = (float)ia64;
+                    const double qa = rd * ia64;
                     float fa, pa;
                     shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
-                    const float Wa = invha3 * (sig * fa);
-                    const float Ga = invha5 * (sig * pa);
+                    const float Wa = (ia * ia * ia) * (sig * fa);
+                    const float Ga = ((ia * ia) * (ia * ia) * ia) * (sig * pa);
                     if (S == kGather) { W = Wa; G = Ga; }
                     else { W = 0.5f * (Wa + Wb); G = 0.5f * (Ga + Gb); }
                 }
             }
             const float wb = Y.w[j];
             const float w = wb * W;
-            S1.add(w);
-            if (cat.ns > 0) sc[0].add(Y.s[0][j] * w);
-            if (cat.ns > 1) sc[1].add(Y.s[1][j] * w);
-            if (!DERIV) continue;
-            if (!have_ref) {
-                if (cat.ng) refg = Y.d[cat.gf][j];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    if (cat.nd) refd[k] = Y.d[cat.df[k]][j];
-                    if (cat.nc) refc[k] = Y.d[cat.cf[k]][j];
-                }
-                have_ref = true;
-            }
-            // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
-            const float cg = wb * G;
-            const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
-            b[0].add(gx); b[1].add(gy); b[2].add(gz);
-            if (cat.ng) {
-                const float A = (float)(Y.d[cat.gf][j] - refg);
-                gf[0].add(A * gx); gf[1].add(A * gy); gf[2].add(A * gz);
-                gs.add(A * w);
-            }
-            if (cat.nd) {
-                const float Ax = (float)(Y.d[cat.df[0]][j] - refd[0]), Ay = (float)(Y.d[cat.df[1]][j] - refd[1]),
-                            Az = (float)(Y.d[cat.df[2]][j] - refd[2]);
-                df.add(Ax * gx + Ay * gy + Az * gz);
-                ds[0].add(Ax * w); ds[1].add(Ay * w); ds[2].add(Az * w);
-            }
-            if (cat.nc) {
-                const float Ax = (float)(Y.d[cat.cf[0]][j] - refc[0]), Ay = (float)(Y.d[cat.cf[1]][j] - refc[1]),
-                            Az = (float)(Y.d[cat.cf[2]][j] - refc[2]);
-                cf[0].add(Ay * gz - Az * gy);
-                cf[1].add(Az * gx - Ax * gz);
-                cf[2].add(Ax * gy - Ay * gx);
-                cs[0].add(Ax * w); cs[1].add(Ay * w); cs[2].add(Az * w);
-            }
-        }
-        __syncthreads();
-    };
-    walk(T, ballot(valid) != 0, need, bucket);
-
-    // finalisation (general_interpolation.jl:574-668) in FP64 from the compensated sums
-    if (valid) {
-        const double s1 = S1.value();
-        if (s1 == 0.0) {
-            const float nan = __builtin_nanf("");
-            for (int j = 0; j < cat.ns; ++j) out[(cat.os + j) * ldo + qi] = nan;
-            for (int j = 0; j < 3 * cat.ng; ++j) out[(cat.og + j) * ldo + qi] = nan;
-            for (int j = 0; j < cat.nd; ++j) out[(cat.od + j) * ldo + qi] = nan;
-            for (int j = 0; j < 3 * cat.nc; ++j) out[(cat.oc + j) * ldo + qi] = nan;
-        } else {
-            const double inv = 1.0 / s1;
-            if (cat.ns > 0) out[cat.os * ldo + qi] = (float)(cat.sh[0] ? sc[0].value() * inv : sc[0].value());
-            if (cat.ns > 1) out[(cat.os + 1) * ldo + qi] = (float)(cat.sh[1] ? sc[1].value() * inv : sc[1].value());
+            atomicAdd(&Y.acc[kA_S1][sa], (double)w);
+            if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)(Y.s[0][j] * w));
+            if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)(Y.s[1][j] * w));
             if (DERIV) {
-                const double bx = b[0].value(), by = b[1].value(), bz = b[2].value();
+                // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
+                const float cg = wb * G;
+                const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
+                atomicAdd(&Y.acc[kA_B][sa], (double)gx);
+                atomicAdd(&Y.acc[kA_B + 1][sa], (double)gy);
+                atomicAdd(&Y.acc[kA_B + 2][sa], (double)gz);
                 if (cat.ng) {
-                    const double A = gs.value() * inv;
-                    out[cat.og * ldo + qi] = (float)(gf[0].value() - bx * A);
-                    out[(cat.og + 1) * ldo + qi] = (float)(gf[1].value() - by * A);
-                    out[(cat.og + 2) * ldo + qi] = (float)(gf[2].value() - bz * A);
+                    const float A = (float)(Y.d[cat.gf][j] - Y.ref[cat.gf][sa]);
+                    atomicAdd(&Y.acc[kA_GF][sa], (double)(A * gx));
+                    atomicAdd(&Y.acc[kA_GF + 1][sa], (double)(A * gy));
+                    atomicAdd(&Y.acc[kA_GF + 2][sa], (double)(A * gz));
+                    atomicAdd(&Y.acc[kA_GS][sa], (double)(A * w));
                 }
                 if (cat.nd) {
-                    const double Ax = ds[0].value() * inv, Ay = ds[1].value() * inv, Az = ds[2].value() * inv;
-                    out[cat.od * ldo + qi] = (float)(df.value() - (Ax * bx + Ay * by + Az * bz));
+                    const float Ax = (float)(Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa]);
+                    const float Ay = (float)(Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa]);
+                    const float Az = (float)(Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa]);
+                    atomicAdd(&Y.acc[kA_DF][sa], (double)(Ax * gx + Ay * gy + Az * gz));
+                    atomicAdd(&Y.acc[kA_DS][sa], (double)(Ax * w));
+                    atomicAdd(&Y.acc[kA_DS + 1][sa], (double)(Ay * w));
+                    atomicAdd(&Y.acc[kA_DS + 2][sa], (double)(Az * w));
                 }
                 if (cat.nc) {
-                    const double Ax = cs[0].value() * inv, Ay = cs[1].value() * inv, Az = cs[2].value() * inv;
-                    const double ex = Ay * bz - Az * by, ey = Az * bx - Ax * bz, ez = Ax * by - Ay * bx;
-                    out[cat.oc * ldo + qi] = (float)(-(cf[0].value() - ex));
-                    out[(cat.oc + 1) * ldo + qi] = (float)(-(cf[1].value() - ey));
-                    out[(cat.oc + 2) * ldo + qi] = (float)(-(cf[2].value() - ez));
+                    const float Ax = (float)(Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa]);
+                    const float Ay = (float)(Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa]);
+                    const float Az = (float)(Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa]);
+                    atomicAdd(&Y.acc[kA_CF][sa], (double)(Ay * gz - Az * gy));
+                    atomicAdd(&Y.acc[kA_CF + 1][sa], (double)(Az * gx - Ax * gz));
+                    atomicAdd(&Y.acc[kA_CF + 2][sa], (double)(Ax * gy - Ay * gx));
+                    atomicAdd(&Y.acc[kA_CS][sa], (double)(Ax * w));
+                    atomicAdd(&Y.acc[kA_CS + 1][sa], (double)(Ay * w));
+                    atomicAdd(&Y.acc[kA_CS + 2][sa], (double)(Az * w));
                 }
             }
         }
+        qhead += cnt;
+        __syncthreads();
     }
 
+    // a run of leaves lo .. lo+cnt-1: stage, test every particle against every lane's sample,
+    // queue the accepted pairs and process them 64 at a time
+    __device__ void run(int lo, int cnt) {
+        const int64_t n = T.n;
+        stage_pos(T, P, lo, cnt);
+        if (lane < cnt) {
+            const int64_t li = lo + lane;
+            Y.w[lane] = T.pay[li];
+            Y.ih64[lane] = T.ih64[li];
+            for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
+            if (DERIV)
+                for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
+        }
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            float dx, dy, dz;
+            const float d2 = d2_pos32(q, P, j, dx, dy, dz);
+            const bool acc = accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64);
+            const uint64_t m = ballot(acc);
+            if (!m) continue;
+            if (acc) {
+                const int at = (qtail + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
+                               (kQueue - 1);
+                Y.pair[at] = (uint16_t)(lane | (j << 6));
+                if (DERIV && !have_ref) {
+                    for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = Y.d[f][j];
+                    have_ref = true;
+                }
+            }
+            qtail += __builtin_popcountll(m);
+            if (qtail - qhead >= kWave) process(kWave);
+        }
+        if (qtail > qhead) process(qtail - qhead);
+        spent += cnt;
+    }
+};
+
+// FP64 finalisation of one sample (general_interpolation.jl:574-668) from its sums acc(a).
+template <bool DERIV, class Acc>
+__device__ inline void finalise_point(const PointCat& cat, Acc&& acc, float* __restrict__ out, int64_t ldo, int64_t qi) {
+    const double s1 = acc(kA_S1);
+    if (s1 == 0.0) {
+        const float nan = __builtin_nanf("");
+        for (int j = 0; j < cat.ns; ++j) out[(cat.os + j) * ldo + qi] = nan;
+        for (int j = 0; j < 3 * cat.ng; ++j) out[(cat.og + j) * ldo + qi] = nan;
+        for (int j = 0; j < cat.nd; ++j) out[(cat.od + j) * ldo + qi] = nan;
+        for (int j = 0; j < 3 * cat.nc; ++j) out[(cat.oc + j) * ldo + qi] = nan;
+        return;
+    }
+    const double inv = 1.0 / s1;
+    for (int j = 0; j < cat.ns; ++j) {
+        const double v = acc(kA_SC + j);
+        out[(cat.os + j) * ldo + qi] = (float)(cat.sh[j] ? v * inv : v);
+    }
+    if (!DERIV) return;
+    const double bx = acc(kA_B), by = acc(kA_B + 1), bz = acc(kA_B + 2);
+    if (cat.ng) {
+        const double A = acc(kA_GS) * inv;
+        out[cat.og * ldo + qi] = (float)(acc(kA_GF) - bx * A);
+        out[(cat.og + 1) * ldo + qi] = (float)(acc(kA_GF + 1) - by * A);
+        out[(cat.og + 2) * ldo + qi] = (float)(acc(kA_GF + 2) - bz * A);
+    }
+    if (cat.nd) {
+        const double Ax = acc(kA_DS) * inv, Ay = acc(kA_DS + 1) * inv, Az = acc(kA_DS + 2) * inv;
+        out[cat.od * ldo + qi] = (float)(acc(kA_DF) - (Ax * bx + Ay * by + Az * bz));
+    }
+    if (cat.nc) {
+        const double Ax = acc(kA_CS) * inv, Ay = acc(kA_CS + 1) * inv, Az = acc(kA_CS + 2) * inv;
+        const double ex = Ay * bz - Az * by, ey = Az * bx - Ax * bz, ez = Ax * by - Ay * bx;
+        out[cat.oc * ldo + qi] = (float)(-(acc(kA_CF) - ex));
+        out[(cat.oc + 1) * ldo + qi] = (float)(-(acc(kA_CF + 1) - ey));
+        out[(cat.oc + 2) * ldo + qi] = (float)(-(acc(kA_CF + 2) - ez));
+    }
+}
+
+__device__ inline unsigned int lane0_atomic_add(unsigned int* p, unsigned int v) {
+    unsigned int r = 0;
+    if (threadIdx.x == 0) r = atomicAdd(p, v);
+    return (unsigned int)__builtin_amdgcn_readfirstlane((int)r);
+}
+
+// One tile of 64 samples (a wavefront; perm gives the samples of tile t).  Each lane first
+// finds its nearest-h particle (gather / symmetric; or takes it from nidx).  The union walk
+// then stages leaf runs in LDS; every lane tests the run's particles against its own sample
+// (FP32 with a guard band, exact FP64 inside the band) and appends the accepted pairs to a
+// queue, and the wavefront works through the queue 64 pairs at a time, one pair per lane,
+// adding the FP32 contributions into the samples' FP64 accumulators (LDS atomics).  Only
+// accepted pairs cost kernel evaluations, however unevenly they fall on the lanes.  A heavy
+// tile may split (PointSplit); otherwise the FP64 finalisation writes the outputs.
+template <int K, int S, bool DERIV>
+__device__ inline void points_tile(const T32& T, const PointCat& cat, const double* __restrict__ qx,
+                                   const double* __restrict__ qy, const double* __restrict__ qz,
+                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nidx,
+                                   int64_t npts, float* __restrict__ out, int64_t ldo, int64_t tile, BPos& P,
+                                   BTile<DERIV>& Y, const PointSplit& sp) {
+    const int lane = threadIdx.x;
+    const int64_t slot = tile * kWave + lane;
+    const bool valid = slot < npts;
+    const int64_t qi = valid ? perm[slot] : 0;
+    const PointLane q = point_lane(qx, qy, qz, valid, qi);
+    int64_t nearest = -1;
+    if (S != kScatter) nearest = nidx ? (valid ? (int64_t)nidx[qi] : -1) : nearest32(T, q, P);
+
+    PointWalk<K, S, DERIV> W(T, cat, P, Y, q);
+    W.setup(nearest);
+    int split = -1;             // split slot; -2 once no slot was free
+    auto need = [&](const Node32& nd) { return W.need(nd); };
+    auto bucket = [&](int lo, int cnt) { W.run(lo, cnt); };
+    auto take = [&](const Node32&, int cnt) { return cnt <= T.bucket; };
+    auto emit = [&](int node, int cnt) -> bool {
+        if (S == kScatter || sp.budget <= 0 || cnt > sp.item_max || split == -2) return false;
+        if (split < 0) {
+            if (W.spent < sp.budget) return false;
+            const unsigned int s_ = lane0_atomic_add(&sp.counters[1], 1u);
+            if (s_ >= (unsigned int)sp.cap_slots) { split = -2; return false; }
+            split = (int)s_;
+        }
+        const unsigned int it = lane0_atomic_add(&sp.counters[0], 1u);
+        if (it >= (unsigned int)sp.cap_items) return false;
+        if (lane == 0) sp.items[it] = make_int2(split, node);
+        return true;
+    };
+    if (ballot(valid)) {
+        if (T.root == 0) bucket(0, (int)T.n);
+        else walk_sub(T, T.root, 0, need, bucket, take, emit);
+    }
+    __syncthreads();
+    if (split >= 0) {
+        // the items add to these sums; the finishing kernel writes the outputs
+        double* acc = sp.slot_acc + (int64_t)split * kAccMax * kWave;
+        for (int a = 0; a < BTile<DERIV>::kAcc; ++a) acc[a * kWave + lane] = Y.acc[a][lane];
+        sp.slot_near[(int64_t)split * kWave + lane] = valid ? (int)nearest : -1;
+        if (lane == 0) {
+            sp.slot_tile[split] = (int)tile;
+            if (sp.slot_unit) sp.slot_unit[split] = sp.unit;
+        }
+    } else if (valid) {
+        finalise_point<DERIV>(cat, [&](int a) { return Y.acc[a][lane]; }, out, ldo, qi);
+    }
+    __syncthreads();
+}
+
+// One queued item of a split tile: the walk of one subtree for the tile's 64 samples; the
+// sums are added into the slot (FP64 atomics).
+template <int K, int S, bool DERIV>
+__device__ inline void points_item(const T32& T, const PointCat& cat, const double* __restrict__ qx,
+                                   const double* __restrict__ qy, const double* __restrict__ qz,
+                                   const int32_t* __restrict__ perm, int64_t npts, const PointSplit& sp,
+                                   int64_t item, BPos& P, BTile<DERIV>& Y) {
+    const int lane = threadIdx.x;
+    const int2 it = sp.items[item];
+    const int split = uniform(it.x), node = uniform(it.y);
+    const int64_t tile = uniform(sp.slot_tile[split]);
+    const int64_t slot = tile * kWave + lane;
+    const bool valid = slot < npts;
+    const int64_t qi = valid ? perm[slot] : 0;
+    const PointLane q = point_lane(qx, qy, qz, valid, qi);
+    const int64_t nearest = valid ? (int64_t)sp.slot_near[(int64_t)split * kWave + lane] : -1;
+    PointWalk<K, S, DERIV> W(T, cat, P, Y, q);
+    W.setup(nearest);
+    auto need = [&](const Node32& nd) { return W.need(nd); };
+    auto bucket = [&](int lo, int cnt) { W.run(lo, cnt); };
+    auto take = [&](const Node32&, int cnt) { return cnt <= T.bucket; };
+    const int stop = uniform(T.nodes[node].escape);
+    walk_sub(T, node, stop, need, bucket, take, [](int, int) { return false; });
+    __syncthreads();
+    double* acc = sp.slot_acc + (int64_t)split * kAccMax * kWave;
+    for (int a = 0; a < BTile<DERIV>::kAcc; ++a) {
+        const double v = Y.acc[a][lane];
+        if (v != 0.0) unsafeAtomicAdd(&acc[a * kWave + lane], v);
+    }
+    __syncthreads();
+}
+
+// Outputs of one split slot from its FP64 sums.
+template <bool DERIV>
+__device__ inline void points_finish(const PointCat& cat, const int32_t* __restrict__ perm, int64_t npts,
+                                     float* __restrict__ out, int64_t ldo, const PointSplit& sp, int split) {
+    const int lane = threadIdx.x;
+    const int64_t tile = sp.slot_tile[split];
+    const int64_t slot = tile * kWave + lane;
+    if (slot >= npts) return;
+    const int64_t qi = perm[slot];
+    const double* acc = sp.slot_acc + (int64_t)split * kAccMax * kWave;
+    finalise_point<DERIV>(cat, [&](int a) { return acc[a * kWave + lane]; }, out, ldo, qi);
 }
 
 }  // namespace pts

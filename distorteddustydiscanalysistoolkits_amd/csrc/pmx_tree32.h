@@ -25,21 +25,19 @@ struct T32 {
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Union walk of the tile over left / escape links.  need(nd) is each lane's conservative
+// Union walk of the tile over left / escape links, from node `start` until the walk leaves
+// its subtree (escape link `stop`; 0 from the root).  need(nd) is each lane's conservative
 // node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
-// take(nd, cnt) says whether a needed node is handed over as one run instead of descended
-// (the default takes every run of at most T.bucket leaves).
-template <class Need, class Bucket, class Take>
-__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket, Take&& take) {
+// take(nd, cnt) says whether a needed node is handed over as one run instead of descended;
+// emit(node, cnt) may hand a needed node's whole subtree to other wavefronts instead
+// (returns true when it did; wave-uniform).
+template <class Need, class Bucket, class Take, class Emit>
+__device__ inline void walk_sub(const T32& T, int start, int stop, Need&& need, Bucket&& bucket, Take&& take,
+                                Emit&& emit) {
     const int64_t n = T.n;
-    if (!any_valid) return;
-    if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
-        bucket(0, (int)n);
-        return;
-    }
-    int node = T.root;
+    int node = start;
     int steps = 0;
-    while (node > 0 && node < 2 * n && ++steps <= 2 * n) {
+    while (node > 0 && node != stop && node < 2 * n && ++steps <= 2 * n) {
         if (node >= n) {
             const int li = node - (int)n;
             bucket(li, 1);
@@ -57,8 +55,22 @@ __device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& 
             node = uniform(nd.escape);
             continue;
         }
+        if (emit(node, cnt)) {
+            node = uniform(nd.escape);
+            continue;
+        }
         node = uniform(nd.left);
     }
+}
+
+template <class Need, class Bucket, class Take>
+__device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& bucket, Take&& take) {
+    if (!any_valid) return;
+    if (T.root == 0) {                         // n == 1: the reference brute-forces the leaf
+        bucket(0, (int)T.n);
+        return;
+    }
+    walk_sub(T, T.root, 0, need, bucket, take, [](int, int) { return false; });
 }
 
 template <class Need, class Bucket>

@@ -208,7 +208,9 @@ int pmx_line_kernel_table(int32_t kernel, double *Q, double *Inorm);
 /* Device time (ms, HIP events on the context stream) of the phases of the last build /
  * interpolation call: [0] build total, [1] encode+sort, [2] tree+refit, [3] column gather,
  * [4] nearest-h (unused: included in [5]), [5] interpolation phase (sample ordering and
- * kernels), [6] last interpolation call total, [7] interpolation kernels only. */
+ * kernels), [6] last interpolation call total, [7] interpolation kernels only; and two
+ * counts of the last FP32c gather / symmetric point pass: [8] heavy sample tiles split,
+ * [9] subtree items they queued.  n <= 10. */
 int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
 
 /* --- batches of snapshots ------------------------------------------------------------ */
@@ -239,10 +241,13 @@ int pmx_batch_create(pmx_batch **out, int32_t device_ordinal, int32_t njobs, con
 int pmx_batch_run(pmx_batch *b, int32_t use_graph);
 /* The persistent interpolation kernel alone, on the current trees. */
 int pmx_batch_interpolate(pmx_batch *b);
-/* Times candidate launch shapes (waves per CU, leaf-run sizes, job order) and keeps the fastest. */
+/* Times candidate launch shapes (waves per CU, leaf-run sizes, job order, heavy-tile split
+ * budget) and keeps the fastest. */
 int pmx_batch_tune(pmx_batch *b);
 /* [0] last run, [1] builds of the last eager run, [2] last interpolation, [3] tiles; then the
- * chosen waves per CU, point leaf run, line leaf run, heavy-first order. */
+ * chosen waves per CU, point leaf run, line leaf run, heavy-first order, split budget (leaf
+ * tests before a gather / symmetric point tile hands its remaining subtrees to all
+ * wavefronts; 0 = off).  n <= 9. */
 int pmx_batch_info(const pmx_batch *b, double *vals, int32_t n);
 const char *pmx_batch_last_error(const pmx_batch *b);
 void pmx_batch_destroy(pmx_batch *b);

@@ -135,6 +135,26 @@ def test_batch_against_oracle(batch_setup):
     assert max(rel_norms(g, ref[0])) <= F32C_TOL
 
 
+def test_batch_split_tiles(batch_setup, monkeypatch):
+    """Every gather / symmetric point tile split (16 leaf tests, items of <= 64 particles):
+    eager and graph replays agree with the per-call path to FP32 rounding."""
+    _, jobs, specs, _ = batch_setup
+    monkeypatch.setenv("PMX_SPLIT_BUDGET", "16")
+    monkeypatch.setenv("PMX_SPLIT_ITEM", "64")
+    b2 = _lib.Batch(jobs)
+    try:
+        assert b2.info()["split_budget"] == 16
+        for graph in (False, True, True):
+            for job in jobs:
+                for o in job["out"]:
+                    o.fill_(-1.0)
+            b2.run(graph=graph)
+            print(f"[batch split] graph={graph}", b2.info(), flush=True)
+            _compare_all(jobs, specs, tol=1e-6)
+    finally:
+        b2.close()
+
+
 def test_batch_rejects_bad_jobs():
     torch = pytest.importorskip("torch")
     dev = torch.device("cuda:0")

@@ -198,6 +198,35 @@ def test_f32c_points_fields(built, strategy):
         assert linf <= F32C_TOL and l2 <= F32C_TOL, f"field {j}: {linf} {l2}"
 
 
+@pytest.mark.parametrize("strategy", ["gather", "symmetric"])
+def test_f32c_points_split_tiles(built, strategy, monkeypatch):
+    """Heavy-tile splitting forced on every tile (budget of 16 leaf tests, items of <= 64
+    particles): the split path must match the unsplit path (same pairs, FP64 sums in another
+    order) and the oracle."""
+    qx, qy, qz = cyl_points(20, 20, 10)
+    cat = _lib.catalog_struct(**CAT_C2)
+    ref_split = [np.empty(qx.size) for _ in range(8)]
+    built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, cat, qx, qy, qz, ref_split)
+    monkeypatch.setenv("PMX_SPLIT_BUDGET", "16")
+    monkeypatch.setenv("PMX_SPLIT_ITEM", "64")
+    c2 = _lib.Context(0)
+    try:
+        cols, quant = disc_columns(N, seed=3, warped=True, dtype=built["dtype"])
+        c2.build(*cols, quant=quant, kernel=0)
+        outs = [np.empty(qx.size) for _ in range(8)]
+        c2.interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, cat, qx, qy, qz, outs)
+        t = c2.timings()
+        print(f"[split {strategy}] tiles split {t[8]:.0f}, items {t[9]:.0f}", flush=True)
+        assert t[8] > 0 and t[9] > t[8]
+    finally:
+        c2.close()
+    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    for j, (g, u, r) in enumerate(zip(outs, ref_split, ref)):
+        assert np.array_equal(np.isnan(g), np.isnan(u)), f"field {j}: NaN pattern differs"
+        assert rel_norms(g, u)[0] <= 1e-6, f"field {j}: split vs unsplit {rel_norms(g, u)}"
+        assert max(rel_norms(g, r)) <= F32C_TOL, f"field {j}: {rel_norms(g, r)}"
+
+
 def test_f32c_lines_fields(built):
     o, d = disc.faceon_lines(32)
     cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
