@@ -57,6 +57,7 @@ struct LineTileLDS {
     int32_t cs[kCells + 1];
     double w_sb[64], w_tb[64], w_r2[64], w_wb[64], w_ih[64];
     int32_t w_p[64], w_cx0[64], w_cx1[64], w_cy0[64], w_start[64];
+    int32_t pend[128];                  // particles that reach the tile, not yet swept
     unsigned char w_own[64 * kTS];
 };
 
@@ -245,10 +246,41 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             }
             __syncthreads();
         };
-        // runs of leaves (contiguous in Morton order), 64 per sweep
+        // Runs of leaves (contiguous in Morton order): the particles whose footprint box can
+        // reach the tile are queued, and every 64 queued particles make one sweep, so that the
+        // sweeps stay full where most of a run misses the tile.
+        int npend = 0;                 // wave-uniform
+        auto reaches = [&](int64_t p) {
+            const float4 H = T.leaf[p];
+            const float4 Lo = T.leaflo[p];
+            const double x = (double)H.x + (double)Lo.x, y = (double)H.y + (double)Lo.y,
+                         z = (double)H.z + (double)Lo.z;
+            const double sb = (x * f.u[0] + y * f.u[1]) + z * f.u[2];
+            const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
+            const double Rpu = Kv * (double)H.w * (1.0 + 1e-6) + f.sqrt_band + f.pad;
+            return !(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1);
+        };
+        auto flush = [&](int cnt) {
+            __syncthreads();
+            const int q = L.pend[lane];
+            const int rest = npend - cnt;
+            const int moved = lane < rest ? L.pend[cnt + lane] : 0;
+            __syncthreads();
+            if (lane < rest) L.pend[lane] = moved;
+            npend = rest;
+            sweep((int64_t)q, lane < cnt);
+        };
         auto bucket = [&](int lo, int cnt) {
-            for (int base = 0; base < cnt; base += 64)
-                sweep((int64_t)lo + base + lane, base + lane < cnt);
+            for (int base = 0; base < cnt; base += 64) {
+                const bool hit = base + lane < cnt && reaches((int64_t)lo + base + lane);
+                const uint64_t mk = ballot(hit);
+                if (hit)
+                    L.pend[npend + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] =
+                        lo + base + lane;
+                npend += __builtin_popcountll(mk);
+                if (npend >= 64) flush(64);
+            }
         };
         // A node is taken whole (up to T.bucket leaves) when its box itself lies near the
         // tile; a node that reaches the tile only through a large hmax is descended down to
@@ -264,6 +296,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             return gs <= near_s && gt <= near_t;
         };
         walk(T, true, need, bucket, take);
+        if (npend > 0) flush(npend);
         __syncthreads();
         for (int k = lane; k < m; k += 64) {
             const int64_t li = perm[C0 + k];
