@@ -277,8 +277,8 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
 }  // namespace
 
 int g_lines_tiled = 1;
-int g_tiled_bucket = 512;
-int g_tiled_near = 4;
+int g_tiled_bucket = 1024;
+int g_tiled_near = 16;
 
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used) {
