@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lines", choices=["lines", "points", "batch"])
+    ap.add_argument("--workload", default="lines", choices=["lines", "points", "batch", "dusty"])
     ap.add_argument("--snapshots", type=int, default=64, help="batch workload: number of snapshots")
     ap.add_argument("--no-tune", action="store_true", help="batch workload: skip the autotuner")
     ap.add_argument("--no-compare", action="store_true",
@@ -137,6 +137,8 @@ def main():
     dev = torch.device("cuda:0")
     if a.workload == "batch":
         return run_batch(a, torch, dev)
+    if a.workload == "dusty":
+        return run_dusty(a, torch, dev)
     if a.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -246,6 +248,7 @@ def main():
         "data": f"synthetic (deterministic SplitMix64 {'warped' if w.kind == 'lines' else 'flat'} disc, seed 0)",
         "config": {"workload": w.desc, "particles": n, "samples": P, "path": main_key},
         "build_particles_per_s": n / (r["build_ms"] * 1e-3),
+        "build_roofline": build_roofline(n, k, r["build_ms"]),
         "interp_points_per_s": P / (r["interp_ms"] * 1e-3),
         "build_ms": r["build_ms"],
         "interp_ms": r["interp_ms"],
@@ -366,6 +369,120 @@ def run_batch(a, torch, dev):
     return 0
 
 
+def run_dusty(a, torch, dev):
+    """Two-species dusty disc: warped gas (default 2.4e7) and dust (a third of it: H_d = 0.3 H,
+    dust-to-gas mass 0.01, v_phi x 0.995).  Per species and step: LBVH build, rho, div v and
+    curl v (itpSymmetric, M4) on a 512x512x64 cylindrical grid, and a face-on 1024^2 column
+    density map; then the dust-to-gas ratio rho_d / rho_g on the grid (device epilogue; the
+    reference has no operator for it).  FP32c timed over the steps; the plain FP64 path timed
+    over one step and used for the per-field errors."""
+    from distorteddustydiscanalysistoolkits_amd import _lib, disc
+    from tests.helpers import rel_norms
+    t0 = time.time()
+    ng = a.particles or 24_000_000
+    nd = ng // 3
+    ns, nphi, nz = (int(v) for v in (a.grid if a.grid != "256,256,64" else "512,512,64").split(","))
+    grid = disc.flatten_cylindrical(*disc.cylindrical_axes((1.0, 100.0, ns), (0.0, 2 * np.pi, nphi), (-5.0, 5.0, nz)))
+    dgrid = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in grid]
+    P = grid[0].size
+    del grid
+    lo, ld = disc.faceon_lines(a.map)
+    dlo = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in lo]
+    dld = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in ld]
+    M = lo[0].size
+    species = []
+    for name, d in (("gas", disc.make_disc(ng, seed=0, warped=True)), ("dust", disc.make_dust(nd, seed=0, warped=True))):
+        cols = [torch.from_numpy(np.ascontiguousarray(d[k], np.float32)).to(dev) for k in ("x", "y", "z", "m", "h", "rho")]
+        quant = [torch.from_numpy(np.ascontiguousarray(d[k], np.float32)).to(dev) for k in ("rho", "vx", "vy", "vz")]
+        species.append(dict(name=name, n=d["x"].size, cols=cols, quant=quant, ctx=_lib.Context(0)))
+        del d
+    log(f"[bench] dusty: gas {ng} + dust {nd} particles, {P} grid nodes + {M} lines per species, "
+        f"prepared in {time.time() - t0:.1f}s")
+    cat_p = _lib.catalog_struct(scalar_slots=[0], scalar_shepard=[0], div_slots=[1, 2, 3], curl_slots=[1, 2, 3])
+    cat_l = _lib.catalog_struct(scalar_slots=[0], scalar_shepard=[0])
+    fields = ("rho", "div_v", "curl_v_x", "curl_v_y", "curl_v_z")
+
+    def run(prec, steps, warmup):
+        ft = torch.float32 if prec == _lib.PMX_PREC_F32C else torch.float64
+        outs = [dict(p=[torch.empty(P, dtype=ft, device=dev) for _ in fields], l=[torch.empty(M, dtype=ft, device=dev)])
+                for _ in species]
+        ratio = torch.empty(P, dtype=ft, device=dev)
+
+        def step():
+            tb = ti = 0.0
+            for sp, o in zip(species, outs):
+                ctx = sp["ctx"]
+                ctx.build(*sp["cols"], quant=sp["quant"], kernel=KERNEL_M4)
+                tb += ctx.timings()[0]
+                ctx.interpolate_points(_lib.PMX_SYMMETRIC, prec, cat_p, *dgrid, o["p"])
+                ti += ctx.timings()[5]
+                ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat_l, dlo, dld, o["l"])
+                ti += ctx.timings()[5]
+            torch.div(outs[1]["p"][0], outs[0]["p"][0], out=ratio)
+            return tb, ti
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        tbs, tis = [], []
+        t = time.perf_counter()
+        for _ in range(steps):
+            tb, ti = step()
+            tbs.append(tb)
+            tis.append(ti)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t) / steps
+        return dict(ms_per_step=ms, build_ms=float(np.median(tbs)), interp_ms=float(np.median(tis)), outs=outs,
+                    ratio=ratio)
+
+    log(f"[bench] dusty: FP32c steps={a.steps} warmup={a.warmup}")
+    r32 = run(_lib.PMX_PREC_F32C, a.steps, a.warmup)
+    log(f"[bench] dusty FP32c: {r32['ms_per_step']:.1f} ms/step (builds {r32['build_ms']:.1f}, interpolation "
+        f"{r32['interp_ms']:.1f})")
+    points_per_step = len(species) * (P + M)
+    res = {"f32c": {k: r32[k] for k in ("ms_per_step", "build_ms", "interp_ms")}}
+    res["f32c"]["grid_points_per_s"] = points_per_step / (r32["ms_per_step"] * 1e-3)
+    errors = None
+    if a.precision in ("both", "best"):
+        log("[bench] dusty: plain FP64 path, one step")
+        r64 = run(_lib.PMX_PREC_F64, 1, 0)
+        log(f"[bench] dusty FP64: {r64['ms_per_step']:.1f} ms/step")
+        res["f64"] = {k: r64[k] for k in ("ms_per_step", "build_ms", "interp_ms")}
+        res["f64"]["grid_points_per_s"] = points_per_step / (r64["ms_per_step"] * 1e-3)
+        errors = {}
+        for sp, o32, o64 in zip(species, r32["outs"], r64["outs"]):
+            for nm, g, r in zip(fields, o32["p"], o64["p"]):
+                linf, l2 = rel_norms(g.double().cpu().numpy(), r.cpu().numpy())
+                errors[f"{sp['name']}_{nm}"] = {"rel_linf": linf, "rel_l2": l2}
+            linf, l2 = rel_norms(o32["l"][0].double().cpu().numpy(), o64["l"][0].cpu().numpy())
+            errors[f"{sp['name']}_rho_column"] = {"rel_linf": linf, "rel_l2": l2}
+        linf, l2 = rel_norms(r32["ratio"].double().cpu().numpy(), r64["ratio"].cpu().numpy())
+        errors["dust_to_gas"] = {"rel_linf": linf, "rel_l2": l2}
+        del r64
+    n_total = sum(sp["n"] for sp in species)
+    line = {
+        "metric": "interp grid points/s + LBVH build particles/s at 16M parts, FP32c vs FP64",
+        "value": res["f32c"]["grid_points_per_s"],
+        "unit": "grid points/s (both species' grid nodes and map lines per second, builds + interpolation per step)",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (deterministic SplitMix64 warped gas + dust discs, seed 0)",
+        "config": {"workload": f"two-species dusty disc: gas {ng} + dust {nd} particles (FP32 storage), M4; per "
+                               f"species rho, div v, curl v on a cylindrical {ns}x{nphi}x{nz} grid (itpSymmetric) "
+                               f"and a face-on {a.map}x{a.map} column-density map; dust-to-gas ratio on the grid",
+                   "particles": n_total, "samples_per_step": points_per_step},
+        "build_particles_per_s": n_total / (r32["build_ms"] * 1e-3),
+        "paths": res,
+        "speedup_vs_f64": (res["f64"]["ms_per_step"] / res["f32c"]["ms_per_step"]) if "f64" in res else None,
+        "field_error_f32c_vs_f64": errors,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    for sp in species:
+        sp["ctx"].close()
+    return 0
+
+
 def per_call_compare(jobs, torch, dev):
     """The same snapshots one call at a time on one context (largest first, after one warm-up
     call): build + interpolation device time per job, summed, for the FP32c path and for the
@@ -396,6 +513,16 @@ def per_call_compare(jobs, torch, dev):
         log(f"[bench] per-call {name}: builds {tb:.1f} ms, interpolation {ti:.1f} ms ({time.time() - t0:.1f} s)")
         res[name] = {"build_ms": tb, "interp_ms": ti, "grid_points_per_s": P_total / ((tb + ti) * 1e-3)}
     return res
+
+
+def build_roofline(n, k, build_ms, passes=8):
+    """Algorithmic HBM bytes of the LBVH build per particle (FP32 storage, k quant columns,
+    8-bit radix passes): bbox 12 + encode 24 + sort (8 + 24 P) + column gather (4 + 8 (6 + k))
+    + BRT (8 + 32) + refit 56 = 192 + 8 k + 24 P, against the 8 TB/s peak."""
+    b = 192 + 8 * k + 24 * passes
+    gbs = b * n / (build_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_particle": b, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS}
 
 
 def cpu_baseline(w, nsample):

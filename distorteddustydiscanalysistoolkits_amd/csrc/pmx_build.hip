@@ -136,10 +136,11 @@ __global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, cons
 // Stable LSD radix sort of (u64 key, i32 value) pairs, 8-bit digits.  Per pass:
 //   k_radix_hist     per-tile digit counts, stored digit-major: hist[d * ntiles + tile]
 //   k_radix_scan     one workgroup per digit: exclusive scan over tiles, digit totals
-//   k_radix_scatter  tile loaded to registers, stable in-tile ranking with wavefront
-//                    ballots, keys staged in LDS in digit order, then written out so that
+//   k_radix_scatter  tile loaded to registers (each wavefront one contiguous quarter),
+//                    stable ranking with wavefront ballots and per-wave running digit
+//                    counts, keys staged in LDS in digit order, then written out so that
 //                    consecutive lanes store consecutive addresses of each digit run.
-// Stability: items are ranked in (round, wave, lane) order = input order within a tile,
+// Stability: items are ranked in (wave, round, lane) order = input order within a tile,
 // and tiles are laid out in order inside each digit.
 // ---------------------------------------------------------------------------------
 constexpr int kRadixBits = 8;
@@ -219,31 +220,33 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
                                                           const uint32_t* __restrict__ totals,
                                                           uint64_t* __restrict__ keys_out,
                                                           int32_t* __restrict__ vals_out) {
+    constexpr int kPerWave = kTile / kWaves;   // each wavefront ranks one contiguous quarter
     __shared__ uint64_t s_keys[kTile];
     __shared__ int32_t s_vals[kTile];
     __shared__ uint32_t s_gofs[kBins];     // global destination of this tile's digit run
     __shared__ uint32_t s_lstart[kBins];   // start of the digit run inside the tile
-    __shared__ uint32_t s_run[kBins];      // items of the digit already placed (rounds so far)
-    __shared__ uint32_t s_wc[kWaves][kBins];
+    __shared__ uint32_t s_wc[kWaves][kBins];   // per-wave digit counts, then per-wave run bases
     __shared__ uint32_t s_tmp[kWaves];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int64_t wbase = base + (int64_t)wave * kPerWave;
 
     // global offsets: digit base (scan of totals) + this tile's prefix within the digit
     uint32_t tot = totals[t], dbase;
     block_exclusive_scan(tot, s_tmp, dbase);
     s_gofs[t] = dbase + hist[(int64_t)t * ntiles + blockIdx.x];
-    s_run[t] = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) s_wc[w][t] = 0;
+    __syncthreads();
 
     uint64_t k[kItems];
     int32_t v[kItems];
     int dg[kItems];
+    uint32_t rk[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        int64_t i = base + r * kBlock + t;
+        const int64_t i = wbase + r * 64 + lane;
         if (i < n) {
             k[r] = keys_in[i];
             v[r] = vals_in[i];
@@ -254,19 +257,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
             dg[r] = -1;
         }
     }
-    // in-tile digit counts -> run starts
-    __shared__ uint32_t s_cnt[kBins];
-    s_cnt[t] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kItems; ++r)
-        if (dg[r] >= 0) atomicAdd(&s_cnt[dg[r]], 1u);
-    __syncthreads();
-    uint32_t lst;
-    block_exclusive_scan(s_cnt[t], s_tmp, lst);
-    s_lstart[t] = lst;
-    __syncthreads();
-
+    // rank inside the wavefront's quarter: lanes of equal digit found by ballots, a running
+    // count per digit kept by the wavefront (its own LDS row, no block barriers)
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
@@ -274,31 +266,45 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
         uint64_t peers = __ballot(d >= 0);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
-            uint64_t bal = __ballot(d >= 0 && ((d >> b) & 1));
+            const uint64_t bal = __ballot(d >= 0 && ((d >> b) & 1));
             peers &= ((d >> b) & 1) ? bal : ~bal;
         }
-        const uint32_t rank = __popcll(peers & lt_mask);
-        if (d >= 0 && rank == 0) s_wc[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (d >= 0) {
-            uint32_t pos = s_run[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += s_wc[w][d];
-            const uint32_t slot = s_lstart[d] + pos;
-            s_keys[slot] = k[r];
-            s_vals[slot] = v[r];
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) { add += s_wc[w][t]; s_wc[w][t] = 0; }
-        s_run[t] += add;
-        __syncthreads();
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        uint32_t run = 0;
+        if (d >= 0) run = s_wc[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (d >= 0 && before == 0) s_wc[wave][d] = run + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rk[r] = run + before;
     }
+    __syncthreads();
+    // digit t: bases of the four quarters inside the digit's run, run starts inside the tile
+    uint32_t cnt = 0, wb[kWaves];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) { wb[w] = cnt; cnt += s_wc[w][t]; }
+    uint32_t lst;
+    block_exclusive_scan(cnt, s_tmp, lst);
+    s_lstart[t] = lst;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_wc[w][t] = lst + wb[w];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const int d = dg[r];
+        if (d >= 0) {
+            const uint32_t slot = s_wc[wave][d] + rk[r];
+            if (slot < (uint32_t)kTile) {
+                s_keys[slot] = k[r];
+                s_vals[slot] = v[r];
+            }
+        }
+    }
+    __syncthreads();
     const int64_t count = (n - base) < kTile ? (n - base) : kTile;
     for (int j = t; j < count; j += kBlock) {
-        uint64_t key = s_keys[j];
-        int d = (int)((key >> shift) & (kBins - 1));
-        uint32_t dst = s_gofs[d] + (uint32_t)(j - s_lstart[d]);
+        const uint64_t key = s_keys[j];
+        const int d = (int)((key >> shift) & (kBins - 1));
+        const uint32_t dst = s_gofs[d] + (uint32_t)(j - s_lstart[d]);
         if (dst < (uint64_t)n) {   // always true for a consistent histogram; keeps stores in bounds
             keys_out[dst] = key;
             vals_out[dst] = s_vals[j];
@@ -355,32 +361,45 @@ __global__ void __launch_bounds__(kBlock) k_dedup_scan(int64_t* __restrict__ tma
 
 __global__ void __launch_bounds__(kBlock) k_dedup_apply(uint64_t* __restrict__ c, int64_t n,
                                                         const int64_t* __restrict__ tpre) {
-    __shared__ int64_t s[kBlock];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
-    int64_t d[kItems];
-    int64_t m = INT64_MIN;
+    // Four rounds of kBlock x 4 consecutive codes (coalesced): per-thread running max, a
+    // wavefront max-scan, then the wavefronts combined with the carry of earlier rounds.
+    constexpr int kVec = 4;
+    __shared__ int64_t s_w[kWaves];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int64_t carry = tpre[blockIdx.x];
+    for (int r = 0; r < kItems / kVec; ++r) {
+        const int64_t i0 = (int64_t)blockIdx.x * kTile + (int64_t)r * kBlock * kVec + (int64_t)t * kVec;
+        int64_t d[kVec], loc = INT64_MIN;
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        int64_t i = base + r;
-        d[r] = i < n ? (int64_t)c[i] - i : INT64_MIN;
-        m = d[r] > m ? d[r] : m;
-    }
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-        int64_t t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : INT64_MIN;
-        __syncthreads();
-        if (t > s[threadIdx.x]) s[threadIdx.x] = t;
-        __syncthreads();
-    }
-    int64_t run = threadIdx.x == 0 ? INT64_MIN : s[threadIdx.x - 1];
-    int64_t pre = tpre[blockIdx.x];
-    run = pre > run ? pre : run;
+        for (int q = 0; q < kVec; ++q) {
+            d[q] = i0 + q < n ? (int64_t)c[i0 + q] - (i0 + q) : INT64_MIN;
+            loc = d[q] > loc ? d[q] : loc;
+        }
+        int64_t inc = loc;
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        int64_t i = base + r;
-        run = d[r] > run ? d[r] : run;
-        if (i < n) c[i] = (uint64_t)(run + i);
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t u = __shfl_up(inc, o, 64);
+            if (lane >= o && u > inc) inc = u;
+        }
+        int64_t excl = __shfl_up(inc, 1, 64);
+        if (lane == 0) excl = INT64_MIN;
+        if (lane == 63) s_w[wave] = inc;
+        __syncthreads();
+        int64_t run = carry, all = carry;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const int64_t v = s_w[w];
+            if (w < wave && v > run) run = v;
+            if (v > all) all = v;
+        }
+        __syncthreads();
+        run = excl > run ? excl : run;
+#pragma unroll
+        for (int q = 0; q < kVec; ++q) {
+            run = d[q] > run ? d[q] : run;
+            if (i0 + q < n) c[i0 + q] = (uint64_t)(run + i0 + q);
+        }
+        carry = all;
     }
 }
 
@@ -672,11 +691,21 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
                                                         uint32_t* __restrict__ visited, Node64* nodes,
                                                         Node32* __restrict__ n32, int small_range,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+    // Internal node i covers leaves with one end at i - 1 or i, so the block's nodes read a
+    // window of leaves around its own ids: staged once in LDS, then read by every node.
+    constexpr int kPad = 64;
+    __shared__ Leaf64 win[kBlock + 2 * kPad + 1];    // leaves i0 - 1 - kPad .. i0 + kBlock + kPad - 1
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                            // internal nodes 1 .. n-1
     const int64_t iters = (m + stride - 1) / stride;    // uniform trip count keeps ballots convergent
     for (int64_t it = 0; it < iters; ++it) {
-        const int64_t i = 1 + it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        const int64_t i0 = 1 + it * stride + (int64_t)blockIdx.x * kBlock;
+        const int64_t w0 = i0 - 1 - kPad > 0 ? i0 - 1 - kPad : 0;
+        const int64_t w1 = i0 + kBlock + kPad < n ? i0 + kBlock + kPad : n;
+        __syncthreads();
+        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) win[k - w0] = leaf[k];
+        __syncthreads();
+        const int64_t i = i0 + threadIdx.x;
         bool take = false;
         if (i < n) {
             const int32_t lo = range_lo[i], hi = range_hi[i];
@@ -684,8 +713,9 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
                 double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
                 double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
                 double hm = -__builtin_inf();
+                const bool staged = lo - 1 >= w0 && hi <= w1;
                 for (int32_t k = lo - 1; k < hi; ++k) {
-                    const Leaf64 L = leaf[k];
+                    const Leaf64 L = staged ? win[k - w0] : leaf[k];
                     mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
                     mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
                     mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];

@@ -71,6 +71,58 @@ __device__ inline void shape_fp32(float q, float e, float& f, float& p) {
     }
 }
 
+// The same shapes in FP64 for derivative fields: their sums cancel (a nearly solenoidal
+// velocity field has a divergence far below its individual pair terms), so the pair terms
+// are formed in FP64.  1/q from the FP32 reciprocal and two FP64 Newton steps.
+__device__ inline double rcp64(double x) {
+    double y = (double)__builtin_amdgcn_rcpf((float)x);
+    y = y * (2.0 - x * y);
+    return y * (2.0 - x * y);
+}
+
+template <int K>
+__device__ inline void shape_fp64(double q, double e, double& f, double& p) {
+    const double rq = q > 0.0 ? rcp64(q) : 0.0;
+    if (K == kM4) {
+        const double a = e, b = 1.0 - q;
+        if (q < 1.0) { f = 0.25 * (a * a * a) - b * b * b; p = -3.0 + 2.25 * q; }
+        else if (q < 2.0) { f = 0.25 * (a * a * a); p = (-0.75 * (a * a)) * rq; }
+        else { f = 0.0; p = 0.0; }
+    } else if (K == kM5) {
+        const double a = e, b = 1.5 - q, c = 0.5 - q;
+        const double a3 = a * a * a, b3 = b * b * b, c3 = c * c * c;
+        if (q < 0.5) { f = (a3 * a - 5.0 * (b3 * b)) + 10.0 * (c3 * c); p = -30.0 + 24.0 * (q * q); }
+        else if (q < 1.5) { f = a3 * a - 5.0 * (b3 * b); p = (-4.0 * a3 + 20.0 * b3) * rq; }
+        else if (q < 2.5) { f = a3 * a; p = (-4.0 * a3) * rq; }
+        else { f = 0.0; p = 0.0; }
+    } else if (K == kM6) {
+        const double a = e, b = 2.0 - q, c = 1.0 - q;
+        const double a4 = (a * a) * (a * a), b4 = (b * b) * (b * b), c4 = (c * c) * (c * c);
+        if (q < 1.0) { f = (a4 * a - 6.0 * (b4 * b)) + 15.0 * (c4 * c); p = -120.0 + (q * q) * (120.0 - 50.0 * q); }
+        else if (q < 2.0) { f = a4 * a - 6.0 * (b4 * b); p = (-5.0 * a4 + 30.0 * b4) * rq; }
+        else if (q < 3.0) { f = a4 * a; p = (-5.0 * a4) * rq; }
+        else { f = 0.0; p = 0.0; }
+    } else if (K == kC2) {
+        if (q < 2.0) {
+            const double u = 0.5 * e, u3 = u * u * u;
+            f = (u3 * u) * (2.0 * q + 1.0);
+            p = -5.0 * u3;
+        } else { f = 0.0; p = 0.0; }
+    } else if (K == kC4) {
+        if (q < 2.0) {
+            const double u = 0.5 * e, u5 = (u * u) * (u * u) * u;
+            f = (u5 * u) * (((35.0 / 12.0) * (q * q) + 3.0 * q) + 1.0);
+            p = (-14.0 / 3.0) * u5 * (1.0 + 2.5 * q);
+        } else { f = 0.0; p = 0.0; }
+    } else {
+        if (q < 2.0) {
+            const double u = 0.5 * e, u7 = (u * u) * (u * u) * (u * u) * u;
+            f = (u7 * u) * (((4.0 * (q * q * q) + 6.25 * (q * q)) + 4.0 * q) + 1.0);
+            p = -5.5 * u7 * ((4.0 * q + 3.5) * q + 1.0);
+        } else { f = 0.0; p = 0.0; }
+    }
+}
+
 // Compensated accumulator (TwoSum; -ffp-contract=off keeps it exact).
 struct CSum {
     float s, c;
@@ -228,6 +280,7 @@ struct BTile {
     static constexpr int kAcc = DERIV ? 20 : 3;
     static constexpr int kRef = DERIV ? kMaxD : 1;
     float w[kMaxBucket];
+    double w64[DERIV ? kMaxBucket : 1];   // m / rho in FP64 (derivative launches)
     double ih64[kMaxBucket];   // 1 / h in FP64
     float s[2][kMaxBucket];
     double d[kRef][kMaxBucket];
@@ -318,75 +371,100 @@ struct PointWalk {
             const int sa = pr & (kWave - 1), j = pr >> 6;
             const float qh0 = Y.ph[0][sa], qh1 = Y.ph[1][sa], qh2 = Y.ph[2][sa];
             const float ql0 = Y.pl[0][sa], ql1 = Y.pl[1][sa], ql2 = Y.pl[2][sa];
-            const float dx = (qh0 - P.x[j]) + (ql0 - P.xl[j]);
-            const float dy = (qh1 - P.y[j]) + (ql1 - P.yl[j]);
-            const float dz = (qh2 - P.z[j]) + (ql2 - P.zl[j]);
             // separation in FP64 from the hi / lo parts (both differences exact in FP64)
             const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
             const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
             const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
             const double rd = sqrt(ex * ex + ey * ey + ez * ez);
-            const float invh = P.invh[j];
-            float W, G;       // W and sigma h^-5 f'(q)/q
-            {
-                const double qb = rd * Y.ih64[j];
-                float fb, pb;
-                shape_fp32<K>((float)qb, (float)(Kv - qb), fb, pb);
-                const float ib2 = invh * invh;
-                const float Wb = (ib2 * invh) * (sig * fb);
-                const float Gb = ((ib2 * ib2) * invh) * (sig * pb);
-                if (S == kScatter) {
-                    W = Wb; G = Gb;
-                } else {
-                    const double ia64 = Y.invha[sa];
-                    const float ia = (float)ia64;
-                    const double qa = rd * ia64;
-                    float fa, pa;
-                    shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
-                    const float Wa = (ia * ia * ia) * (sig * fa);
-                    const float Ga = ((ia * ia) * (ia * ia) * ia) * (sig * pa);
-                    if (S == kGather) { W = Wa; G = Ga; }
-                    else { W = 0.5f * (Wa + Wb); G = 0.5f * (Ga + Gb); }
+            if (!DERIV) {
+                const float invh = P.invh[j];
+                float W;
+                {
+                    const double qb = rd * Y.ih64[j];
+                    float fb, pb;
+                    shape_fp32<K>((float)qb, (float)(Kv - qb), fb, pb);
+                    const float Wb = ((invh * invh) * invh) * (sig * fb);
+                    if (S == kScatter) {
+                        W = Wb;
+                    } else {
+                        const double ia64 = Y.invha[sa];
+                        const float ia = (float)ia64;
+                        const double qa = rd * ia64;
+                        float fa, pa;
+                        shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
+                        const float Wa = (ia * ia * ia) * (sig * fa);
+                        W = S == kGather ? Wa : 0.5f * (Wa + Wb);
+                    }
                 }
-            }
-            const float wb = Y.w[j];
-            const float w = wb * W;
-            atomicAdd(&Y.acc[kA_S1][sa], (double)w);
-            if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)(Y.s[0][j] * w));
-            if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)(Y.s[1][j] * w));
-            if (DERIV) {
+                const float w = Y.w[j] * W;
+                atomicAdd(&Y.acc[kA_S1][sa], (double)w);
+                if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)(Y.s[0][j] * w));
+                if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)(Y.s[1][j] * w));
+            } else {
+                // FP64 pair terms: W, sigma h^-5 f'(q)/q, the separation and the field
+                // differences (the derivative sums cancel; FP32 terms left 4e-6 on a
+                // near-solenoidal velocity field)
+                const double sig64 = norm3<double>(K);
+                double W, G;
+                {
+                    const double ihb = Y.ih64[j];
+                    const double qb = rd * ihb;
+                    double fb, pb;
+                    shape_fp64<K>(qb, Kv - qb, fb, pb);
+                    const double ib2 = ihb * ihb;
+                    const double Wb = (ib2 * ihb) * (sig64 * fb);
+                    const double Gb = ((ib2 * ib2) * ihb) * (sig64 * pb);
+                    if (S == kScatter) {
+                        W = Wb; G = Gb;
+                    } else {
+                        const double ia = Y.invha[sa];
+                        const double qa = rd * ia;
+                        double fa, pa;
+                        shape_fp64<K>(qa, Kv - qa, fa, pa);
+                        const double ia2 = ia * ia;
+                        const double Wa = (ia2 * ia) * (sig64 * fa);
+                        const double Ga = ((ia2 * ia2) * ia) * (sig64 * pa);
+                        if (S == kGather) { W = Wa; G = Ga; }
+                        else { W = 0.5 * (Wa + Wb); G = 0.5 * (Ga + Gb); }
+                    }
+                }
+                const double wb = Y.w64[j];
+                const double w = wb * W;
+                atomicAdd(&Y.acc[kA_S1][sa], w);
+                if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)Y.s[0][j] * w);
+                if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)Y.s[1][j] * w);
                 // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
-                const float cg = wb * G;
-                const float gx = cg * dx, gy = cg * dy, gz = cg * dz;
-                atomicAdd(&Y.acc[kA_B][sa], (double)gx);
-                atomicAdd(&Y.acc[kA_B + 1][sa], (double)gy);
-                atomicAdd(&Y.acc[kA_B + 2][sa], (double)gz);
+                const double cg = wb * G;
+                const double gx = cg * ex, gy = cg * ey, gz = cg * ez;
+                atomicAdd(&Y.acc[kA_B][sa], gx);
+                atomicAdd(&Y.acc[kA_B + 1][sa], gy);
+                atomicAdd(&Y.acc[kA_B + 2][sa], gz);
                 if (cat.ng) {
-                    const float A = (float)(Y.d[cat.gf][j] - Y.ref[cat.gf][sa]);
-                    atomicAdd(&Y.acc[kA_GF][sa], (double)(A * gx));
-                    atomicAdd(&Y.acc[kA_GF + 1][sa], (double)(A * gy));
-                    atomicAdd(&Y.acc[kA_GF + 2][sa], (double)(A * gz));
-                    atomicAdd(&Y.acc[kA_GS][sa], (double)(A * w));
+                    const double A = Y.d[cat.gf][j] - Y.ref[cat.gf][sa];
+                    atomicAdd(&Y.acc[kA_GF][sa], A * gx);
+                    atomicAdd(&Y.acc[kA_GF + 1][sa], A * gy);
+                    atomicAdd(&Y.acc[kA_GF + 2][sa], A * gz);
+                    atomicAdd(&Y.acc[kA_GS][sa], A * w);
                 }
                 if (cat.nd) {
-                    const float Ax = (float)(Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa]);
-                    const float Ay = (float)(Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa]);
-                    const float Az = (float)(Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa]);
-                    atomicAdd(&Y.acc[kA_DF][sa], (double)(Ax * gx + Ay * gy + Az * gz));
-                    atomicAdd(&Y.acc[kA_DS][sa], (double)(Ax * w));
-                    atomicAdd(&Y.acc[kA_DS + 1][sa], (double)(Ay * w));
-                    atomicAdd(&Y.acc[kA_DS + 2][sa], (double)(Az * w));
+                    const double Ax = Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa];
+                    const double Ay = Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa];
+                    const double Az = Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa];
+                    atomicAdd(&Y.acc[kA_DF][sa], (Ax * gx + Ay * gy) + Az * gz);
+                    atomicAdd(&Y.acc[kA_DS][sa], Ax * w);
+                    atomicAdd(&Y.acc[kA_DS + 1][sa], Ay * w);
+                    atomicAdd(&Y.acc[kA_DS + 2][sa], Az * w);
                 }
                 if (cat.nc) {
-                    const float Ax = (float)(Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa]);
-                    const float Ay = (float)(Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa]);
-                    const float Az = (float)(Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa]);
-                    atomicAdd(&Y.acc[kA_CF][sa], (double)(Ay * gz - Az * gy));
-                    atomicAdd(&Y.acc[kA_CF + 1][sa], (double)(Az * gx - Ax * gz));
-                    atomicAdd(&Y.acc[kA_CF + 2][sa], (double)(Ax * gy - Ay * gx));
-                    atomicAdd(&Y.acc[kA_CS][sa], (double)(Ax * w));
-                    atomicAdd(&Y.acc[kA_CS + 1][sa], (double)(Ay * w));
-                    atomicAdd(&Y.acc[kA_CS + 2][sa], (double)(Az * w));
+                    const double Ax = Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa];
+                    const double Ay = Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa];
+                    const double Az = Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa];
+                    atomicAdd(&Y.acc[kA_CF][sa], Ay * gz - Az * gy);
+                    atomicAdd(&Y.acc[kA_CF + 1][sa], Az * gx - Ax * gz);
+                    atomicAdd(&Y.acc[kA_CF + 2][sa], Ax * gy - Ay * gx);
+                    atomicAdd(&Y.acc[kA_CS][sa], Ax * w);
+                    atomicAdd(&Y.acc[kA_CS + 1][sa], Ay * w);
+                    atomicAdd(&Y.acc[kA_CS + 2][sa], Az * w);
                 }
             }
         }
@@ -402,6 +480,7 @@ struct PointWalk {
         if (lane < cnt) {
             const int64_t li = lo + lane;
             Y.w[lane] = T.pay[li];
+            if (DERIV) Y.w64[lane] = T.w64[li];
             Y.ih64[lane] = T.ih64[li];
             for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
             if (DERIV)
