@@ -258,6 +258,10 @@ def main():
                        "grid_points_per_s": P / (v["ms_per_step"] * 1e-3),
                        "interp_points_per_s": P / (v["interp_ms"] * 1e-3),
                        "build_particles_per_s": n / (v["build_ms"] * 1e-3)} for kk, v in results.items()},
+        "speedup_f32c_vs_f64": ({"step": results["f64"]["ms_per_step"] / results["f32c"]["ms_per_step"],
+                                 "interpolation": results["f64"]["interp_ms"] / results["f32c"]["interp_ms"],
+                                 "kernel": results["f64"]["kernel_ms"] / results["f32c"]["kernel_ms"]}
+                                if "f32c" in results and "f64" in results else None),
         "field_error_f32c_vs_f64": errors or None,
         "traversal": {"nearest_internal": n_near_int, "nearest_leaves": n_near_leaf, "internal_tested": n_int,
                       "leaves_tested": n_leaf, "leaves_accepted": n_acc},

@@ -434,8 +434,7 @@ struct Leaf32Out {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__ rec, int ncols,
                                                            const int32_t* __restrict__ order, int64_t n,
-                                                           double* __restrict__ cols64, Leaf64* __restrict__ leaf,
-                                                           Leaf32Out o32) {
+                                                           double* __restrict__ cols64, Leaf32Out o32) {
     const int ncol = ncols - 6;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         int32_t o = order[i];
@@ -452,12 +451,6 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
             cols64[(int64_t)c * n + i] = q;
             o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
         }
-        Leaf64 L;
-        L.x = v[0];
-        L.y = v[1];
-        L.z = v[2];
-        L.h = v[4];
-        leaf[i] = L;
         const float xh = (float)v[0], yh = (float)v[1], zh = (float)v[2];
         o32.leaf[i] = make_float4(xh, yh, zh, (float)v[4]);
         o32.leaflo[i] = make_float4((float)(v[0] - (double)xh), (float)(v[1] - (double)yh),
@@ -613,23 +606,25 @@ __device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __r
     n32[i] = b;
 }
 
-__device__ inline void combine_node(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+__device__ inline void combine_node(const double* __restrict__ cols64, const int32_t* __restrict__ lr,
                                     const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p,
                                     const int32_t* __restrict__ rlo, const int32_t* __restrict__ rhi,
                                     Node32* __restrict__ n32) {
     const int32_t l = lr[2 * p], r = lr[2 * p + 1];
     double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
     if (l >= n) {
-        const Leaf64 L = leaf[l - n];
-        lmn[0] = lmx[0] = L.x; lmn[1] = lmx[1] = L.y; lmn[2] = lmx[2] = L.z; hl = L.h;
+        const int64_t li = l - n;
+        lmn[0] = lmx[0] = cols64[li]; lmn[1] = lmx[1] = cols64[n + li]; lmn[2] = lmx[2] = cols64[2 * n + li];
+        hl = cols64[4 * n + li];
     } else {
         const Node64 N = nodes[l];
         for (int k = 0; k < 3; ++k) { lmn[k] = N.mn[k]; lmx[k] = N.mx[k]; }
         hl = N.hmax;
     }
     if (r >= n) {
-        const Leaf64 R = leaf[r - n];
-        rmn[0] = rmx[0] = R.x; rmn[1] = rmx[1] = R.y; rmn[2] = rmx[2] = R.z; hr = R.h;
+        const int64_t ri = r - n;
+        rmn[0] = rmx[0] = cols64[ri]; rmn[1] = rmx[1] = cols64[n + ri]; rmn[2] = rmx[2] = cols64[2 * n + ri];
+        hr = cols64[4 * n + ri];
     } else {
         const Node64 N = nodes[r];
         for (int k = 0; k < 3; ++k) { rmn[k] = N.mn[k]; rmx[k] = N.mx[k]; }
@@ -684,7 +679,7 @@ __device__ inline void block_append(bool take, int32_t p, int32_t* __restrict__ 
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict__ leaf, const int32_t* __restrict__ lr,
+__global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
                                                         const int32_t* __restrict__ range_lo,
                                                         const int32_t* __restrict__ range_hi, int64_t n,
@@ -703,7 +698,14 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
         const int64_t w0 = i0 - 1 - kPad > 0 ? i0 - 1 - kPad : 0;
         const int64_t w1 = i0 + kBlock + kPad < n ? i0 + kBlock + kPad : n;
         __syncthreads();
-        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) win[k - w0] = leaf[k];
+        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) {
+            Leaf64 L;
+            L.x = cols64[k];
+            L.y = cols64[n + k];
+            L.z = cols64[2 * n + k];
+            L.h = cols64[4 * n + k];
+            win[k - w0] = L;
+        }
         __syncthreads();
         const int64_t i = i0 + threadIdx.x;
         bool take = false;
@@ -715,7 +717,15 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
                 double hm = -__builtin_inf();
                 const bool staged = lo - 1 >= w0 && hi <= w1;
                 for (int32_t k = lo - 1; k < hi; ++k) {
-                    const Leaf64 L = staged ? win[k - w0] : leaf[k];
+                    Leaf64 L;
+                    if (staged) {
+                        L = win[k - w0];
+                    } else {
+                        L.x = cols64[k];
+                        L.y = cols64[n + k];
+                        L.z = cols64[2 * n + k];
+                        L.h = cols64[4 * n + k];
+                    }
                     mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
                     mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
                     mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];
@@ -745,7 +755,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const Leaf64* __restrict
 // One bottom-up round: complete every node of the list from its two complete children, then
 // count it at its parent; a parent counted twice is complete-able in the next round.
 struct RefitTree {
-    const Leaf64* leaf;
+    const double* cols64;       // x | y | z | m | h ... in Morton order
     const int32_t* lr;
     const int32_t* parent;
     const int32_t* escape;
@@ -764,7 +774,7 @@ __device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int
     bool take = false;
     int32_t p = 0;
     if (valid && x > 0 && x < n) {
-        combine_node(R.leaf, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
+        combine_node(R.cols64, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
         const int32_t* parent = R.parent;
         uint32_t* visited = R.visited;
         p = parent[x];
@@ -858,7 +868,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     uint32_t* counts = reinterpret_cast<uint32_t*>(c->scratch_c.p);
     hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, kMaxRounds, 0u);
     RefitTree R;
-    R.leaf = c->leaf64.as<Leaf64>();
+    R.cols64 = c->cols64.as<double>();
     R.lr = c->lr.as<int32_t>();
     R.parent = c->parent.as<int32_t>();
     R.escape = c->escape.as<int32_t>();
@@ -869,7 +879,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     R.nodes = c->nodes64.as<Node64>();
     R.n32 = c->nodes32.as<Node32>();
     const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
-    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.leaf, R.lr, R.escape, R.rlo, R.rhi, n,
+    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.lr, R.escape, R.rlo, R.rhi, n,
                        R.visited, R.nodes, R.n32, g_refit_small, listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
@@ -961,7 +971,6 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     ColPtrs cp;
     for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
     const unsigned g = grid_for(n, kBlock, 16384);
-    PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)n));
     PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
     // FP32 data of the compensated path (layout shared with tree32 / prepare_f32)
     const size_t payf = (size_t)n * (2 + p->ncol);
@@ -977,11 +986,11 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     if (p->dtype == PMX_F64) {
         hipLaunchKernelGGL(k_pack_records<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<double>());
         hipLaunchKernelGGL(k_gather_records<double>, dim3(g), dim3(kBlock), 0, s, c->records.as<double>(), ncols,
-                           order, n, cols64, c->leaf64.as<Leaf64>(), o32);
+                           order, n, cols64, o32);
     } else {
         hipLaunchKernelGGL(k_pack_records<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<float>());
         hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
-                           order, n, cols64, c->leaf64.as<Leaf64>(), o32);
+                           order, n, cols64, o32);
     }
     if (!async) PMX_HIP(c, hipEventRecord(c->ev[2], s));
     // ---- radix tree, escapes, refit ----
@@ -1013,6 +1022,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     if (n >= 2) PMX_TRY(refit_rounds(c, n, g));
     PMX_HIP(c, hipGetLastError());
     c->tree.n = n;
+    c->tree.leaf64_ready = false;
     c->tree.root = n >= 2 ? 1 : 0;
     c->tree.ncol = p->ncol;
     c->tree.kernel = p->kernel;
@@ -1057,6 +1067,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     c->timings[2] = ms[2];
     c->timings[3] = ms[1];
     c->tree.n = n;
+    c->tree.leaf64_ready = false;
     c->tree.root = n >= 2 ? 1 : 0;
     c->tree.ncol = p->ncol;
     c->tree.kernel = p->kernel;
@@ -1069,5 +1080,16 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
 
 namespace pmx {
 // Device address of the NaN-coordinate counter of the context's last build.
+int ensure_leaf64(pmx_ctx* c) {
+    if (c->tree.leaf64_ready) return PMX_OK;
+    const int64_t n = c->tree.n;
+    PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)std::max<int64_t>(n, 1)));
+    hipLaunchKernelGGL(k_pack_leaf, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, c->stream,
+                       c->cols64.as<double>(), n, c->leaf64.as<Leaf64>());
+    PMX_HIP(c, hipGetLastError());
+    c->tree.leaf64_ready = true;
+    return PMX_OK;
+}
+
 int* nan_count_device(pmx_ctx* c) { return reinterpret_cast<int*>(c->misc.as<double>() + 6 * 1024 + 6); }
 }  // namespace pmx

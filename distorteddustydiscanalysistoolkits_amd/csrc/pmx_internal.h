@@ -43,6 +43,7 @@ struct Tree {
     int32_t kernel = 0;
     bool built = false;
     bool f32_ready = false;
+    bool leaf64_ready = false;    // Leaf64 records (plain FP64 path), packed on first use
 };
 
 }  // namespace pmx
@@ -145,6 +146,8 @@ int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, con
 // Morton order of the samples (FP32-compensated path tiles); the permutation lives in the
 // context's scratch buffers until the next call that uses them.
 int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm);
+// Leaf64 records of the current tree from cols64 (the plain FP64 path packs them on first use).
+int ensure_leaf64(pmx_ctx* c);
 int nearest_pass_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts,
                       const int32_t* perm, const int32_t** nidx_out, int32_t** perm_out);
 int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm, const int32_t* nidx,

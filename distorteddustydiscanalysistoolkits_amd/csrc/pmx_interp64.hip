@@ -502,6 +502,7 @@ void launch_lines_k(pmx_ctx* c, const LineCat& cat, const double* o, const doubl
 
 int interp_points_f64(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx, const double* qy,
                       const double* qz, int64_t npts, double* out, int64_t ldo, const double* ha_in, double* ha_out) {
+    PMX_TRY(ensure_leaf64(c));
     switch (c->tree.kernel) {
     case kM4: launch_points_k<kM4>(c, strategy, cat, qx, qy, qz, npts, out, ldo, ha_in, ha_out); break;
     case kM5: launch_points_k<kM5>(c, strategy, cat, qx, qy, qz, npts, out, ldo, ha_in, ha_out); break;
@@ -516,6 +517,7 @@ int interp_points_f64(pmx_ctx* c, int strategy, const CatChunk& cat, const doubl
 
 int interp_lines_f64(pmx_ctx* c, int strategy, const int* slots, const int* shep, int ncols, const double* o,
                      const double* d, int64_t nl, double* out, int64_t ldo) {
+    PMX_TRY(ensure_leaf64(c));
     (void)strategy;
     for (int base = 0; base < ncols || (ncols == 0 && base == 0); base += 8) {
         LineCat cat{};
@@ -537,6 +539,7 @@ int interp_lines_f64(pmx_ctx* c, int strategy, const int* slots, const int* shep
 }
 
 int nearest_f64(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx) {
+    PMX_TRY(ensure_leaf64(c));
     hipLaunchKernelGGL(k_nearest, dim3(grid_for(npts, kBlock, 1 << 16)), dim3(kBlock), 0, c->stream, tree_view(c), qx,
                        qy, qz, npts, idx);
     PMX_HIP(c, hipGetLastError());
@@ -545,6 +548,7 @@ int nearest_f64(pmx_ctx* c, const double* qx, const double* qy, const double* qz
 
 int neighbors_points_f64(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,
                          const double* ha, int64_t npts, int64_t* counts, const int64_t* offsets, int32_t* idx) {
+    PMX_TRY(ensure_leaf64(c));
     const dim3 g(grid_for(npts, kBlock, 1 << 16)), b(kBlock);
     const double Kv = support(c->tree.kernel);
     TreeView T = tree_view(c);
@@ -560,6 +564,7 @@ int neighbors_points_f64(pmx_ctx* c, int strategy, const double* qx, const doubl
 
 int neighbors_lines_f64(pmx_ctx* c, int strategy, const double* o, const double* d, const double* ha, int64_t nl,
                         int64_t* counts, const int64_t* offsets, int32_t* idx) {
+    PMX_TRY(ensure_leaf64(c));
     const dim3 g(grid_for(nl, kBlock, 1 << 16)), b(kBlock);
     const double Kv = support(c->tree.kernel);
     TreeView T = tree_view(c);
@@ -575,6 +580,7 @@ int neighbors_lines_f64(pmx_ctx* c, int strategy, const double* o, const double*
 
 int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, const double* b, const double* cc,
                         int64_t n, unsigned long long* tot) {
+    PMX_TRY(ensure_leaf64(c));
     const dim3 g(grid_for(n, kBlock, 1 << 16)), bl(kBlock);
     const double Kv = support(c->tree.kernel);
     TreeView T = tree_view(c);
