@@ -414,10 +414,22 @@ struct ColPtrs {
 // The input columns are first interleaved into one record per particle (coalesced reads
 // and writes), so the permutation reads one record per particle instead of one element per
 // column from unrelated places.
+// A workgroup transposes kBlock particles through LDS: column reads and record writes are
+// both contiguous runs.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pack_records(ColPtrs src, int ncols, int64_t n, T* __restrict__ rec) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        for (int c = 0; c < ncols; ++c) rec[i * ncols + c] = static_cast<const T*>(src.p[c])[i];
+    __shared__ T tile[kBlock * 24];
+    const int t = threadIdx.x;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        const int cnt = (int)((n - base) < kBlock ? (n - base) : kBlock);
+        __syncthreads();
+        if (t < cnt)
+            for (int c = 0; c < ncols; ++c) tile[t * ncols + c] = static_cast<const T*>(src.p[c])[base + t];
+        __syncthreads();
+        T* out = rec + base * ncols;
+        const int total = cnt * ncols;
+        for (int e = t; e < total; e += kBlock) out[e] = tile[e];
+    }
 }
 
 // Also writes the FP32 leaf data of the compensated path: {x_hi, y_hi, z_hi, h} and
@@ -679,6 +691,8 @@ __device__ inline void block_append(bool take, int32_t p, int32_t* __restrict__ 
     __syncthreads();
 }
 
+constexpr int kRefitPad = 64;   // largest leaf range of a node refit directly (staged window margin)
+
 __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
                                                         const int32_t* __restrict__ range_lo,
@@ -688,7 +702,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
     // Internal node i covers leaves with one end at i - 1 or i, so the block's nodes read a
     // window of leaves around its own ids: staged once in LDS, then read by every node.
-    constexpr int kPad = 64;
+    constexpr int kPad = kRefitPad;
     __shared__ Leaf64 win[kBlock + 2 * kPad + 1];    // leaves i0 - 1 - kPad .. i0 + kBlock + kPad - 1
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                            // internal nodes 1 .. n-1
@@ -715,17 +729,9 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                 double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
                 double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
                 double hm = -__builtin_inf();
-                const bool staged = lo - 1 >= w0 && hi <= w1;
+                // small_range <= kPad: the leaf range lies inside the staged window
                 for (int32_t k = lo - 1; k < hi; ++k) {
-                    Leaf64 L;
-                    if (staged) {
-                        L = win[k - w0];
-                    } else {
-                        L.x = cols64[k];
-                        L.y = cols64[n + k];
-                        L.z = cols64[2 * n + k];
-                        L.h = cols64[4 * n + k];
-                    }
+                    const Leaf64 L = win[k - w0];
                     mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
                     mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
                     mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];
@@ -880,7 +886,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     R.n32 = c->nodes32.as<Node32>();
     const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
     hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.lr, R.escape, R.rlo, R.rhi, n,
-                       R.visited, R.nodes, R.n32, g_refit_small, listA, &counts[0]);
+                       R.visited, R.nodes, R.n32, std::min(g_refit_small, kRefitPad), listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
         int32_t* cur = (round & 1) ? listB : listA;
