@@ -86,6 +86,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);
     float* w_A = reinterpret_cast<float*>(dyn + (2 + nacc) * kCap);     // [ncol][64]
     const double Kv = support(K);
+    const double xmax = Kv * inv_dq;       // support in table steps
     const int64_t n = T.n;
 
     __syncthreads();
@@ -192,7 +193,18 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                 const int b = min(cs[row * kTS + w_cx1[j] + 1], C0 + m);
                 if (a >= b) continue;
                 const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
+                const double r2hi = r2 + f.band, r2lo = r2 - f.band;
                 const int64_t pj = w_p[j];
+                // per-item factors in registers: (m/rho) / h, 1 / (h dq), the item's fields
+                double wih = 0.0, xscale = 0.0;
+                float A0 = 0.0f, A1 = 0.0f;
+                if (MODE == 0) {
+                    const double invh = w_ih[j];
+                    wih = w_wb[j] * invh;
+                    xscale = invh * inv_dq;
+                    if (cat.n > 0) A0 = w_A[j];
+                    if (cat.n > 1) A1 = w_A[64 + j];
+                }
                 // lanes on the same row start at different columns (fewer LDS bank conflicts)
                 const int len = b - a;
                 int k = a + (lane % len);
@@ -200,8 +212,8 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     const int kk = k - C0;
                     const double ds = sb - S[kk], dt = tb - Tt[kk];
                     const double d2 = ds * ds + dt * dt;
-                    if (d2 > r2 + f.band) continue;
-                    bool ok = d2 < r2 - f.band;
+                    if (d2 > r2hi) continue;
+                    bool ok = d2 < r2lo;
                     if (!ok) {
                         // reference formula on the FP64 columns (_squared_distance_line_coords)
                         const int64_t li = perm[k];
@@ -217,24 +229,24 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                         // reciprocal square root and one FP64 correction step (relative error
                         // ~1e-14), q and the table interpolation in FP64 on FP32 table values:
                         // FP32 q would carry ~1e-7 relative error, amplified by dlnI/dlnq
-                        const double invh = w_ih[j];
                         const float d2f = (float)d2;
                         const float rs = d2f > 0.0f ? __builtin_amdgcn_rsqf(d2f) : 0.0f;
                         const double r0 = (double)(d2f * rs);
                         const double r = r0 + (d2 - r0 * r0) * (0.5 * (double)rs);
-                        const double q = r * invh;
+                        const double x = r * xscale;             // q / dq
                         double I = 0.0;
-                        if (q < Kv) {
-                            const double x = q * inv_dq;
+                        if (x < xmax) {
                             int i = (int)x;
                             i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
                             const double t = x - (double)i;
                             const double L0 = (double)lut[i];
                             I = L0 + t * ((double)lut[i + 1] - L0);
                         }
-                        const double w = w_wb[j] * (invh * I);
+                        const double w = wih * I;
                         atomicAdd(&acc[kk], w);
-                        for (int cc = 0; cc < cat.n; ++cc)
+                        if (cat.n > 0) atomicAdd(&acc[kCap + kk], (double)A0 * w);
+                        if (cat.n > 1) atomicAdd(&acc[2 * kCap + kk], (double)A1 * w);
+                        for (int cc = 2; cc < cat.n; ++cc)
                             atomicAdd(&acc[(1 + cc) * kCap + kk], (double)w_A[cc * 64 + j] * w);
                     } else {
                         const unsigned long long ip = (unsigned long long)pj;
