@@ -54,7 +54,8 @@ template <int K>
 __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ units, int nunits, int64_t total,
                                               unsigned long long* __restrict__ counter, pts::PointSplit split) {
     // dynamic LDS shared by the two tile kinds: points {BPos, BTile}, lines {LineTileLDS, dyn}
-    extern __shared__ double lds[];
+    extern __shared__ double2 lds2[];             // 16-byte aligned dynamic LDS
+    double* lds = reinterpret_cast<double*>(lds2);
     __shared__ long long item_s;
     pts::BPos& P = *reinterpret_cast<pts::BPos*>(lds);
     pts::BTile<false>& Y = *reinterpret_cast<pts::BTile<false>*>(reinterpret_cast<char*>(lds) + sizeof(pts::BPos));

@@ -53,7 +53,7 @@ __device__ inline void project_box(const Node32& b, const double a[3], double& l
 // LDS of one line-map tile: CSR starts of the tile's cells and the sweep state (one particle
 // per lane, and the owner lane of each (particle, row) item).  The line data, accumulators
 // and particle fields live in the caller's dynamic LDS (`dyn`, lines_tile_lds bytes).
-struct LineTileLDS {
+struct alignas(16) LineTileLDS {
     int32_t cs[kCells + 1];
     double w_sb[64], w_tb[64], w_r2[64], w_wb[64], w_ih[64];
     int32_t w_p[64], w_cx0[64], w_cx1[64], w_cy0[64], w_start[64];
@@ -80,8 +80,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     unsigned char* w_own = L.w_own;
     const int lane = threadIdx.x;
     const int nacc = MODE == 0 ? 1 + cat.n : 3;
-    double* S = dyn;
-    double* Tt = dyn + kCap;
+    double2* ST = reinterpret_cast<double2*>(dyn);     // (s, t) of the chunk's lines
     double* acc = dyn + 2 * kCap;
     unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);
     float* w_A = reinterpret_cast<float*>(dyn + (2 + nacc) * kCap);     // [ncol][64]
@@ -109,8 +108,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
         for (int k = lane; k < kCap; k += 64) {
             if (k < m) {
                 const double2 v = st_sorted[C0 + k];
-                S[k] = v.x;
-                Tt[k] = v.y;
+                ST[k] = v;
             }
             for (int a = 0; a < nacc; ++a) {
                 if (MODE == 0) acc[a * kCap + k] = 0.0;
@@ -210,7 +208,8 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                 int k = a + (lane % len);
                 for (int it = 0; it < len; ++it, k = (k + 1 == b) ? a : k + 1) {
                     const int kk = k - C0;
-                    const double ds = sb - S[kk], dt = tb - Tt[kk];
+                    const double2 L = ST[kk];
+                    const double ds = sb - L.x, dt = tb - L.y;
                     const double d2 = ds * ds + dt * dt;
                     if (d2 > r2hi) continue;
                     bool ok = d2 < r2lo;
@@ -239,8 +238,10 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                             int i = (int)x;
                             i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
                             const double t = x - (double)i;
-                            const double L0 = (double)lut[i];
-                            I = L0 + t * ((double)lut[i + 1] - L0);
+                            float2 Lv;                                 // lut[i], lut[i + 1]
+                            __builtin_memcpy(&Lv, lut + i, sizeof(Lv));
+                            const double L0 = (double)Lv.x;
+                            I = L0 + t * ((double)Lv.y - L0);
                         }
                         const double w = wih * I;
                         atomicAdd(&acc[kk], w);

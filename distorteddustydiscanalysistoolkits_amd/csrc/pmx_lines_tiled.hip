@@ -107,7 +107,8 @@ __global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat,
                                                     const float* __restrict__ lut, double inv_dq,
                                                     float* __restrict__ out, int64_t ldo,
                                                     uint64_t* __restrict__ digest) {
-    extern __shared__ double dyn[];
+    extern __shared__ double2 dyn2[];             // 16-byte aligned: the line coordinates are double2
+    double* dyn = reinterpret_cast<double*>(dyn2);
     __shared__ LineTileLDS L;
     const int64_t ntiles = (int64_t)f.tx * f.ty;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)

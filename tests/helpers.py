@@ -4,9 +4,9 @@ import numpy as np
 from distorteddustydiscanalysistoolkits_amd import disc
 from oracle import oracle as O
 
-# C2 field set: columns (rho, vx, vy, vz); scalars (rho,), gradients (rho,), divergences (v,),
+# The 8-output field set: columns (rho, vx, vy, vz); scalars (rho,), gradients (rho,), divergences (v,),
 # curls (v,) -> L = 8 (rho is not Shepard-normalised, InterpolationCatalog.jl:222-225).
-CAT_C2 = dict(scalar_slots=[0], scalar_shepard=[0], grad_slots=[0], div_slots=[1, 2, 3],
+CAT_RHO_V = dict(scalar_slots=[0], scalar_shepard=[0], grad_slots=[0], div_slots=[1, 2, 3],
               curl_slots=[1, 2, 3])
 
 

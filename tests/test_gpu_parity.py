@@ -11,7 +11,7 @@ import pytest
 from distorteddustydiscanalysistoolkits_amd import _lib, disc
 from oracle import oracle as O
 
-from .helpers import CAT_C2, csr_sets, cyl_points, disc_columns, oracle_tree, rel_norms
+from .helpers import CAT_RHO_V, csr_sets, cyl_points, disc_columns, oracle_tree, rel_norms
 
 pytestmark = pytest.mark.gpu
 
@@ -89,10 +89,10 @@ def test_line_neighbor_sets_bitexact(built):
 @pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
 def test_points_f64_fields(built, strategy):
     qx, qy, qz = cyl_points(20, 20, 10)
-    cat = _lib.catalog_struct(**CAT_C2)
+    cat = _lib.catalog_struct(**CAT_RHO_V)
     outs = [np.empty(qx.size) for _ in range(8)]
     built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F64, cat, qx, qy, qz, outs)
-    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    ref = built["tree"].point_samples("M4", strategy, CAT_RHO_V, qx, qy, qz)
     for j, (g, r) in enumerate(zip(outs, ref)):
         assert np.array_equal(np.isnan(g), np.isnan(r)), f"field {j}: NaN pattern differs"
         linf, l2 = rel_norms(g, r)
@@ -185,15 +185,45 @@ def test_f32c_line_neighbor_sets_bitexact(built):
 @pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
 def test_f32c_points_fields(built, strategy):
     qx, qy, qz = cyl_points(20, 20, 10)
-    cat = _lib.catalog_struct(**CAT_C2)
+    cat = _lib.catalog_struct(**CAT_RHO_V)
     outs = [np.empty(qx.size) for _ in range(8)]
     built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, cat, qx, qy, qz, outs)
-    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    ref = built["tree"].point_samples("M4", strategy, CAT_RHO_V, qx, qy, qz)
     worst = []
     for j, (g, r) in enumerate(zip(outs, ref)):
         assert np.array_equal(np.isnan(g), np.isnan(r)), f"field {j}: NaN pattern differs"
         worst.append(rel_norms(g, r))
     print(f"[f32c points {strategy}] rel (Linf, L2) per field: {worst}")
+    for j, (linf, l2) in enumerate(worst):
+        assert linf <= F32C_TOL and l2 <= F32C_TOL, f"field {j}: {linf} {l2}"
+
+
+@pytest.mark.parametrize("strategy", ["gather", "scatter", "symmetric"])
+def test_f32c_dust_species_fields(strategy):
+    """A thin dust layer (H_d = 0.3 H, v_phi x 0.995): a nearly solenoidal velocity field whose
+    divergence is far below its pair terms.  rho, div v and curl v of the FP32c path against the
+    oracle on a cylindrical grid reaching far above the layer (regression: FP32 pair terms gave
+    4e-6 on these fields at 8e6 particles)."""
+    d = disc.make_dust(N, seed=5, warped=True)
+    cols = [np.ascontiguousarray(d[k], np.float32) for k in ("x", "y", "z", "m", "h", "rho")]
+    quant = [np.ascontiguousarray(d[k], np.float32) for k in ("rho", "vx", "vy", "vz")]
+    cat_d = dict(scalar_slots=[0], scalar_shepard=[0], div_slots=[1, 2, 3], curl_slots=[1, 2, 3])
+    qx, qy, qz = cyl_points(24, 24, 12, zr=2.0)
+    outs = [np.empty(qx.size) for _ in range(5)]
+    c2 = _lib.Context(0)               # its own context: the module's tree stays as built
+    try:
+        c2.build(*cols, quant=quant, kernel=0)
+        c2.interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, _lib.catalog_struct(**cat_d), qx, qy, qz,
+                              outs)
+    finally:
+        c2.close()
+    tree, _, _ = oracle_tree(cols, quant)
+    ref = tree.point_samples("M4", strategy, cat_d, qx, qy, qz)
+    worst = []
+    for j, (g, r) in enumerate(zip(outs, ref)):
+        assert np.array_equal(np.isnan(g), np.isnan(r)), f"field {j}: NaN pattern differs"
+        worst.append(rel_norms(g, r))
+    print(f"[f32c dust {strategy}] rel (Linf, L2) per field: {worst}", flush=True)
     for j, (linf, l2) in enumerate(worst):
         assert linf <= F32C_TOL and l2 <= F32C_TOL, f"field {j}: {linf} {l2}"
 
@@ -204,7 +234,7 @@ def test_f32c_points_split_tiles(built, strategy, monkeypatch):
     particles): the split path must match the unsplit path (same pairs, FP64 sums in another
     order) and the oracle."""
     qx, qy, qz = cyl_points(20, 20, 10)
-    cat = _lib.catalog_struct(**CAT_C2)
+    cat = _lib.catalog_struct(**CAT_RHO_V)
     ref_split = [np.empty(qx.size) for _ in range(8)]
     built["ctx"].interpolate_points(O.STRATEGIES[strategy], _lib.PMX_PREC_F32C, cat, qx, qy, qz, ref_split)
     monkeypatch.setenv("PMX_SPLIT_BUDGET", "16")
@@ -220,7 +250,7 @@ def test_f32c_points_split_tiles(built, strategy, monkeypatch):
         assert t[8] > 0 and t[9] > t[8]
     finally:
         c2.close()
-    ref = built["tree"].point_samples("M4", strategy, CAT_C2, qx, qy, qz)
+    ref = built["tree"].point_samples("M4", strategy, CAT_RHO_V, qx, qy, qz)
     for j, (g, u, r) in enumerate(zip(outs, ref_split, ref)):
         assert np.array_equal(np.isnan(g), np.isnan(u)), f"field {j}: NaN pattern differs"
         assert rel_norms(g, u)[0] <= 1e-6, f"field {j}: split vs unsplit {rel_norms(g, u)}"
