@@ -22,7 +22,8 @@ PMX_PREC_F64, PMX_PREC_F32C = 0, 1
 PMX_BUILD_WRITEBACK = 1
 
 EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_build_lbvh",
-           "pmx_interpolate_points", "pmx_interpolate_lines", "pmx_point_samples_interpolation",
+           "pmx_interpolate_points", "pmx_interpolate_cartesian_grid", "pmx_interpolate_lines",
+           "pmx_point_samples_interpolation",
            "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
            "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
            "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings", "pmx_point_traversal_stats",
@@ -73,6 +74,10 @@ class Points(C.Structure):
     _fields_ = [("mem", C.c_int32), ("n", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p)]
 
 
+class Axes(C.Structure):
+    _fields_ = [("mem", C.c_int32), ("n", C.c_int64 * 3), ("a", C.c_void_p * 3)]
+
+
 class Lines(C.Structure):
     _fields_ = [("mem", C.c_int32), ("n", C.c_int64), ("ox", C.c_void_p), ("oy", C.c_void_p),
                 ("oz", C.c_void_p), ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p)]
@@ -108,6 +113,8 @@ def lib():
         L.pmx_build_lbvh.argtypes = [vp, C.POINTER(Particles), i32, vp, vp]
         L.pmx_interpolate_points.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Points), C.POINTER(Output)]
         L.pmx_interpolate_lines.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Lines), C.POINTER(Output)]
+        L.pmx_interpolate_cartesian_grid.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Axes),
+                                                     C.POINTER(Output)]
         L.pmx_point_samples_interpolation.argtypes = [vp, C.POINTER(Particles), i32, i32, C.POINTER(Catalog),
                                                       C.POINTER(Points), C.POINTER(Output)]
         L.pmx_line_samples_interpolation.argtypes = [vp, C.POINTER(Particles), i32, i32, C.POINTER(Catalog),
@@ -251,6 +258,18 @@ class Context:
         out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
         self._check(lib().pmx_interpolate_points(self._h, int(strategy), int(precision), C.byref(cat),
                                                  C.byref(pts), C.byref(out)))
+
+    def interpolate_cartesian_grid(self, strategy, precision, cat, xa, ya, za, out_cols):
+        """Structured Cartesian grid from its axes; samples flattened on the device in column-major
+        order (x fastest); out_cols hold len(xa) * len(ya) * len(za) values in that order."""
+        ax = Axes(_mem_of([xa, ya, za]), (C.c_int64 * 3)(*(int(a.shape[0]) for a in (xa, ya, za))),
+                  (C.c_void_p * 3)(*(_ptr(a) for a in (xa, ya, za))))
+        mem = _mem_of(out_cols) if out_cols else PMX_MEM_HOST
+        dt = _dtype_code(out_cols[0]) if out_cols else PMX_F64
+        cp = (C.c_void_p * max(1, len(out_cols)))(*[_ptr(o) for o in out_cols])
+        out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
+        self._check(lib().pmx_interpolate_cartesian_grid(self._h, int(strategy), int(precision), C.byref(cat),
+                                                         C.byref(ax), C.byref(out)))
 
     def interpolate_lines(self, strategy, precision, cat, origin, direction, out_cols):
         ln = self._lines(origin, direction)

@@ -55,6 +55,20 @@ __global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         out[i] = (float)in[i];
 }
+// coordinate_grid(Cartesian, grid) (StructuredGrid.jl:126-141): point i = ix + nx (iy + ny iz)
+// takes (ax[ix], ay[iy], az[iz]) — copies, no arithmetic on the values.
+__global__ void k_cartesian_points(const double* __restrict__ ax, const double* __restrict__ ay,
+                                   const double* __restrict__ az, int64_t nx, int64_t ny, int64_t npts,
+                                   double* __restrict__ x, double* __restrict__ y, double* __restrict__ z) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ix = i % nx, r = i / nx;
+        const int64_t iy = r % ny, iz = r / ny;
+        x[i] = ax[ix];
+        y[i] = ay[iy];
+        z[i] = az[iz];
+    }
+}
+
 __global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ out, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         out[i] = (double)in[i];
@@ -279,7 +293,7 @@ void pmx_destroy(pmx_ctx* c) {
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
-                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records, &c->nidx, &c->split};
+                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records, &c->nidx, &c->split, &c->gridpts};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : c->ev)
@@ -294,6 +308,46 @@ int pmx_build_lbvh(pmx_ctx* c, const pmx_particles* p, int32_t flags, uint64_t* 
     if (!c) return PMX_ERR_ARGUMENT;
     PMX_HIP(c, hipSetDevice(c->device));
     return build_lbvh(c, p, flags, codes_out, order_out);
+}
+
+int pmx_interpolate_cartesian_grid(pmx_ctx* c, int32_t strategy, int32_t precision, const pmx_catalog* cat,
+                                   const pmx_axes* axes, const pmx_output* out) {
+    if (!c) return PMX_ERR_ARGUMENT;
+    if (!axes) return set_error(c, PMX_ERR_ARGUMENT, "axes: null pointer");
+    if (axes->mem != PMX_MEM_HOST && axes->mem != PMX_MEM_DEVICE) return set_error(c, PMX_ERR_ARGUMENT, "axes: mem");
+    int64_t npts = 1;
+    for (int k = 0; k < 3; ++k) {
+        if (axes->n[k] < 1) return set_error(c, PMX_ERR_DIMENSION, "axes: every axis needs at least one value");
+        if (!axes->a[k]) return set_error(c, PMX_ERR_ARGUMENT, "axes: null axis pointer");
+        if (npts > ((int64_t)1 << 40) / axes->n[k]) return set_error(c, PMX_ERR_ARGUMENT, "axes: grid too large");
+        npts *= axes->n[k];
+    }
+    PMX_TRY(require_tree(c));
+    PMX_HIP(c, hipSetDevice(c->device));
+    const int64_t naxes = axes->n[0] + axes->n[1] + axes->n[2];
+    PMX_TRY(ensure(c, c->gridpts, 8 * (size_t)(3 * npts + naxes)));
+    double* xyz = c->gridpts.as<double>();
+    const double* dax[3];
+    double* tail = xyz + 3 * npts;
+    for (int k = 0; k < 3; ++k) {
+        if (axes->mem == PMX_MEM_DEVICE) {
+            dax[k] = axes->a[k];
+        } else {
+            PMX_HIP(c, hipMemcpyAsync(tail, axes->a[k], 8 * (size_t)axes->n[k], hipMemcpyHostToDevice, c->stream));
+            dax[k] = tail;
+            tail += axes->n[k];
+        }
+    }
+    hipLaunchKernelGGL(k_cartesian_points, dim3(grid_for(npts, 256, 16384)), dim3(256), 0, c->stream, dax[0], dax[1],
+                       dax[2], axes->n[0], axes->n[1], npts, xyz, xyz + npts, xyz + 2 * npts);
+    PMX_HIP(c, hipGetLastError());
+    pmx_points pts;
+    pts.mem = PMX_MEM_DEVICE;
+    pts.n = npts;
+    pts.x = xyz;
+    pts.y = xyz + npts;
+    pts.z = xyz + 2 * npts;
+    return pmx_interpolate_points(c, strategy, precision, cat, &pts, out);
 }
 
 int pmx_interpolate_points(pmx_ctx* c, int32_t strategy, int32_t precision, const pmx_catalog* cat,

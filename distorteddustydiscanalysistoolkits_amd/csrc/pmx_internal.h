@@ -74,6 +74,7 @@ struct pmx_ctx {
     int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
     pmx::DevBuf split;         // split-tile counters, item queue and slots (FP32c points)
+    pmx::DevBuf gridpts;       // samples of a Cartesian structured grid (x | y | z), then its axes
     pmx::DevBuf records;       // particle records (AoS) for the Morton-order permutation
     pmx::DevBuf tl_st, tl_part, tl_cells;  // line-map binning (pmx_lines_tiled.hip)
     hipEvent_t ev[12] = {};

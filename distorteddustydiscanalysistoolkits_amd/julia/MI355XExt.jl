@@ -3,7 +3,8 @@
 # Adds MI355XComputeBackend methods to Partia.PointSamples_interpolation and
 # Partia.LineSamples_interpolation, the two methods the CUDA extension adds
 # (Partia.jl/ext/CUDAExt/KernelInterpolation/grid_interpolation/PointSamples_interpolation.jl:12,
-#  LineSamples_interpolation.jl:11); StructuredGrid_interpolation dispatches through them.
+#  LineSamples_interpolation.jl:11); StructuredGrid_interpolation dispatches through them, except
+# for Cartesian grids, whose samples the library flattens on the device from the axes.
 module MI355XExt
 
 using Partia
@@ -13,6 +14,7 @@ import PartiaMI355X_jll: libpartia_mi355x
 const PMX_F32, PMX_F64 = Int32(0), Int32(1)
 const PMX_MEM_HOST = Int32(0)
 const PMX_PREC_F64, PMX_PREC_F32C = Int32(0), Int32(1)
+const PMX_BUILD_WRITEBACK = Int32(1)
 
 struct PmxParticles
     dtype::Int32; mem::Int32; n::Int64
@@ -33,6 +35,9 @@ struct PmxLines
     mem::Int32; n::Int64
     ox::Ptr{Float64}; oy::Ptr{Float64}; oz::Ptr{Float64}
     dx::Ptr{Float64}; dy::Ptr{Float64}; dz::Ptr{Float64}
+end
+struct PmxAxes
+    mem::Int32; n::NTuple{3,Int64}; a::NTuple{3,Ptr{Float64}}
 end
 struct PmxOutput
     dtype::Int32; mem::Int32; cols::Ptr{Ptr{Cvoid}}
@@ -140,6 +145,40 @@ function Partia.LineSamples_interpolation(backend::MI355XComputeBackend, grid_te
             _check(ctx, rc)
         end
     end
+    return GridBundle(grids, catalog.ordered_names)
+end
+
+# StructuredGrid_interpolation (StructuredGrid_interpolation.jl:46-64) for Cartesian grids: the
+# build (LinearBVH!, permuting the input like the reference), then the samples generated on the
+# device from the axes in coordinate_grid's column-major order (StructuredGrid.jl:126-141).
+function Partia.StructuredGrid_interpolation(backend::MI355XComputeBackend, ::Type{Cartesian},
+        grid_template::StructuredGrid{3,TF}, input::InterpolationInput{3,TF},
+        catalog::InterpolationCatalog{3,N,G,Div,C,L},
+        itp_strategy::Type{ITP} = itpSymmetric) where {N,G,Div,C,L,TF<:AbstractFloat,ITP<:AbstractInterpolationStrategy}
+    vals = ntuple(_ -> similar(grid_template.grid), Val(L))
+    ss, sh, gs, ds, cs = _catalog(catalog)
+    qptrs = Ptr{Cvoid}[pointer(q) for q in input.quant]
+    outp = Ptr{Cvoid}[pointer(v) for v in vals]
+    ax = ntuple(d -> Float64.(grid_template.axes[d]), 3)
+    lock(_LOCK) do
+        ctx = _ctx(backend.device)
+        GC.@preserve input qptrs outp vals ss sh gs ds cs ax begin
+            p = Ref(_particles(input, qptrs))
+            rc = ccall((:pmx_build_lbvh, libpartia_mi355x), Cint,
+                       (Ptr{Cvoid}, Ref{PmxParticles}, Int32, Ptr{UInt64}, Ptr{Int64}),
+                       ctx, p, PMX_BUILD_WRITEBACK, C_NULL, C_NULL)
+            _check(ctx, rc)
+            cat = Ref(PmxCatalog(length(ss), pointer(ss), pointer(sh), length(gs), pointer(gs),
+                                 length(ds) ÷ 3, pointer(ds), length(cs) ÷ 3, pointer(cs)))
+            axes = Ref(PmxAxes(PMX_MEM_HOST, ntuple(d -> Int64(length(ax[d])), 3), ntuple(d -> pointer(ax[d]), 3)))
+            out = Ref(PmxOutput(_dtype(TF), PMX_MEM_HOST, pointer(outp)))
+            rc = ccall((:pmx_interpolate_cartesian_grid, libpartia_mi355x), Cint,
+                       (Ptr{Cvoid}, Int32, Int32, Ref{PmxCatalog}, Ref{PmxAxes}, Ref{PmxOutput}),
+                       ctx, _strategy(itp_strategy), _precision(backend), cat, axes, out)
+            _check(ctx, rc)
+        end
+    end
+    grids = ntuple(i -> StructuredGrid(vals[i], grid_template.axes, grid_template.size), Val(L))
     return GridBundle(grids, catalog.ordered_names)
 end
 

@@ -364,6 +364,19 @@ def LineSamples_interpolation(backend, grid_template: LineSamples, input: Interp
 
 def StructuredGrid_interpolation(backend, COORD, grid_template: StructuredGrid, input: InterpolationInput,
                                  catalog: InterpolationCatalog, itp_strategy=itpSymmetric) -> GridBundle:
+    if COORD is Cartesian and len(grid_template.axes) == 3:
+        # Cartesian samples are copies of the axis values: flattened on the device (bit-equal
+        # to flatten + coordinate_grid, transform.jl:53-56, StructuredGrid.jl:126-141)
+        if not isinstance(backend, MI355XComputeBackend):
+            raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "this package provides the MI355XComputeBackend only")
+        ctx = _build(backend, input)
+        axes = [_f64(a) for a in grid_template.axes]
+        size = tuple(len(a) for a in axes)
+        dt = np.float32 if str(input.dtype).endswith("float32") else np.float64
+        outs = [np.empty(int(np.prod(size)), dt) for _ in range(catalog.L)]
+        ctx.interpolate_cartesian_grid(itp_strategy.code, backend.precision, catalog.struct(), *axes, outs)
+        grids = [StructuredGrid(np.reshape(o, size, order="F"), grid_template.axes) for o in outs]
+        return GridBundle(grids, catalog.ordered_names)
     flat = flatten(COORD, grid_template)
     bundle = PointSamples_interpolation(backend, flat, input, catalog, itp_strategy)
     grids = [restore_struct(COORD, g, grid_template.axes) for g in bundle.grids]

@@ -92,6 +92,14 @@ typedef struct pmx_points {
     const double *x, *y, *z;
 } pmx_points;
 
+/* Axes of a Cartesian structured grid (≙ StructuredGrid axes, P/src/julia/Grids/grids/
+ * StructuredGrid.jl:271-518); a 2-D slice in 3-D has n[2] = 1. */
+typedef struct pmx_axes {
+    int32_t mem;                     /* PMX_MEM_HOST | PMX_MEM_DEVICE; axis values are FP64 */
+    int64_t n[3];
+    const double *a[3];
+} pmx_axes;
+
 /* Sample lines (≙ LineSamples origin/direction, P/src/julia/Grids/grids/LineSamples.jl:31). */
 typedef struct pmx_lines {
     int32_t mem;
@@ -135,6 +143,16 @@ int pmx_build_lbvh(struct pmx_ctx *ctx, const pmx_particles *p, int32_t flags,
  * _general_quantity_interpolate_kernel (general_interpolation.jl:1-669). */
 int pmx_interpolate_points(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
                            const pmx_catalog *cat, const pmx_points *pts, const pmx_output *out);
+
+/* ≙ StructuredGrid_interpolation(backend, Cartesian, grid, ...) (StructuredGrid_interpolation.jl:46-64)
+ * on the context's tree: the sample coordinates are generated on the device from the axes,
+ * point i = ix + nx (iy + ny iz) taking (ax[ix], ay[iy], az[iz]) — the column-major order of
+ * coordinate_grid(Cartesian, grid) (StructuredGrid.jl:126-141) and flatten (transform.jl:53-56);
+ * the values are copies, so the samples equal the host flatten bit for bit.  Outputs have
+ * nx ny nz entries in the same order (≙ restore_struct's reshape, transform.jl:107-116).
+ * Otherwise as pmx_interpolate_points. */
+int pmx_interpolate_cartesian_grid(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                                   const pmx_catalog *cat, const pmx_axes *axes, const pmx_output *out);
 
 /* ≙ LineSamples_interpolation (LineSamples_interpolation.jl:47-99) with
  * _line_integrated_quantities_interpolate_kernel (line_integrated_scalar_interpolation.jl:123-166).

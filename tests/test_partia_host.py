@@ -8,7 +8,7 @@ import pytest
 from distorteddustydiscanalysistoolkits_amd import partia as P
 
 
-def test_catalog_c2_field_set():
+def test_catalog_rho_v_field_set():
     cat = P.InterpolationCatalog(("rho", "vx", "vy", "vz"), scalars=("rho",), gradients=("rho",),
                                  divergences=("v",), curls=("v",))
     assert cat.ordered_names == ("rho", "∇rhoˣ", "∇rhoʸ", "∇rhoᶻ", "∇⋅v", "∇×vˣ", "∇×vʸ", "∇×vᶻ")
