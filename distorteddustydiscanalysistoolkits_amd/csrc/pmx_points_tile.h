@@ -272,22 +272,34 @@ struct PointCat {
     int os, og, od, oc;
 };
 
-// LDS of one sample tile: the staged payload of a leaf run, the tile's samples (read by the
-// lanes that process their pairs), the queue of accepted (sample, particle) pairs and the
-// FP64 accumulators of every sample.
+// LDS of one sample tile.  Scalar catalogs (DERIV false): the staged payload of a leaf run,
+// the tile's samples (read by the lanes that process their pairs), the queue of accepted
+// (sample, particle) pairs and the FP64 accumulators of every sample.  Derivative catalogs
+// (DERIV true): the staged payload only; each lane keeps its sample's sums in registers.
 template <bool DERIV>
-struct BTile {
-    static constexpr int kAcc = DERIV ? 20 : 3;
-    static constexpr int kRef = DERIV ? kMaxD : 1;
+struct BTile;
+
+template <>
+struct BTile<false> {
     float w[kMaxBucket];
-    double w64[DERIV ? kMaxBucket : 1];   // m / rho in FP64 (derivative launches)
     double ih64[kMaxBucket];   // 1 / h in FP64
     float s[2][kMaxBucket];
-    double d[kRef][kMaxBucket];
     float ph[3][kWave], pl[3][kWave];
     double invha[kWave];
-    double ref[kRef][kWave];
-    double acc[kAcc][kWave];
+    double acc[3][kWave];
+    uint16_t pair[kQueue];
+};
+
+template <>
+struct BTile<true> {
+    double w64[kMaxBucket];    // m / rho in FP64
+    double ih64[kMaxBucket];
+    float s[2][kMaxBucket];
+    double d[kMaxD][kMaxBucket];
+    float ph[3][kWave], pl[3][kWave];
+    double invha[kWave];
+    double ref[kMaxD][kWave];
+    double acc[20][kWave];
     uint16_t pair[kQueue];
 };
 
@@ -315,28 +327,154 @@ struct PointSplit {
     double* slot_acc;           // [slot][kAccMax][64] FP64 sums
 };
 
-// The per-lane state of a tile walk and the two steps of a staged leaf run: the accept test
-// with the pair queue (run) and the kernel evaluation of 64 queued pairs (process).
+// Tile walkers.  Both give the per-lane state of a tile walk and the steps of a staged leaf
+// run: setup(nearest), need(node), run(lo, cnt), sum(a) (the lane's FP64 sums), spent.
 template <int K, int S, bool DERIV>
-struct PointWalk {
+struct PointWalk;
+
+// Scalar catalogs: every lane tests the run's particles against its own sample and appends
+// the accepted pairs to an LDS queue; the wavefront then evaluates 64 queued pairs at a time,
+// one pair per lane, adding FP32 weights into the samples' FP64 LDS sums.  Only accepted pairs
+// cost kernel evaluations, however unevenly they fall on the lanes.
+template <int K, int S>
+struct PointWalk<K, S, false> {
+    static constexpr int kAcc = 3;
     const T32& T;
     const PointCat& cat;
     BPos& P;
-    BTile<DERIV>& Y;
+    BTile<false>& Y;
     PointLane q;
     int lane;
     double Kv, radius2_64;
     float Kvf, sig, radius2_32;
+    int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
+    int spent;                  // leaves tested so far (wave-uniform)
+
+    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<false>& Y_, const PointLane& q_)
+        : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
+          Kvf((float)support(K)), sig(norm3<float>(K)), radius2_32(0.0f), qhead(0), qtail(0), spent(0) {}
+
+    // lane state in LDS; nearest < 0 for scatter (or no particle)
+    __device__ void setup(int64_t nearest) {
+        double invha64 = 0.0;
+        if (S != kScatter) {
+            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * T.n + nearest];
+            const double radius = Kv * ha64;
+            radius2_64 = radius * radius;
+            radius2_32 = (float)radius2_64;
+            invha64 = 1.0 / ha64;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { Y.ph[k][lane] = q.ph[k]; Y.pl[k][lane] = q.pl[k]; }
+        Y.invha[lane] = invha64;
+#pragma unroll
+        for (int a = 0; a < kAcc; ++a) Y.acc[a][lane] = 0.0;
+    }
+
+    __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
+
+    __device__ double sum(int a) const { return Y.acc[a][lane]; }
+
+    // one pair per lane: pairs qhead .. qhead + cnt - 1 of the queue
+    __device__ void process(int cnt) {
+        __syncthreads();
+        if (lane < cnt) {
+            const int pr = Y.pair[(qhead + lane) & (kQueue - 1)];
+            const int sa = pr & (kWave - 1), j = pr >> 6;
+            const float qh0 = Y.ph[0][sa], qh1 = Y.ph[1][sa], qh2 = Y.ph[2][sa];
+            const float ql0 = Y.pl[0][sa], ql1 = Y.pl[1][sa], ql2 = Y.pl[2][sa];
+            // separation in FP64 from the hi / lo parts (both differences exact in FP64)
+            const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
+            const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
+            const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
+            const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+            const float invh = P.invh[j];
+            float W;
+            {
+                const double qb = rd * Y.ih64[j];
+                float fb, pb;
+                shape_fp32<K>((float)qb, (float)(Kv - qb), fb, pb);
+                const float Wb = ((invh * invh) * invh) * (sig * fb);
+                if (S == kScatter) {
+                    W = Wb;
+                } else {
+                    const double ia64 = Y.invha[sa];
+                    const float ia = (float)ia64;
+                    const double qa = rd * ia64;
+                    float fa, pa;
+                    shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
+                    const float Wa = (ia * ia * ia) * (sig * fa);
+                    W = S == kGather ? Wa : 0.5f * (Wa + Wb);
+                }
+            }
+            const float w = Y.w[j] * W;
+            atomicAdd(&Y.acc[kA_S1][sa], (double)w);
+            if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)(Y.s[0][j] * w));
+            if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)(Y.s[1][j] * w));
+        }
+        qhead += cnt;
+        __syncthreads();
+    }
+
+    // a run of leaves lo .. lo+cnt-1: stage, test every particle against every lane's sample,
+    // queue the accepted pairs and process them 64 at a time
+    __device__ void run(int lo, int cnt) {
+        const int64_t n = T.n;
+        stage_pos(T, P, lo, cnt);
+        if (lane < cnt) {
+            const int64_t li = lo + lane;
+            Y.w[lane] = T.pay[li];
+            Y.ih64[lane] = T.ih64[li];
+            for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
+        }
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            float dx, dy, dz;
+            const float d2 = d2_pos32(q, P, j, dx, dy, dz);
+            const bool acc = accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64);
+            const uint64_t m = ballot(acc);
+            if (!m) continue;
+            if (acc) {
+                const int at = (qtail + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
+                               (kQueue - 1);
+                Y.pair[at] = (uint16_t)(lane | (j << 6));
+            }
+            qtail += __builtin_popcountll(m);
+            if (qtail - qhead >= kWave) process(kWave);
+        }
+        if (qtail > qhead) process(qtail - qhead);
+        spent += cnt;
+    }
+};
+
+// Derivative catalogs: the same pair queue with FP64 pair terms (W, sigma h^-5 f'(q)/q, the
+// separation, the field differences A_b - A_ref; the derivative sums cancel, and FP32 terms
+// left 4e-6 on a nearly solenoidal velocity field) and about 20 FP64 sums per sample.  Keeping
+// the sums in registers instead (each lane evaluating its own accepted particles) needs 255
+// VGPRs, one wavefront per SIMD; the queue keeps them in LDS.
+template <int K, int S>
+struct PointWalk<K, S, true> {
+    static constexpr int kAcc = kAccMax;
+    const T32& T;
+    const PointCat& cat;
+    BPos& P;
+    BTile<true>& Y;
+    PointLane q;
+    int lane;
+    double Kv, radius2_64;
+    float Kvf, radius2_32;
     bool have_ref;
     int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
     int spent;                  // leaves tested so far (wave-uniform)
 
-    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<DERIV>& Y_, const PointLane& q_)
+    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<true>& Y_, const PointLane& q_)
         : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
-          Kvf((float)support(K)), sig(norm3<float>(K)), radius2_32(0.0f), have_ref(false), qhead(0), qtail(0),
-          spent(0) {}
+          Kvf((float)support(K)), radius2_32(0.0f), have_ref(false), qhead(0), qtail(0), spent(0) {}
 
-    // lane state in LDS; nearest < 0 for scatter (or no particle)
+    // lane state in LDS; the references are the nearest particle (gather / symmetric) or the
+    // first accepted neighbour (scatter)
     __device__ void setup(int64_t nearest) {
         const int64_t n = T.n;
         double invha64 = 0.0;
@@ -352,10 +490,8 @@ struct PointWalk {
         for (int k = 0; k < 3; ++k) { Y.ph[k][lane] = q.ph[k]; Y.pl[k][lane] = q.pl[k]; }
         Y.invha[lane] = invha64;
 #pragma unroll
-        for (int a = 0; a < BTile<DERIV>::kAcc; ++a) Y.acc[a][lane] = 0.0;
-        // per-sample FP64 reference values of the derivative fields: the nearest particle
-        // (gather / symmetric) or the first accepted neighbour (scatter)
-        if (DERIV && nearest >= 0) {
+        for (int a = 0; a < kAcc; ++a) Y.acc[a][lane] = 0.0;
+        if (nearest >= 0) {
             for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + nearest];
             have_ref = true;
         }
@@ -363,7 +499,8 @@ struct PointWalk {
 
     __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
 
-    // one pair per lane: pairs qhead .. qhead + cnt - 1 of the queue
+    __device__ double sum(int a) const { return Y.acc[a][lane]; }
+
     __device__ void process(int cnt) {
         __syncthreads();
         if (lane < cnt) {
@@ -371,120 +508,86 @@ struct PointWalk {
             const int sa = pr & (kWave - 1), j = pr >> 6;
             const float qh0 = Y.ph[0][sa], qh1 = Y.ph[1][sa], qh2 = Y.ph[2][sa];
             const float ql0 = Y.pl[0][sa], ql1 = Y.pl[1][sa], ql2 = Y.pl[2][sa];
-            // separation in FP64 from the hi / lo parts (both differences exact in FP64)
             const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
             const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
             const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
             const double rd = sqrt(ex * ex + ey * ey + ez * ez);
-            if (!DERIV) {
-                const float invh = P.invh[j];
-                float W;
-                {
-                    const double qb = rd * Y.ih64[j];
-                    float fb, pb;
-                    shape_fp32<K>((float)qb, (float)(Kv - qb), fb, pb);
-                    const float Wb = ((invh * invh) * invh) * (sig * fb);
-                    if (S == kScatter) {
-                        W = Wb;
-                    } else {
-                        const double ia64 = Y.invha[sa];
-                        const float ia = (float)ia64;
-                        const double qa = rd * ia64;
-                        float fa, pa;
-                        shape_fp32<K>((float)qa, (float)(Kv - qa), fa, pa);
-                        const float Wa = (ia * ia * ia) * (sig * fa);
-                        W = S == kGather ? Wa : 0.5f * (Wa + Wb);
-                    }
+            const double sig64 = norm3<double>(K);
+            double W, G;
+            {
+                const double ihb = Y.ih64[j];
+                const double qb = rd * ihb;
+                double fb, pb;
+                shape_fp64<K>(qb, Kv - qb, fb, pb);
+                const double ib2 = ihb * ihb;
+                const double Wb = (ib2 * ihb) * (sig64 * fb);
+                const double Gb = ((ib2 * ib2) * ihb) * (sig64 * pb);
+                if (S == kScatter) {
+                    W = Wb; G = Gb;
+                } else {
+                    const double ia = Y.invha[sa];
+                    const double qa = rd * ia;
+                    double fa, pa;
+                    shape_fp64<K>(qa, Kv - qa, fa, pa);
+                    const double ia2 = ia * ia;
+                    const double Wa = (ia2 * ia) * (sig64 * fa);
+                    const double Ga = ((ia2 * ia2) * ia) * (sig64 * pa);
+                    if (S == kGather) { W = Wa; G = Ga; }
+                    else { W = 0.5 * (Wa + Wb); G = 0.5 * (Ga + Gb); }
                 }
-                const float w = Y.w[j] * W;
-                atomicAdd(&Y.acc[kA_S1][sa], (double)w);
-                if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)(Y.s[0][j] * w));
-                if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)(Y.s[1][j] * w));
-            } else {
-                // FP64 pair terms: W, sigma h^-5 f'(q)/q, the separation and the field
-                // differences (the derivative sums cancel; FP32 terms left 4e-6 on a
-                // near-solenoidal velocity field)
-                const double sig64 = norm3<double>(K);
-                double W, G;
-                {
-                    const double ihb = Y.ih64[j];
-                    const double qb = rd * ihb;
-                    double fb, pb;
-                    shape_fp64<K>(qb, Kv - qb, fb, pb);
-                    const double ib2 = ihb * ihb;
-                    const double Wb = (ib2 * ihb) * (sig64 * fb);
-                    const double Gb = ((ib2 * ib2) * ihb) * (sig64 * pb);
-                    if (S == kScatter) {
-                        W = Wb; G = Gb;
-                    } else {
-                        const double ia = Y.invha[sa];
-                        const double qa = rd * ia;
-                        double fa, pa;
-                        shape_fp64<K>(qa, Kv - qa, fa, pa);
-                        const double ia2 = ia * ia;
-                        const double Wa = (ia2 * ia) * (sig64 * fa);
-                        const double Ga = ((ia2 * ia2) * ia) * (sig64 * pa);
-                        if (S == kGather) { W = Wa; G = Ga; }
-                        else { W = 0.5 * (Wa + Wb); G = 0.5 * (Ga + Gb); }
-                    }
-                }
-                const double wb = Y.w64[j];
-                const double w = wb * W;
-                atomicAdd(&Y.acc[kA_S1][sa], w);
-                if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)Y.s[0][j] * w);
-                if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)Y.s[1][j] * w);
-                // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
-                const double cg = wb * G;
-                const double gx = cg * ex, gy = cg * ey, gz = cg * ez;
-                atomicAdd(&Y.acc[kA_B][sa], gx);
-                atomicAdd(&Y.acc[kA_B + 1][sa], gy);
-                atomicAdd(&Y.acc[kA_B + 2][sa], gz);
-                if (cat.ng) {
-                    const double A = Y.d[cat.gf][j] - Y.ref[cat.gf][sa];
-                    atomicAdd(&Y.acc[kA_GF][sa], A * gx);
-                    atomicAdd(&Y.acc[kA_GF + 1][sa], A * gy);
-                    atomicAdd(&Y.acc[kA_GF + 2][sa], A * gz);
-                    atomicAdd(&Y.acc[kA_GS][sa], A * w);
-                }
-                if (cat.nd) {
-                    const double Ax = Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa];
-                    const double Ay = Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa];
-                    const double Az = Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa];
-                    atomicAdd(&Y.acc[kA_DF][sa], (Ax * gx + Ay * gy) + Az * gz);
-                    atomicAdd(&Y.acc[kA_DS][sa], Ax * w);
-                    atomicAdd(&Y.acc[kA_DS + 1][sa], Ay * w);
-                    atomicAdd(&Y.acc[kA_DS + 2][sa], Az * w);
-                }
-                if (cat.nc) {
-                    const double Ax = Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa];
-                    const double Ay = Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa];
-                    const double Az = Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa];
-                    atomicAdd(&Y.acc[kA_CF][sa], Ay * gz - Az * gy);
-                    atomicAdd(&Y.acc[kA_CF + 1][sa], Az * gx - Ax * gz);
-                    atomicAdd(&Y.acc[kA_CF + 2][sa], Ax * gy - Ay * gx);
-                    atomicAdd(&Y.acc[kA_CS][sa], Ax * w);
-                    atomicAdd(&Y.acc[kA_CS + 1][sa], Ay * w);
-                    atomicAdd(&Y.acc[kA_CS + 2][sa], Az * w);
-                }
+            }
+            const double wb = Y.w64[j];
+            const double w = wb * W;
+            atomicAdd(&Y.acc[kA_S1][sa], w);
+            if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)Y.s[0][j] * w);
+            if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)Y.s[1][j] * w);
+            // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
+            const double cg = wb * G;
+            const double gx = cg * ex, gy = cg * ey, gz = cg * ez;
+            atomicAdd(&Y.acc[kA_B][sa], gx);
+            atomicAdd(&Y.acc[kA_B + 1][sa], gy);
+            atomicAdd(&Y.acc[kA_B + 2][sa], gz);
+            if (cat.ng) {
+                const double A = Y.d[cat.gf][j] - Y.ref[cat.gf][sa];
+                atomicAdd(&Y.acc[kA_GF][sa], A * gx);
+                atomicAdd(&Y.acc[kA_GF + 1][sa], A * gy);
+                atomicAdd(&Y.acc[kA_GF + 2][sa], A * gz);
+                atomicAdd(&Y.acc[kA_GS][sa], A * w);
+            }
+            if (cat.nd) {
+                const double Ax = Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa];
+                const double Ay = Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa];
+                const double Az = Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa];
+                atomicAdd(&Y.acc[kA_DF][sa], (Ax * gx + Ay * gy) + Az * gz);
+                atomicAdd(&Y.acc[kA_DS][sa], Ax * w);
+                atomicAdd(&Y.acc[kA_DS + 1][sa], Ay * w);
+                atomicAdd(&Y.acc[kA_DS + 2][sa], Az * w);
+            }
+            if (cat.nc) {
+                const double Ax = Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa];
+                const double Ay = Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa];
+                const double Az = Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa];
+                atomicAdd(&Y.acc[kA_CF][sa], Ay * gz - Az * gy);
+                atomicAdd(&Y.acc[kA_CF + 1][sa], Az * gx - Ax * gz);
+                atomicAdd(&Y.acc[kA_CF + 2][sa], Ax * gy - Ay * gx);
+                atomicAdd(&Y.acc[kA_CS][sa], Ax * w);
+                atomicAdd(&Y.acc[kA_CS + 1][sa], Ay * w);
+                atomicAdd(&Y.acc[kA_CS + 2][sa], Az * w);
             }
         }
         qhead += cnt;
         __syncthreads();
     }
 
-    // a run of leaves lo .. lo+cnt-1: stage, test every particle against every lane's sample,
-    // queue the accepted pairs and process them 64 at a time
     __device__ void run(int lo, int cnt) {
         const int64_t n = T.n;
         stage_pos(T, P, lo, cnt);
         if (lane < cnt) {
             const int64_t li = lo + lane;
-            Y.w[lane] = T.pay[li];
-            if (DERIV) Y.w64[lane] = T.w64[li];
+            Y.w64[lane] = T.w64[li];
             Y.ih64[lane] = T.ih64[li];
             for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
-            if (DERIV)
-                for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
+            for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
         }
         __syncthreads();
         for (int j = 0; j < cnt; ++j) {
@@ -498,7 +601,7 @@ struct PointWalk {
                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
                                (kQueue - 1);
                 Y.pair[at] = (uint16_t)(lane | (j << 6));
-                if (DERIV && !have_ref) {
+                if (!have_ref) {
                     for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = Y.d[f][j];
                     have_ref = true;
                 }
@@ -524,7 +627,9 @@ __device__ inline void finalise_point(const PointCat& cat, Acc&& acc, float* __r
         return;
     }
     const double inv = 1.0 / s1;
-    for (int j = 0; j < cat.ns; ++j) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= cat.ns) break;
         const double v = acc(kA_SC + j);
         out[(cat.os + j) * ldo + qi] = (float)(cat.sh[j] ? v * inv : v);
     }
@@ -604,14 +709,15 @@ __device__ inline void points_tile(const T32& T, const PointCat& cat, const doub
     if (split >= 0) {
         // the items add to these sums; the finishing kernel writes the outputs
         double* acc = sp.slot_acc + (int64_t)split * kAccMax * kWave;
-        for (int a = 0; a < BTile<DERIV>::kAcc; ++a) acc[a * kWave + lane] = Y.acc[a][lane];
+#pragma unroll
+        for (int a = 0; a < PointWalk<K, S, DERIV>::kAcc; ++a) acc[a * kWave + lane] = W.sum(a);
         sp.slot_near[(int64_t)split * kWave + lane] = valid ? (int)nearest : -1;
         if (lane == 0) {
             sp.slot_tile[split] = (int)tile;
             if (sp.slot_unit) sp.slot_unit[split] = sp.unit;
         }
     } else if (valid) {
-        finalise_point<DERIV>(cat, [&](int a) { return Y.acc[a][lane]; }, out, ldo, qi);
+        finalise_point<DERIV>(cat, [&](int a) { return W.sum(a); }, out, ldo, qi);
     }
     __syncthreads();
 }
@@ -641,8 +747,9 @@ __device__ inline void points_item(const T32& T, const PointCat& cat, const doub
     walk_sub(T, node, stop, need, bucket, take, [](int, int) { return false; });
     __syncthreads();
     double* acc = sp.slot_acc + (int64_t)split * kAccMax * kWave;
-    for (int a = 0; a < BTile<DERIV>::kAcc; ++a) {
-        const double v = Y.acc[a][lane];
+#pragma unroll
+    for (int a = 0; a < PointWalk<K, S, DERIV>::kAcc; ++a) {
+        const double v = W.sum(a);
         if (v != 0.0) unsafeAtomicAdd(&acc[a * kWave + lane], v);
     }
     __syncthreads();
