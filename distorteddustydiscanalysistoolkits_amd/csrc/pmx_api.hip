@@ -242,7 +242,6 @@ static void read_tuning() {
     if (const char* v = std::getenv("PMX_BUCKET_LINES")) g_bucket_lines = std::max(1, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_BUCKET")) g_tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_NEAR")) g_tiled_near = std::max(0, std::min(1 << 20, std::atoi(v)));
-    if (const char* v = std::getenv("PMX_POINT_ORDER")) g_point_order = std::max(0, std::min(4, std::atoi(v)));
     if (const char* v = std::getenv("PMX_REFIT_SMALL")) g_refit_small = std::max(1, std::min(1024, std::atoi(v)));
 }
 
@@ -293,7 +292,7 @@ void pmx_destroy(pmx_ctx* c) {
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
-                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records, &c->nidx, &c->split, &c->gridpts};
+                      &c->tl_st, &c->tl_part, &c->tl_cells, &c->records, &c->split, &c->gridpts};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : c->ev)
@@ -373,11 +372,9 @@ int pmx_interpolate_points(pmx_ctx* c, int32_t strategy, int32_t precision, cons
         PMX_TRY(prepare_f32(c));
         int32_t* perm = nullptr;
         PMX_TRY(sort_queries(c, qx, qy, qz, n, &perm));
-        const int32_t* nidx = nullptr;
-        if (strategy != kScatter && g_point_order > 0) PMX_TRY(nearest_pass_f32c(c, qx, qy, qz, n, perm, &nidx, &perm));
         PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
         for (const CatChunk& ch : chunks)
-            PMX_TRY(interp_points_f32c(c, strategy, ch, perm, nidx, qx, qy, qz, n, c->outbuf.as<float>(), n));
+            PMX_TRY(interp_points_f32c(c, strategy, ch, perm, qx, qy, qz, n, c->outbuf.as<float>(), n));
         c->timings[8] = c->timings[9] = 0.0;
         if (strategy != kScatter && c->split_budget > 0 && c->split.p) {
             // split counters of the last pass (queued items, slots taken), capped as the kernels cap them

@@ -79,11 +79,11 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
             pts::PointSplit sp = split;
             sp.unit = lo;
             if (U.strategy == kGather)
-                pts::points_tile<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t, P, Y, sp);
+                pts::points_tile<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y, sp);
             else if (U.strategy == kScatter)
-                pts::points_tile<K, kScatter, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t, P, Y, sp);
+                pts::points_tile<K, kScatter, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y, sp);
             else
-                pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, nullptr, U.npts, U.out, U.ldo, t,
+                pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t,
                                                        P, Y, sp);
         } else {
             lmap::lines_tile<K, 0>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut, U.inv_dq, U.out, U.ldo,

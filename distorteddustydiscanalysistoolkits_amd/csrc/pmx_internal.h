@@ -69,7 +69,6 @@ struct pmx_ctx {
     pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
     pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
-    pmx::DevBuf nidx;          // nearest particle per sample (gather / symmetric, FP32c path)
     int ncu = 256;             // compute units of the device
     int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
@@ -149,9 +148,7 @@ int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, con
 int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm);
 // Leaf64 records of the current tree from cols64 (the plain FP64 path packs them on first use).
 int ensure_leaf64(pmx_ctx* c);
-int nearest_pass_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts,
-                      const int32_t* perm, const int32_t** nidx_out, int32_t** perm_out);
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm, const int32_t* nidx,
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm,
                        const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
 int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,
                       const double* o, const double* d, int64_t nl, float* out, int64_t ldo);
@@ -160,7 +157,6 @@ int prepare_f32(pmx_ctx* c);
 // nothing is written) when the lines do not qualify; the caller then uses the union walk.
 extern int g_lines_tiled;
 extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near, g_refit_small;
-extern int g_point_order;
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
                        int64_t nl, float* out, int64_t ldo, bool* used);
 int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);

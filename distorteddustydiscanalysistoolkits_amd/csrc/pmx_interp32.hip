@@ -76,13 +76,13 @@ template <int K, int S, bool DERIV>
 __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const double* __restrict__ qx,
                                                     const double* __restrict__ qy, const double* __restrict__ qz,
                                                     const int32_t* __restrict__ perm,
-                                                    const int32_t* __restrict__ nidx, int64_t npts,
+                                                    int64_t npts,
                                                     float* __restrict__ out, int64_t ldo, PointSplit sp) {
     __shared__ BPos P;
     __shared__ BTile<DERIV> Y;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        points_tile<K, S, DERIV>(T, cat, qx, qy, qz, perm, nidx, npts, out, ldo, tile, P, Y, sp);
+        points_tile<K, S, DERIV>(T, cat, qx, qy, qz, perm, npts, out, ldo, tile, P, Y, sp);
 }
 
 // Items of split tiles, taken from the queue by every wavefront until it is empty.
@@ -115,48 +115,6 @@ __global__ void k_zero_u32(unsigned int* __restrict__ p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-// Nearest-h pass over Morton-ordered sample tiles: nidx[sample] = sorted index of the nearest
-// particle (LBVH_find_nearest_h, LinearBVHQueries.jl:118-141), -1 when there is none.
-__global__ void __launch_bounds__(kWave) k_nearest_tiles(T32 T, const double* __restrict__ qx,
-                                                         const double* __restrict__ qy, const double* __restrict__ qz,
-                                                         const int32_t* __restrict__ perm, int64_t npts,
-                                                         int32_t* __restrict__ nidx) {
-    __shared__ BPos P;
-    const int64_t ntiles = (npts + kWave - 1) / kWave;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t slot = tile * kWave + threadIdx.x;
-        const bool valid = slot < npts;
-        const int64_t qi = valid ? perm[slot] : 0;
-        const PointLane q = point_lane(qx, qy, qz, valid, qi);
-        const int64_t best = nearest32(T, q, P);
-        if (valid) nidx[qi] = (int32_t)best;
-    }
-}
-
-// Tile order for the gather / symmetric walk: samples grouped by the binary exponent of their
-// h_a (so a tile's lanes have radii within a factor 2), Morton order inside a group.
-__global__ void __launch_bounds__(256) k_radius_keys(const double* __restrict__ x, const double* __restrict__ y,
-                                                     const double* __restrict__ z, int64_t npts,
-                                                     const int32_t* __restrict__ nidx, const double* __restrict__ hcol,
-                                                     const Node64* __restrict__ root, int order,
-                                                     uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
-    const Node64 r = root[1];
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < npts; i += (int64_t)gridDim.x * 256) {
-        const int32_t b = nidx[i];
-        int cls = 127;
-        if (b >= 0) {
-            int e = 0;
-            (void)frexp(hcol[b], &e);
-            cls = e + 64 < 0 ? 0 : (e + 64 > 126 ? 126 : e + 64);
-        }
-        const uint64_t m = (spread10(q10(x[i], r.mn[0], r.mx[0])) << 2) | (spread10(q10(y[i], r.mn[1], r.mx[1])) << 1) |
-                           spread10(q10(z[i], r.mn[2], r.mx[2]));
-        if (order == 3) cls = 127 - cls;
-        if (order == 4) keys[i] = ((m >> 12) << 19) | ((uint64_t)(127 - cls) << 12) | (m & 0xfff);
-        else keys[i] = ((uint64_t)cls << 30) | m;
-        vals[i] = (int32_t)i;
-    }
-}
 
 // Neighbour lists / counts of the compensated path (inspection hook): same walk and
 // decisions as k_points32, recording accepted leaves in increasing sorted index.
@@ -445,13 +403,13 @@ unsigned tile_grid(int64_t nq) {
 // split tiles, the outputs of the split tiles.
 template <int K, int S, bool DERIV>
 void launch_points32_s(pmx_ctx* c, const PointCat& cat, const double* qx, const double* qy, const double* qz,
-                       const int32_t* perm, const int32_t* nidx, int64_t npts, float* out, int64_t ldo,
+                       const int32_t* perm, int64_t npts, float* out, int64_t ldo,
                        const PointSplit& sp) {
     const dim3 g(tile_grid(npts)), b(kWave);
     T32 T = tree32(c, g_bucket_points);
     const bool split = S != kScatter && sp.budget > 0;
     if (split) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, sp.counters, 4);
-    hipLaunchKernelGGL((k_points32<K, S, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, nidx, npts, out, ldo,
+    hipLaunchKernelGGL((k_points32<K, S, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo,
                        sp);
     if (!split) return;
     hipLaunchKernelGGL((k_points32_items<K, S, DERIV>), dim3(c->ncu * 32), b, 0, c->stream, T, cat, qx, qy, qz, perm,
@@ -462,19 +420,19 @@ void launch_points32_s(pmx_ctx* c, const PointCat& cat, const double* qx, const 
 
 template <int K, bool DERIV>
 void launch_points32(pmx_ctx* c, int S, const PointCat& cat, const double* qx, const double* qy, const double* qz,
-                     const int32_t* perm, const int32_t* nidx, int64_t npts, float* out, int64_t ldo,
+                     const int32_t* perm, int64_t npts, float* out, int64_t ldo,
                      const PointSplit& sp) {
-    if (S == kGather) launch_points32_s<K, kGather, DERIV>(c, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp);
-    else if (S == kScatter) launch_points32_s<K, kScatter, DERIV>(c, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp);
-    else launch_points32_s<K, kSymmetric, DERIV>(c, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp);
+    if (S == kGather) launch_points32_s<K, kGather, DERIV>(c, cat, qx, qy, qz, perm, npts, out, ldo, sp);
+    else if (S == kScatter) launch_points32_s<K, kScatter, DERIV>(c, cat, qx, qy, qz, perm, npts, out, ldo, sp);
+    else launch_points32_s<K, kSymmetric, DERIV>(c, cat, qx, qy, qz, perm, npts, out, ldo, sp);
 }
 
 template <int K>
 void launch_points32_k(pmx_ctx* c, int S, const PointCat& cat, const double* qx, const double* qy, const double* qz,
-                       const int32_t* perm, const int32_t* nidx, int64_t npts, float* out, int64_t ldo,
+                       const int32_t* perm, int64_t npts, float* out, int64_t ldo,
                        const PointSplit& sp) {
-    if (cat.ng + cat.nd + cat.nc > 0) launch_points32<K, true>(c, S, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp);
-    else launch_points32<K, false>(c, S, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp);
+    if (cat.ng + cat.nd + cat.nc > 0) launch_points32<K, true>(c, S, cat, qx, qy, qz, perm, npts, out, ldo, sp);
+    else launch_points32<K, false>(c, S, cat, qx, qy, qz, perm, npts, out, ldo, sp);
 }
 
 // Split-tile storage of the context: counters, item queue and slots (FP64 sums of up to
@@ -555,37 +513,7 @@ int prepare_f32(pmx_ctx* c) {
     return PMX_OK;
 }
 
-// Gather / symmetric: nearest-h for every sample first (Morton-ordered tiles, perm), then the
-// samples re-tiled by radius class (nidx in c->nidx, the new order in *perm_out).
-int g_point_order = 0;
-
-int nearest_pass_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts,
-                      const int32_t* perm, const int32_t** nidx_out, int32_t** perm_out) {
-    PMX_TRY(ensure(c, c->nidx, 4 * (size_t)npts + 64));
-    int32_t* nidx = c->nidx.as<int32_t>();
-    hipLaunchKernelGGL(k_nearest_tiles, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_points), qx,
-                       qy, qz, perm, npts, nidx);
-    // perm lives in scratch_a / scratch_b; it is consumed above before the keys overwrite it
-    uint64_t* k0 = c->scratch_a.as<uint64_t>();
-    int32_t* v0 = reinterpret_cast<int32_t*>(k0 + npts);
-    uint64_t* k1 = c->scratch_b.as<uint64_t>();
-    int32_t* v1 = reinterpret_cast<int32_t*>(k1 + npts);
-    const dim3 g(grid_for(npts, 256, 8192)), b(256);
-    if (c->tree.n >= 2 && g_point_order >= 2) {
-        hipLaunchKernelGGL(k_radius_keys, g, b, 0, c->stream, qx, qy, qz, npts, nidx,
-                           c->cols64.as<double>() + 4 * c->tree.n, c->nodes64.as<Node64>(), g_point_order, k0, v0);
-        bool alt = false;
-        PMX_TRY(radix_sort_pairs(c, k0, v0, npts, k1, v1, 0, 40, &alt));
-        *perm_out = alt ? v1 : v0;
-    } else {
-        *perm_out = const_cast<int32_t*>(perm);
-    }
-    *nidx_out = nidx;
-    PMX_HIP(c, hipGetLastError());
-    return PMX_OK;
-}
-
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32_t* perm, const int32_t* nidx,
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32_t* perm,
                        const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo) {
     // split the chunk into sub-chunks of <= 2 scalars, 1 gradient, 1 divergence, 1 curl
     const int nsub = std::max({(ch.ns + 1) / 2, ch.ng, ch.nd, ch.nc, 1});
@@ -613,12 +541,12 @@ int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32
         PointSplit sp;
         PMX_TRY(point_split(c, npts, strategy, &sp));
         switch (c->tree.kernel) {
-        case kM4: launch_points32_k<kM4>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
-        case kM5: launch_points32_k<kM5>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
-        case kM6: launch_points32_k<kM6>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
-        case kC2: launch_points32_k<kC2>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
-        case kC4: launch_points32_k<kC4>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
-        default: launch_points32_k<kC6>(c, strategy, cat, qx, qy, qz, perm, nidx, npts, out, ldo, sp); break;
+        case kM4: launch_points32_k<kM4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
+        case kM5: launch_points32_k<kM5>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
+        case kM6: launch_points32_k<kM6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
+        case kC2: launch_points32_k<kC2>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
+        case kC4: launch_points32_k<kC4>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
+        default: launch_points32_k<kC6>(c, strategy, cat, qx, qy, qz, perm, npts, out, ldo, sp); break;
         }
     }
     PMX_HIP(c, hipGetLastError());

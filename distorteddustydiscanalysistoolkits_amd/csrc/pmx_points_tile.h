@@ -661,7 +661,7 @@ __device__ inline unsigned int lane0_atomic_add(unsigned int* p, unsigned int v)
 }
 
 // One tile of 64 samples (a wavefront; perm gives the samples of tile t).  Each lane first
-// finds its nearest-h particle (gather / symmetric; or takes it from nidx).  The union walk
+// finds its nearest-h particle (gather / symmetric).  The union walk
 // then stages leaf runs in LDS; every lane tests the run's particles against its own sample
 // (FP32 with a guard band, exact FP64 inside the band) and appends the accepted pairs to a
 // queue, and the wavefront works through the queue 64 pairs at a time, one pair per lane,
@@ -671,8 +671,8 @@ __device__ inline unsigned int lane0_atomic_add(unsigned int* p, unsigned int v)
 template <int K, int S, bool DERIV>
 __device__ inline void points_tile(const T32& T, const PointCat& cat, const double* __restrict__ qx,
                                    const double* __restrict__ qy, const double* __restrict__ qz,
-                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ nidx,
-                                   int64_t npts, float* __restrict__ out, int64_t ldo, int64_t tile, BPos& P,
+                                   const int32_t* __restrict__ perm, int64_t npts, float* __restrict__ out,
+                                   int64_t ldo, int64_t tile, BPos& P,
                                    BTile<DERIV>& Y, const PointSplit& sp) {
     const int lane = threadIdx.x;
     const int64_t slot = tile * kWave + lane;
@@ -680,7 +680,7 @@ __device__ inline void points_tile(const T32& T, const PointCat& cat, const doub
     const int64_t qi = valid ? perm[slot] : 0;
     const PointLane q = point_lane(qx, qy, qz, valid, qi);
     int64_t nearest = -1;
-    if (S != kScatter) nearest = nidx ? (valid ? (int64_t)nidx[qi] : -1) : nearest32(T, q, P);
+    if (S != kScatter) nearest = nearest32(T, q, P);
 
     PointWalk<K, S, DERIV> W(T, cat, P, Y, q);
     W.setup(nearest);
