@@ -10,12 +10,15 @@
 //     guard band decides the clear cases; inside the band the reference's FP64 formula is
 //     evaluated on the FP64 particle columns.  Node boxes are rounded outwards and hmax up,
 //     so pruning is conservative: the accepted set is the FP64 predicate set.
-//   * Point weights, gradients and field terms are FP32 with compensated (TwoSum) sums per
-//     sample; derivative sums accumulate A_b - A_ref, formed in FP64 and rounded once, with
-//     A_ref the sample's nearest particle (first accepted neighbour for scatter), which
-//     leaves f - A~ b unchanged and removes the cancellation of large means.  Line maps test pairs in FP32; accepted pairs get their
-//     perpendicular distance, weight and column sums in FP64 (the weight is a small share of
-//     the work), and all Shepard divisions are FP64.
+//   * Accepted (sample, particle) pairs go through an LDS queue and are evaluated 64 at a
+//     time, one pair per lane, into FP64 LDS sums.  Scalar catalogs weigh with FP32 kernel
+//     shapes; derivative catalogs form their pair terms in FP64 (W, grad W, separation,
+//     A_b - A_ref with A_ref the sample's nearest particle, or first accepted neighbour for
+//     scatter, which leaves f - A~ b unchanged and removes the cancellation of large means).
+//     Heavy tiles hand their remaining subtrees to an item queue walked by all wavefronts.
+//     Line maps of one parallel set use the particle-centric tiles (pmx_lines_tiled.hip);
+//     other lines test pairs in FP32 and evaluate accepted pairs in FP64.  All Shepard
+//     divisions are FP64.
 // Reference semantics: general_interpolation.jl:1-669 (points), line_integrated_scalar_
 // interpolation.jl:123-166 (lines), LinearBVHQueries.jl:118-141 (nearest-h).
 #include <hip/hip_runtime.h>

@@ -123,18 +123,6 @@ __device__ inline void shape_fp64(double q, double e, double& f, double& p) {
     }
 }
 
-// Compensated accumulator (TwoSum; -ffp-contract=off keeps it exact).
-struct CSum {
-    float s, c;
-    __device__ inline void add(float x) {
-        const float t = s + x;
-        const float bp = t - s;
-        c += (s - (t - bp)) + (x - bp);
-        s = t;
-    }
-    __device__ inline double value() const { return (double)s + (double)c; }
-};
-
 // ---------------------------------------------------------------------------------------
 // Points
 // ---------------------------------------------------------------------------------------
