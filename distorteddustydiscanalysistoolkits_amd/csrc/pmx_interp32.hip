@@ -82,10 +82,17 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                                                     int64_t npts,
                                                     float* __restrict__ out, int64_t ldo, PointSplit sp) {
     __shared__ BPos P;
-    __shared__ BTile<DERIV> Y;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        points_tile<K, S, DERIV>(T, cat, qx, qy, qz, perm, npts, out, ldo, tile, P, Y, sp);
+    if constexpr (DERIV) {
+        extern __shared__ double dyn_tile[];        // deriv_tile_lds(cat.nder, cat.nacc) bytes
+        BTile<true> Y(dyn_tile, cat.nder, cat.nacc);
+        for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+            points_tile<K, S, true>(T, cat, qx, qy, qz, perm, npts, out, ldo, tile, P, Y, sp);
+    } else {
+        __shared__ BTile<false> Y;
+        for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+            points_tile<K, S, false>(T, cat, qx, qy, qz, perm, npts, out, ldo, tile, P, Y, sp);
+    }
 }
 
 // Items of split tiles, taken from the queue by every wavefront until it is empty.
@@ -96,12 +103,22 @@ __global__ void __launch_bounds__(kWave) k_points32_items(T32 T, PointCat cat, c
                                                           const int32_t* __restrict__ perm, int64_t npts,
                                                           PointSplit sp) {
     __shared__ BPos P;
-    __shared__ BTile<DERIV> Y;
     const unsigned int nitems = min(sp.counters[0], (unsigned int)sp.cap_items);
-    for (;;) {
-        const unsigned int item = lane0_atomic_add(&sp.counters[2], 1u);
-        if (item >= nitems) break;
-        points_item<K, S, DERIV>(T, cat, qx, qy, qz, perm, npts, sp, item, P, Y);
+    if constexpr (DERIV) {
+        extern __shared__ double dyn_tile[];
+        BTile<true> Y(dyn_tile, cat.nder, cat.nacc);
+        for (;;) {
+            const unsigned int item = lane0_atomic_add(&sp.counters[2], 1u);
+            if (item >= nitems) break;
+            points_item<K, S, true>(T, cat, qx, qy, qz, perm, npts, sp, item, P, Y);
+        }
+    } else {
+        __shared__ BTile<false> Y;
+        for (;;) {
+            const unsigned int item = lane0_atomic_add(&sp.counters[2], 1u);
+            if (item >= nitems) break;
+            points_item<K, S, false>(T, cat, qx, qy, qz, perm, npts, sp, item, P, Y);
+        }
     }
 }
 
@@ -411,12 +428,13 @@ void launch_points32_s(pmx_ctx* c, const PointCat& cat, const double* qx, const 
     const dim3 g(tile_grid(npts)), b(kWave);
     T32 T = tree32(c, g_bucket_points);
     const bool split = S != kScatter && sp.budget > 0;
+    const size_t lds = DERIV ? deriv_tile_lds(cat.nder, cat.nacc) : 0;
     if (split) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, sp.counters, 4);
-    hipLaunchKernelGGL((k_points32<K, S, DERIV>), g, b, 0, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo,
+    hipLaunchKernelGGL((k_points32<K, S, DERIV>), g, b, lds, c->stream, T, cat, qx, qy, qz, perm, npts, out, ldo,
                        sp);
     if (!split) return;
-    hipLaunchKernelGGL((k_points32_items<K, S, DERIV>), dim3(c->ncu * 32), b, 0, c->stream, T, cat, qx, qy, qz, perm,
-                       npts, sp);
+    hipLaunchKernelGGL((k_points32_items<K, S, DERIV>), dim3(c->ncu * 32), b, lds, c->stream, T, cat, qx, qy, qz,
+                       perm, npts, sp);
     hipLaunchKernelGGL(k_points32_finish<DERIV>, dim3(std::min(sp.cap_slots, 4096)), b, 0, c->stream, cat, perm, npts,
                        out, ldo, sp);
 }
@@ -540,6 +558,24 @@ int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32
         cat.og = ch.og + 3 * k;
         cat.od = ch.od + k;
         cat.oc = ch.oc + 3 * k;
+        // LDS rows of the sums a derivative launch holds
+        for (int a = 0; a < kAccMax; ++a) cat.amap[a] = -1;
+        int rows = 0;
+        auto hold = [&](int a, int cnt) { for (int j = 0; j < cnt; ++j) cat.amap[a + j] = rows++; };
+        hold(kA_S1, 1);
+        hold(kA_SC, cat.ns);
+        hold(kA_B, 3);
+        if (cat.ng) { hold(kA_GF, 3); hold(kA_GS, 1); }
+        if (cat.nd) { hold(kA_DF, 1); hold(kA_DS, 3); }
+        if (cat.nc) {
+            hold(kA_CF, 3);
+            cat.cs_alias = cat.nd && cat.cf[0] == cat.df[0] && cat.cf[1] == cat.df[1] && cat.cf[2] == cat.df[2];
+            if (cat.cs_alias)
+                for (int j = 0; j < 3; ++j) cat.amap[kA_CS + j] = cat.amap[kA_DS + j];
+            else
+                hold(kA_CS, 3);
+        }
+        cat.nacc = rows;
         if (cat.ns + cat.ng + cat.nd + cat.nc == 0 && k > 0) continue;
         PointSplit sp;
         PMX_TRY(point_split(c, npts, strategy, &sp));

@@ -258,12 +258,17 @@ struct PointCat {
     int df[3];
     int cf[3];
     int os, og, od, oc;
+    // derivative launches: the sums held per sample (LDS rows) and each logical sum's row
+    // (-1: not held).  When the curl and divergence fields coincide, the curl's sums of A w
+    // reuse the divergence's rows (cs_alias).
+    int nacc, cs_alias;
+    int amap[20];
 };
 
-// LDS of one sample tile.  Scalar catalogs (DERIV false): the staged payload of a leaf run,
-// the tile's samples (read by the lanes that process their pairs), the queue of accepted
-// (sample, particle) pairs and the FP64 accumulators of every sample.  Derivative catalogs
-// (DERIV true): the staged payload only; each lane keeps its sample's sums in registers.
+// LDS of one sample tile: the staged payload of a leaf run, the tile's samples (read by the
+// lanes that process their pairs), the queue of accepted (sample, particle) pairs and the
+// FP64 sums of every sample.  Scalar catalogs (DERIV false) use a fixed layout; derivative
+// catalogs (DERIV true) a layout sized by the catalog in dynamic LDS.
 template <bool DERIV>
 struct BTile;
 
@@ -278,18 +283,39 @@ struct BTile<false> {
     uint16_t pair[kQueue];
 };
 
+// Derivative tiles: views of the dynamic LDS, sized per launch by the catalog (deriv_tile_lds):
+// staged m/rho, 1/h and derivative fields of a leaf run; the tile's samples; the held sums;
+// the pair queue.
 template <>
 struct BTile<true> {
-    double w64[kMaxBucket];    // m / rho in FP64
-    double ih64[kMaxBucket];
-    float s[2][kMaxBucket];
-    double d[kMaxD][kMaxBucket];
-    float ph[3][kWave], pl[3][kWave];
-    double invha[kWave];
-    double ref[kMaxD][kWave];
-    double acc[20][kWave];
-    uint16_t pair[kQueue];
+    double* w64;
+    double* ih64;
+    double* d;        // [nder][64]
+    double* invha;    // [64]
+    double* acc;      // [nacc][64]
+    float* s;         // [2][64]
+    float* ph;        // [3][64]
+    float* pl;        // [3][64]
+    uint16_t* pair;   // [kQueue]
+    __device__ BTile(void* base, int nder, int nacc) {
+        char* p = static_cast<char*>(base);
+        w64 = reinterpret_cast<double*>(p); p += 8 * kMaxBucket;
+        ih64 = reinterpret_cast<double*>(p); p += 8 * kMaxBucket;
+        d = reinterpret_cast<double*>(p); p += 8 * kMaxBucket * (size_t)nder;
+        invha = reinterpret_cast<double*>(p); p += 8 * kWave;
+        acc = reinterpret_cast<double*>(p); p += 8 * kWave * (size_t)nacc;
+        s = reinterpret_cast<float*>(p); p += 4 * 2 * kMaxBucket;
+        ph = reinterpret_cast<float*>(p); p += 4 * 3 * kWave;
+        pl = reinterpret_cast<float*>(p); p += 4 * 3 * kWave;
+        pair = reinterpret_cast<uint16_t*>(p);
+    }
 };
+
+__host__ __device__ inline size_t deriv_tile_lds(int nder, int nacc) {
+    return 8 * (size_t)(2 * kMaxBucket + kMaxBucket * nder + kWave + kWave * nacc) + 4 * (size_t)(2 * kMaxBucket + 6 * kWave) +
+           2 * (size_t)kQueue;
+}
+
 
 // accumulator slots: 0 = sum W (Shepard), 1.. = scalars, then the derivative sums
 enum : int { kA_S1 = 0, kA_SC = 1, kA_B = 3, kA_GF = 6, kA_GS = 9, kA_DF = 10, kA_DS = 11, kA_CF = 14, kA_CS = 17 };
@@ -438,10 +464,12 @@ struct PointWalk<K, S, false> {
 };
 
 // Derivative catalogs: the same pair queue with FP64 pair terms (W, sigma h^-5 f'(q)/q, the
-// separation, the field differences A_b - A_ref; the derivative sums cancel, and FP32 terms
-// left 4e-6 on a nearly solenoidal velocity field) and about 20 FP64 sums per sample.  Keeping
-// the sums in registers instead (each lane evaluating its own accepted particles) needs 255
-// VGPRs, one wavefront per SIMD; the queue keeps them in LDS.
+// separation and the fields; FP32 terms left 4e-6 on a nearly solenoidal velocity field,
+// whose divergence is far below its pair terms) into FP64 LDS sums.  With FP64 terms the
+// sums need no per-sample field reference: the cancellation of f - A~ b costs digits of an
+// FP64 result.  Keeping the sums in registers instead (each lane evaluating its own accepted
+// particles) needs 255 VGPRs, one wavefront per SIMD; the LDS holds only the sums the catalog
+// uses.
 template <int K, int S>
 struct PointWalk<K, S, true> {
     static constexpr int kAcc = kAccMax;
@@ -453,21 +481,17 @@ struct PointWalk<K, S, true> {
     int lane;
     double Kv, radius2_64;
     float Kvf, radius2_32;
-    bool have_ref;
     int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
     int spent;                  // leaves tested so far (wave-uniform)
 
     __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<true>& Y_, const PointLane& q_)
         : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
-          Kvf((float)support(K)), radius2_32(0.0f), have_ref(false), qhead(0), qtail(0), spent(0) {}
+          Kvf((float)support(K)), radius2_32(0.0f), qhead(0), qtail(0), spent(0) {}
 
-    // lane state in LDS; the references are the nearest particle (gather / symmetric) or the
-    // first accepted neighbour (scatter)
     __device__ void setup(int64_t nearest) {
-        const int64_t n = T.n;
         double invha64 = 0.0;
         if (S != kScatter) {
-            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * n + nearest];
+            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * T.n + nearest];
             const double radius = Kv * ha64;
             radius2_64 = radius * radius;
             radius2_32 = (float)radius2_64;
@@ -475,27 +499,25 @@ struct PointWalk<K, S, true> {
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { Y.ph[k][lane] = q.ph[k]; Y.pl[k][lane] = q.pl[k]; }
+        for (int k = 0; k < 3; ++k) { Y.ph[k * kWave + lane] = q.ph[k]; Y.pl[k * kWave + lane] = q.pl[k]; }
         Y.invha[lane] = invha64;
-#pragma unroll
-        for (int a = 0; a < kAcc; ++a) Y.acc[a][lane] = 0.0;
-        if (nearest >= 0) {
-            for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + nearest];
-            have_ref = true;
-        }
+        for (int a = 0; a < cat.nacc; ++a) Y.acc[a * kWave + lane] = 0.0;
     }
 
     __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
 
-    __device__ double sum(int a) const { return Y.acc[a][lane]; }
+    __device__ double sum(int a) const { return cat.amap[a] < 0 ? 0.0 : Y.acc[cat.amap[a] * kWave + lane]; }
+
+    __device__ void add(int a, int sa, double v) { atomicAdd(&Y.acc[cat.amap[a] * kWave + sa], v); }
 
     __device__ void process(int cnt) {
         __syncthreads();
         if (lane < cnt) {
             const int pr = Y.pair[(qhead + lane) & (kQueue - 1)];
             const int sa = pr & (kWave - 1), j = pr >> 6;
-            const float qh0 = Y.ph[0][sa], qh1 = Y.ph[1][sa], qh2 = Y.ph[2][sa];
-            const float ql0 = Y.pl[0][sa], ql1 = Y.pl[1][sa], ql2 = Y.pl[2][sa];
+            const float qh0 = Y.ph[sa], qh1 = Y.ph[kWave + sa], qh2 = Y.ph[2 * kWave + sa];
+            const float ql0 = Y.pl[sa], ql1 = Y.pl[kWave + sa], ql2 = Y.pl[2 * kWave + sa];
+            // separation in FP64 from the hi / lo parts (both differences exact in FP64)
             const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
             const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
             const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
@@ -526,41 +548,41 @@ struct PointWalk<K, S, true> {
             }
             const double wb = Y.w64[j];
             const double w = wb * W;
-            atomicAdd(&Y.acc[kA_S1][sa], w);
-            if (cat.ns > 0) atomicAdd(&Y.acc[kA_SC][sa], (double)Y.s[0][j] * w);
-            if (cat.ns > 1) atomicAdd(&Y.acc[kA_SC + 1][sa], (double)Y.s[1][j] * w);
+            add(kA_S1, sa, w);
+            if (cat.ns > 0) add(kA_SC, sa, (double)Y.s[j] * w);
+            if (cat.ns > 1) add(kA_SC + 1, sa, (double)Y.s[kMaxBucket + j] * w);
             // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
             const double cg = wb * G;
             const double gx = cg * ex, gy = cg * ey, gz = cg * ez;
-            atomicAdd(&Y.acc[kA_B][sa], gx);
-            atomicAdd(&Y.acc[kA_B + 1][sa], gy);
-            atomicAdd(&Y.acc[kA_B + 2][sa], gz);
+            add(kA_B, sa, gx);
+            add(kA_B + 1, sa, gy);
+            add(kA_B + 2, sa, gz);
             if (cat.ng) {
-                const double A = Y.d[cat.gf][j] - Y.ref[cat.gf][sa];
-                atomicAdd(&Y.acc[kA_GF][sa], A * gx);
-                atomicAdd(&Y.acc[kA_GF + 1][sa], A * gy);
-                atomicAdd(&Y.acc[kA_GF + 2][sa], A * gz);
-                atomicAdd(&Y.acc[kA_GS][sa], A * w);
+                const double A = Y.d[cat.gf * kMaxBucket + j];
+                add(kA_GF, sa, A * gx);
+                add(kA_GF + 1, sa, A * gy);
+                add(kA_GF + 2, sa, A * gz);
+                add(kA_GS, sa, A * w);
             }
             if (cat.nd) {
-                const double Ax = Y.d[cat.df[0]][j] - Y.ref[cat.df[0]][sa];
-                const double Ay = Y.d[cat.df[1]][j] - Y.ref[cat.df[1]][sa];
-                const double Az = Y.d[cat.df[2]][j] - Y.ref[cat.df[2]][sa];
-                atomicAdd(&Y.acc[kA_DF][sa], (Ax * gx + Ay * gy) + Az * gz);
-                atomicAdd(&Y.acc[kA_DS][sa], Ax * w);
-                atomicAdd(&Y.acc[kA_DS + 1][sa], Ay * w);
-                atomicAdd(&Y.acc[kA_DS + 2][sa], Az * w);
+                const double Ax = Y.d[cat.df[0] * kMaxBucket + j], Ay = Y.d[cat.df[1] * kMaxBucket + j],
+                             Az = Y.d[cat.df[2] * kMaxBucket + j];
+                add(kA_DF, sa, (Ax * gx + Ay * gy) + Az * gz);
+                add(kA_DS, sa, Ax * w);
+                add(kA_DS + 1, sa, Ay * w);
+                add(kA_DS + 2, sa, Az * w);
             }
             if (cat.nc) {
-                const double Ax = Y.d[cat.cf[0]][j] - Y.ref[cat.cf[0]][sa];
-                const double Ay = Y.d[cat.cf[1]][j] - Y.ref[cat.cf[1]][sa];
-                const double Az = Y.d[cat.cf[2]][j] - Y.ref[cat.cf[2]][sa];
-                atomicAdd(&Y.acc[kA_CF][sa], Ay * gz - Az * gy);
-                atomicAdd(&Y.acc[kA_CF + 1][sa], Az * gx - Ax * gz);
-                atomicAdd(&Y.acc[kA_CF + 2][sa], Ax * gy - Ay * gx);
-                atomicAdd(&Y.acc[kA_CS][sa], Ax * w);
-                atomicAdd(&Y.acc[kA_CS + 1][sa], Ay * w);
-                atomicAdd(&Y.acc[kA_CS + 2][sa], Az * w);
+                const double Ax = Y.d[cat.cf[0] * kMaxBucket + j], Ay = Y.d[cat.cf[1] * kMaxBucket + j],
+                             Az = Y.d[cat.cf[2] * kMaxBucket + j];
+                add(kA_CF, sa, Ay * gz - Az * gy);
+                add(kA_CF + 1, sa, Az * gx - Ax * gz);
+                add(kA_CF + 2, sa, Ax * gy - Ay * gx);
+                if (!cat.cs_alias) {
+                    add(kA_CS, sa, Ax * w);
+                    add(kA_CS + 1, sa, Ay * w);
+                    add(kA_CS + 2, sa, Az * w);
+                }
             }
         }
         qhead += cnt;
@@ -574,8 +596,9 @@ struct PointWalk<K, S, true> {
             const int64_t li = lo + lane;
             Y.w64[lane] = T.w64[li];
             Y.ih64[lane] = T.ih64[li];
-            for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
-            for (int f = 0; f < cat.nder; ++f) Y.d[f][lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
+            for (int f = 0; f < cat.ns; ++f) Y.s[f * kMaxBucket + lane] = T.pay[(int64_t)cat.scol[f] * n + li];
+            for (int f = 0; f < cat.nder; ++f)
+                Y.d[f * kMaxBucket + lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
         }
         __syncthreads();
         for (int j = 0; j < cnt; ++j) {
@@ -589,10 +612,6 @@ struct PointWalk<K, S, true> {
                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
                                (kQueue - 1);
                 Y.pair[at] = (uint16_t)(lane | (j << 6));
-                if (!have_ref) {
-                    for (int f = 0; f < cat.nder; ++f) Y.ref[f][lane] = Y.d[f][j];
-                    have_ref = true;
-                }
             }
             qtail += __builtin_popcountll(m);
             if (qtail - qhead >= kWave) process(kWave);
