@@ -51,18 +51,21 @@ __device__ inline void project_box(const Node32& b, const double a[3], double& l
 }
 
 // LDS of one line-map tile: CSR starts of the tile's cells and the sweep state (one particle
-// per lane, and the owner lane of each (particle, row) item).  The line data, accumulators
-// and particle fields live in the caller's dynamic LDS (`dyn`, lines_tile_lds bytes).
+// per lane, and the owner lane of each (particle, row) item).  The accumulators and particle
+// fields live in the caller's dynamic LDS (`dyn`, lines_tile_lds bytes).  The lines' (s, t)
+// are read from the sorted global array (1 KB per chunk, cache-resident): a global read waits
+// on the vector-memory counter, so prefetching the next line does not wait for the LDS
+// accumulator atomics issued after it.
 struct alignas(16) LineTileLDS {
     int32_t cs[kCells + 1];
-    double w_sb[64], w_tb[64], w_r2[64], w_wb[64], w_ih[64];
-    int32_t w_p[64], w_cx0[64], w_cx1[64], w_cy0[64], w_start[64];
+    double w_sb[64], w_tb[64], w_r2[64], w_wih[64], w_xs[64];
+    int32_t w_p[64], w_cc[64], w_start[64];   // w_cc: cx0 | cx1 << 8 | cy0 << 16 (tile-local)
     int32_t pend[128];                  // particles that reach the tile, not yet swept
     unsigned char w_own[64 * kTS];
 };
 
 __host__ __device__ inline size_t lines_tile_lds(int ncol, int mode) {
-    return sizeof(double) * (size_t)kCap * (2 + (mode == 0 ? 1 + ncol : 3)) + sizeof(float) * 64 * (size_t)(mode == 0 ? ncol : 0);
+    return sizeof(double) * (size_t)kCap * (mode == 0 ? 1 + ncol : 3) + sizeof(float) * 64 * (size_t)(mode == 0 ? ncol : 0);
 }
 
 // One 8 x 8-cell tile of a line map (a wavefront).  MODE 0: interpolation (FP64 weight sums).
@@ -75,15 +78,14 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                                   double inv_dq, float* __restrict__ out, int64_t ldo, uint64_t* __restrict__ digest,
                                   int64_t tile, LineTileLDS& L, double* dyn) {
     int32_t* cs = L.cs;
-    double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wb = L.w_wb, *w_ih = L.w_ih;
-    int32_t *w_p = L.w_p, *w_cx0 = L.w_cx0, *w_cx1 = L.w_cx1, *w_cy0 = L.w_cy0, *w_start = L.w_start;
+    double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wih = L.w_wih, *w_xs = L.w_xs;
+    int32_t *w_p = L.w_p, *w_cc = L.w_cc, *w_start = L.w_start;
     unsigned char* w_own = L.w_own;
     const int lane = threadIdx.x;
     const int nacc = MODE == 0 ? 1 + cat.n : 3;
-    double2* ST = reinterpret_cast<double2*>(dyn);     // (s, t) of the chunk's lines
-    double* acc = dyn + 2 * kCap;
-    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn + 2 * kCap);
-    float* w_A = reinterpret_cast<float*>(dyn + (2 + nacc) * kCap);     // [ncol][64]
+    double* acc = dyn;
+    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn);
+    float* w_A = reinterpret_cast<float*>(dyn + nacc * kCap);     // [ncol][64]
     const double Kv = support(K);
     const double xmax = Kv * inv_dq;       // support in table steps
     const int64_t n = T.n;
@@ -106,10 +108,6 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     for (int C0 = L0; C0 < L1; C0 += kCap) {
         const int m = min(kCap, L1 - C0);
         for (int k = lane; k < kCap; k += 64) {
-            if (k < m) {
-                const double2 v = st_sorted[C0 + k];
-                ST[k] = v;
-            }
             for (int a = 0; a < nacc; ++a) {
                 if (MODE == 0) acc[a * kCap + k] = 0.0;
                 else dg[a * kCap + k] = 0ull;
@@ -161,12 +159,12 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                         w_tb[lane] = tb;
                         w_r2[lane] = r2;
                         w_p[lane] = (int32_t)p;
-                        w_cx0[lane] = cx0;
-                        w_cx1[lane] = cx1;
-                        w_cy0[lane] = cy0;
+                        w_cc[lane] = cx0 | cx1 << 8 | cy0 << 16;
                         if (MODE == 0) {
-                            w_wb[lane] = T.w64[p];
-                            w_ih[lane] = T.ih64[p];
+                            // (m/rho) / h and 1 / (h dq) of line_integrated_kernel_function
+                            const double invh = T.ih64[p];
+                            w_wih[lane] = T.w64[p] * invh;
+                            w_xs[lane] = invh * inv_dq;
                             for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                         }
                     }
@@ -186,9 +184,10 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             __syncthreads();
             for (int t = lane; t < total; t += 64) {
                 const int j = w_own[t];
-                const int row = w_cy0[j] + (t - w_start[j]);
-                const int a = max(cs[row * kTS + w_cx0[j]], C0);
-                const int b = min(cs[row * kTS + w_cx1[j] + 1], C0 + m);
+                const int cc = w_cc[j];
+                const int row = (cc >> 16) + (t - w_start[j]);
+                const int a = max(cs[row * kTS + (cc & 0xff)], C0);
+                const int b = min(cs[row * kTS + ((cc >> 8) & 0xff) + 1], C0 + m);
                 if (a >= b) continue;
                 const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
                 const double r2hi = r2 + f.band, r2lo = r2 - f.band;
@@ -197,25 +196,28 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                 double wih = 0.0, xscale = 0.0;
                 float A0 = 0.0f, A1 = 0.0f;
                 if (MODE == 0) {
-                    const double invh = w_ih[j];
-                    wih = w_wb[j] * invh;
-                    xscale = invh * inv_dq;
+                    wih = w_wih[j];
+                    xscale = w_xs[j];
                     if (cat.n > 0) A0 = w_A[j];
                     if (cat.n > 1) A1 = w_A[64 + j];
                 }
                 // lanes on the same row start at different columns (fewer LDS bank conflicts)
                 const int len = b - a;
                 int k = a + (lane % len);
-                for (int it = 0; it < len; ++it, k = (k + 1 == b) ? a : k + 1) {
+                // the next line's (s, t) is read one pair ahead
+                double2 Lnext = st_sorted[k];
+                for (int it = 0; it < len; ++it) {
                     const int kk = k - C0;
-                    const double2 L = ST[kk];
+                    const double2 L = Lnext;
+                    k = (k + 1 == b) ? a : k + 1;
+                    Lnext = st_sorted[k];
                     const double ds = sb - L.x, dt = tb - L.y;
                     const double d2 = ds * ds + dt * dt;
                     if (d2 > r2hi) continue;
                     bool ok = d2 < r2lo;
                     if (!ok) {
                         // reference formula on the FP64 columns (_squared_distance_line_coords)
-                        const int64_t li = perm[k];
+                        const int64_t li = perm[C0 + kk];
                         double D2 = 0.0, Dm = 0.0, e;
                         e = T.cols64[pj] - o[li]; D2 += e * e; Dm += e * f.d[0];
                         e = T.cols64[n + pj] - o[nl + li]; D2 += e * e; Dm += e * f.d[1];
@@ -224,14 +226,14 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     }
                     if (!ok) continue;
                     if (MODE == 0) {
-                        // line_integrated_kernel_function: invh * Inorm(r invh).  r from an FP32
-                        // reciprocal square root and one FP64 correction step (relative error
-                        // ~1e-14), q and the table interpolation in FP64 on FP32 table values:
-                        // FP32 q would carry ~1e-7 relative error, amplified by dlnI/dlnq
-                        const float d2f = (float)d2;
-                        const float rs = d2f > 0.0f ? __builtin_amdgcn_rsqf(d2f) : 0.0f;
-                        const double r0 = (double)(d2f * rs);
-                        const double r = r0 + (d2 - r0 * r0) * (0.5 * (double)rs);
+                        // line_integrated_kernel_function: invh * Inorm(r invh).  r from the
+                        // hardware FP64 reciprocal square root (relative error ~2^-23) and one
+                        // Newton step (~1e-14; d2 = 0 gives r = 0 through the clamp), q and the
+                        // table interpolation in FP64 on FP32 table values: FP32 q would carry
+                        // ~1e-7 relative error, amplified by dlnI/dlnq
+                        const double y = __builtin_amdgcn_rsq(fmax(d2, 1e-300));
+                        const double r0 = d2 * y;
+                        const double r = fma(fma(-r0, r0, d2), 0.5 * y, r0);
                         const double x = r * xscale;             // q / dq
                         double I = 0.0;
                         if (x < xmax) {
@@ -241,7 +243,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                             float2 Lv;                                 // lut[i], lut[i + 1]
                             __builtin_memcpy(&Lv, lut + i, sizeof(Lv));
                             const double L0 = (double)Lv.x;
-                            I = L0 + t * ((double)Lv.y - L0);
+                            I = fma(t, (double)Lv.y - L0, L0);
                         }
                         const double w = wih * I;
                         atomicAdd(&acc[kk], w);
