@@ -57,10 +57,11 @@ __device__ inline void project_box(const Node32& b, const double a[3], double& l
 // on the vector-memory counter, so prefetching the next line does not wait for the LDS
 // accumulator atomics issued after it.
 struct alignas(16) LineTileLDS {
-    int32_t cs[kCells + 1];
     double w_sb[64], w_tb[64], w_r2[64], w_wih[64], w_xs[64];
-    int32_t w_p[64], w_cc[64], w_start[64];   // w_cc: cx0 | cx1 << 8 | cy0 << 16 (tile-local)
+    int32_t cs[kCells + 1];
+    int32_t w_p[64], w_cc[64];          // w_cc: cx0 | cx1 << 8 | cy0 << 16 (tile-local)
     int32_t pend[128];                  // particles that reach the tile, not yet swept
+    uint16_t w_start[64];
     unsigned char w_own[64 * kTS];
 };
 
@@ -68,7 +69,9 @@ __host__ __device__ inline size_t lines_tile_lds(int ncol, int mode) {
     return sizeof(double) * (size_t)kCap * (mode == 0 ? 1 + ncol : 3) + sizeof(float) * 64 * (size_t)(mode == 0 ? ncol : 0);
 }
 
-// One 8 x 8-cell tile of a line map (a wavefront).  MODE 0: interpolation (FP64 weight sums).
+// One 8 x 8-cell tile of a line map (one wavefront; other wavefronts of the workgroup work on
+// other tiles, so only wavefront barriers are used).  `lut` is the kernel's Inorm table (in LDS
+// or global memory).  MODE 0: interpolation (FP64 weight sums).
 // MODE 1: neighbour digest per line (count, sum of sorted particle indices, sum of their
 // squares; unsigned 64-bit, wrapping).
 template <int K, int MODE>
@@ -79,9 +82,10 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                                   int64_t tile, LineTileLDS& L, double* dyn) {
     int32_t* cs = L.cs;
     double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wih = L.w_wih, *w_xs = L.w_xs;
-    int32_t *w_p = L.w_p, *w_cc = L.w_cc, *w_start = L.w_start;
+    int32_t *w_p = L.w_p, *w_cc = L.w_cc;
+    uint16_t* w_start = L.w_start;
     unsigned char* w_own = L.w_own;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int nacc = MODE == 0 ? 1 + cat.n : 3;
     double* acc = dyn;
     unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn);
@@ -90,10 +94,10 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     const double xmax = Kv * inv_dq;       // support in table steps
     const int64_t n = T.n;
 
-    __syncthreads();
+    wave_sync();
     cs[lane] = cstart[tile * kCells + lane];
     if (lane == 0) cs[kCells] = cstart[(tile + 1) * kCells];
-    __syncthreads();
+    wave_sync();
     const int L0 = cs[0], L1 = cs[kCells];
     if (L0 >= L1) return;
     const int txi = (int)(tile % f.tx), tyi = (int)(tile / f.tx);
@@ -113,7 +117,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                 else dg[a * kCap + k] = 0ull;
             }
         }
-        __syncthreads();
+        wave_sync();
 
         auto need = [&](const Node32& nd) {
             double s0, s1, t0, t1;
@@ -179,9 +183,9 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             const int total = __shfl(incl, 63);
             if (total == 0) return;
             const int excl = incl - nrow;
-            w_start[lane] = excl;
+            w_start[lane] = (uint16_t)excl;
             for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
-            __syncthreads();
+            wave_sync();
             for (int t = lane; t < total; t += 64) {
                 const int j = w_own[t];
                 const int cc = w_cc[j];
@@ -200,6 +204,9 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     xscale = w_xs[j];
                     if (cat.n > 0) A0 = w_A[j];
                     if (cat.n > 1) A1 = w_A[64 + j];
+                    // take the LDS values here: waited for at their first use inside the pair
+                    // loop, the wait would also cover the accumulator atomics of every pair
+                    asm volatile("" ::"v"(wih), "v"(xscale), "v"(A0), "v"(A1));
                 }
                 // lanes on the same row start at different columns (fewer LDS bank conflicts)
                 const int len = b - a;
@@ -259,7 +266,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
         };
         // Runs of leaves (contiguous in Morton order): the particles whose footprint box can
         // reach the tile are queued, and every 64 queued particles make one sweep, so that the
@@ -276,11 +283,11 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             return !(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1);
         };
         auto flush = [&](int cnt) {
-            __syncthreads();
+            wave_sync();
             const int q = L.pend[lane];
             const int rest = npend - cnt;
             const int moved = lane < rest ? L.pend[cnt + lane] : 0;
-            __syncthreads();
+            wave_sync();
             if (lane < rest) L.pend[lane] = moved;
             npend = rest;
             sweep((int64_t)q, lane < cnt);
@@ -312,7 +319,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
         };
         walk(T, true, need, bucket, take);
         if (npend > 0) flush(npend);
-        __syncthreads();
+        wave_sync();
         for (int k = lane; k < m; k += 64) {
             const int64_t li = perm[C0 + k];
             if (MODE == 0) {

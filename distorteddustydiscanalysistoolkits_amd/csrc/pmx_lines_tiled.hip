@@ -99,45 +99,71 @@ __global__ void __launch_bounds__(256) k_line_cells(const uint64_t* __restrict__
     }
 }
 
+// Workgroups of kTileWaves wavefronts share one LDS copy of the Inorm table: its reads are
+// per-lane gathers over 16 KB, which the vector L1 serves one cache line at a time while LDS
+// serves them in a few cycles.  Each wavefront takes tiles from a device counter, so the
+// wavefronts of a workgroup stay busy until the map is done.
+constexpr int kTileWaves = 8;
+
 template <int K, int MODE>
-__global__ void __launch_bounds__(64) k_lines_tiled(T32 T, Frame f, TileCat cat, const int32_t* __restrict__ cstart,
-                                                    const int32_t* __restrict__ perm,
-                                                    const double2* __restrict__ st_sorted,
-                                                    const double* __restrict__ o, int64_t nl,
-                                                    const float* __restrict__ lut, double inv_dq,
-                                                    float* __restrict__ out, int64_t ldo,
-                                                    uint64_t* __restrict__ digest) {
-    extern __shared__ double2 dyn2[];             // 16-byte aligned: the line coordinates are double2
-    double* dyn = reinterpret_cast<double*>(dyn2);
-    __shared__ LineTileLDS L;
+__global__ void __launch_bounds__(64 * kTileWaves) k_lines_tiled(T32 T, Frame f, TileCat cat,
+                                                                 const int32_t* __restrict__ cstart,
+                                                                 const int32_t* __restrict__ perm,
+                                                                 const double2* __restrict__ st_sorted,
+                                                                 const double* __restrict__ o, int64_t nl,
+                                                                 const float* __restrict__ lut, double inv_dq,
+                                                                 float* __restrict__ out, int64_t ldo,
+                                                                 uint64_t* __restrict__ digest,
+                                                                 unsigned* __restrict__ next_tile) {
+    extern __shared__ double2 dyn2[];             // per wavefront: accumulators, particle fields
+    __shared__ LineTileLDS Ls[kTileWaves];
+    __shared__ float lut_s[kLutN];
+    for (int i = threadIdx.x; i < kLutN; i += 64 * kTileWaves) lut_s[i] = lut[i];
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    double* dyn = reinterpret_cast<double*>(dyn2) + (size_t)w * (lines_tile_lds(cat.n, MODE) / sizeof(double));
     const int64_t ntiles = (int64_t)f.tx * f.ty;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        lines_tile<K, MODE>(T, f, cat, cstart, perm, st_sorted, o, nl, lut, inv_dq, out, ldo, digest, tile, L, dyn);
+    for (;;) {
+        unsigned t = 0;
+        if ((threadIdx.x & 63) == 0) t = atomicAdd(next_tile, 1u);
+        const int64_t tile = (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0));
+        if (tile >= ntiles) break;
+        lines_tile<K, MODE>(T, f, cat, cstart, perm, st_sorted, o, nl, lut_s, inv_dq, out, ldo, digest, tile,
+                            Ls[w], dyn);
+    }
 }
 
 template <int K, int MODE>
-void launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
-                  const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest) {
+int launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
+                 const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest,
+                 unsigned* next_tile) {
     const int64_t ntiles = (int64_t)f.tx * f.ty;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20));
-    const size_t lds = lines_tile_lds(cat.n, MODE);
+    const size_t lds = lines_tile_lds(cat.n, MODE) * kTileWaves;
+    int per_cu = 0;
+    PMX_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lines_tiled<K, MODE>, 64 * kTileWaves, lds));
+    const int64_t resident = (int64_t)std::max(1, per_cu) * c->ncu;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(resident, (ntiles + kTileWaves - 1) / kTileWaves));
     T32 T = tree32(c, 64);
     T.bucket = g_tiled_bucket;
-    hipLaunchKernelGGL((k_lines_tiled<K, MODE>), dim3(grid), dim3(64), lds, c->stream, T, f, cat, cells,
+    PMX_HIP(c, hipMemsetAsync(next_tile, 0, sizeof(unsigned), c->stream));
+    hipLaunchKernelGGL((k_lines_tiled<K, MODE>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat, cells,
                        perm, sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out, ldo,
-                       digest);
+                       digest, next_tile);
+    PMX_HIP(c, hipGetLastError());
+    return PMX_OK;
 }
 
 template <int MODE>
-void launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
-                    const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest) {
+int launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
+                   const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest,
+                   unsigned* next_tile) {
     switch (c->tree.kernel) {
-    case kM4: launch_tiled<kM4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
-    case kM5: launch_tiled<kM5, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
-    case kM6: launch_tiled<kM6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
-    case kC2: launch_tiled<kC2, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
-    case kC4: launch_tiled<kC4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
-    default: launch_tiled<kC6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest); break;
+    case kM4: return launch_tiled<kM4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kM5: return launch_tiled<kM5, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kM6: return launch_tiled<kM6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kC2: return launch_tiled<kC2, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kC4: return launch_tiled<kC4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    default: return launch_tiled<kC6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
     }
 }
 
@@ -148,6 +174,7 @@ struct Binned {
     const int32_t* cells;
     const int32_t* perm;
     const double2* sts;
+    unsigned* next_tile;       // tile counter of the line-map kernel
 };
 
 int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok, Binned* out) {
@@ -271,6 +298,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     out->cells = cells;
     out->perm = perm;
     out->sts = sts;
+    out->next_tile = reinterpret_cast<unsigned*>(flag + 1);
     *ok = true;
     return PMX_OK;
 }
@@ -294,7 +322,8 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
         TileCat cat{};
         cat.n = std::min(kMaxCols, ncols - base);
         for (int j = 0; j < cat.n; ++j) { cat.fcol[j] = 1 + slots[base + j]; cat.shep[j] = shep[base + j]; }
-        launch_tiled_k<0>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, out + (int64_t)base * ldo, ldo, nullptr);
+        PMX_TRY(launch_tiled_k<0>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, out + (int64_t)base * ldo, ldo, nullptr,
+                                  B.next_tile));
         if (ncols == 0) break;
     }
     PMX_HIP(c, hipGetLastError());
@@ -321,7 +350,7 @@ int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, 
     PMX_TRY(bin_lines(c, o, d, nl, &ok, &B));
     if (!ok) return PMX_OK;
     TileCat cat{};
-    launch_tiled_k<1>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, nullptr, 0, digest);
+    PMX_TRY(launch_tiled_k<1>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, nullptr, 0, digest, B.next_tile));
     PMX_HIP(c, hipGetLastError());
     *used = true;
     return PMX_OK;

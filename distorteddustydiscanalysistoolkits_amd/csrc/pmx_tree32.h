@@ -23,6 +23,14 @@ struct T32 {
 };
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+// Barrier of one wavefront over its LDS traffic: a wavefront's LDS operations complete in
+// issue order, so ordering them in the compiler is enough (workgroups of several independent
+// wavefronts must not use __syncthreads inside per-wavefront work).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Union walk of the tile over left / escape links, from node `start` until the walk leaves
