@@ -23,6 +23,7 @@ struct Frame {
     int gx, gy, tx, ty;                // cells and tiles along s and t
     double band;                       // absolute band on d^2 around (K h)^2 -> exact re-check
     double pad;                        // absolute padding of footprint radii
+    double pad32;                      // pad + bound on the projection of a position's FP32 residual
     double sqrt_band;                  // sqrt(band): R + sqrt(band) >= sqrt(R^2 + band)
     double inv_cs, inv_ct;             // 1 / cell size (binning and footprints use the same products)
     double near;                       // cells: nodes whose box is this close are read whole
@@ -272,14 +273,13 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
         // reach the tile are queued, and every 64 queued particles make one sweep, so that the
         // sweeps stay full where most of a run misses the tile.
         int npend = 0;                 // wave-uniform
+        // (on the FP32 position alone: its residual moves the projection by less than pad32)
         auto reaches = [&](int64_t p) {
             const float4 H = T.leaf[p];
-            const float4 Lo = T.leaflo[p];
-            const double x = (double)H.x + (double)Lo.x, y = (double)H.y + (double)Lo.y,
-                         z = (double)H.z + (double)Lo.z;
+            const double x = (double)H.x, y = (double)H.y, z = (double)H.z;
             const double sb = (x * f.u[0] + y * f.u[1]) + z * f.u[2];
             const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
-            const double Rpu = Kv * (double)H.w * (1.0 + 1e-6) + f.sqrt_band + f.pad;
+            const double Rpu = Kv * (double)H.w * (1.0 + 1e-6) + f.sqrt_band + f.pad32;
             return !(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1);
         };
         auto flush = [&](int cnt) {

@@ -245,6 +245,9 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     // normalisation within 1e-12); band 1e-11 E^2 leaves a wide margin
     f.band = 1e-11 * E * E;
     f.pad = 1e-12 * E;
+    // |x - hi| <= 2^-24 |x| <= 2^-25 E per coordinate, so the hi-only projection on a unit axis
+    // is within sqrt(3) 2^-25 E = 5.2e-8 E of the exact one
+    f.pad32 = f.pad + 2e-7 * E;
     f.sqrt_band = std::sqrt(f.band);
     // cells: about one line per cell, lines at cell centres for a regular lattice
     const double es = smax - smin, et = tmax - tmin;
