@@ -8,8 +8,9 @@
 // path works particle-centric:
 //   * lines are binned into a grid of cells in the (s, t) plane; 8 x 8 cells form a tile, and
 //     lines are sorted by (tile, cell) so a tile's lines are one contiguous run;
-//   * one wavefront per tile (8 per workgroup, sharing an LDS copy of the Inorm table) walks the LBVH once (wave-uniform node test: the node box
-//     projected on the plane against the tile rectangle, radius K hmax), and each bucket of
+//   * one wavefront per tile (8 per workgroup, sharing an LDS copy of the Inorm table) walks
+//     the LBVH once (wave-uniform node test: the node box projected on the plane against the
+//     tile rectangle, radius K hmax), and each bucket of
 //     up to 64 particles (contiguous in Morton order) is taken one particle per lane;
 //   * each lane visits only the cells its disc covers, tests those lines in FP64 on the
 //     projected coordinates with a guard band, re-checks band cases with the reference's
