@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libpartia_mi355x.so")
 
 PMX_OK, PMX_ERR_ARGUMENT, PMX_ERR_DIMENSION, PMX_ERR_STATE, PMX_ERR_HIP, PMX_ERR_NOMEM, PMX_ERR_CAPACITY = (
     0, -1, -2, -3, -4, -5, -6)
+PMX_ERR_FORMAT, PMX_ERR_IO = -7, -8
+PMX_I8, PMX_I16, PMX_I32, PMX_I64 = 10, 11, 12, 13
 PMX_F32, PMX_F64 = 0, 1
 PMX_MEM_HOST, PMX_MEM_DEVICE = 0, 1
 PMX_GATHER, PMX_SCATTER, PMX_SYMMETRIC = 0, 1, 2
@@ -29,7 +31,11 @@ EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_bu
            "pmx_kernel_eval", "pmx_line_kernel_table", "pmx_last_timings", "pmx_point_traversal_stats",
            "pmx_line_traversal_stats", "pmx_line_neighbor_digest", "pmx_batch_create", "pmx_batch_run",
            "pmx_batch_interpolate", "pmx_batch_tune", "pmx_batch_info", "pmx_batch_last_error",
-           "pmx_batch_destroy")
+           "pmx_batch_destroy",
+           # include/partia_phantom.h
+           "pmx_phantom_open", "pmx_phantom_open_error", "pmx_phantom_close", "pmx_phantom_last_error",
+           "pmx_phantom_info", "pmx_phantom_file_identifier", "pmx_phantom_block_size", "pmx_phantom_column_info",
+           "pmx_phantom_header_var", "pmx_phantom_read_column", "pmx_phantom_density")
 PMX_JOB_POINTS, PMX_JOB_LINES = 0, 1
 
 
@@ -48,11 +54,23 @@ class DimensionMismatch(PmxError, ValueError):
     """≙ Julia DimensionMismatch."""
 
 
+class FormatError(PmxError):
+    """≙ the ErrorException read_phantom raises for a file that is not a Phantom dump."""
+
+
+class PmxIOError(PmxError, OSError):
+    """≙ Julia SystemError (the file cannot be opened or read)."""
+
+
 def _raise(code, msg):
     if code == PMX_ERR_ARGUMENT:
         raise ArgumentError(code, msg)
     if code == PMX_ERR_DIMENSION:
         raise DimensionMismatch(code, msg)
+    if code == PMX_ERR_FORMAT:
+        raise FormatError(code, msg)
+    if code == PMX_ERR_IO:
+        raise PmxIOError(code, msg)
     raise PmxError(code, msg)
 
 
@@ -138,6 +156,21 @@ def lib():
         L.pmx_batch_info.argtypes = [vp, vp, i32]
         L.pmx_batch_last_error.argtypes = [vp]
         L.pmx_batch_last_error.restype = C.c_char_p
+        L.pmx_phantom_open.argtypes = [C.POINTER(vp), C.c_char_p]
+        L.pmx_phantom_open_error.argtypes = []
+        L.pmx_phantom_open_error.restype = C.c_char_p
+        L.pmx_phantom_close.argtypes = [vp]
+        L.pmx_phantom_close.restype = None
+        L.pmx_phantom_last_error.argtypes = [vp]
+        L.pmx_phantom_last_error.restype = C.c_char_p
+        L.pmx_phantom_info.argtypes = [vp, vp, i32]
+        L.pmx_phantom_file_identifier.argtypes = [vp, C.c_char_p, i32]
+        L.pmx_phantom_block_size.argtypes = [vp, i32, C.POINTER(i64)]
+        L.pmx_phantom_column_info.argtypes = [vp, i32, C.c_char_p, C.POINTER(i32), C.POINTER(i32)]
+        L.pmx_phantom_header_var.argtypes = [vp, i32, C.c_char_p, C.POINTER(i32), C.POINTER(C.c_double),
+                                             C.POINTER(i64)]
+        L.pmx_phantom_read_column.argtypes = [vp, i32, i32, i32, vp, vp, i64]
+        L.pmx_phantom_density.argtypes = [i32, i32, i32, i64, vp, C.c_double, C.c_double, vp]
         L.pmx_batch_destroy.argtypes = [vp]
         L.pmx_batch_destroy.restype = None
         _lib = L
@@ -448,3 +481,98 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- Phantom dump reader (include/partia_phantom.h) ----
+_NP_OF = {PMX_I8: np.int8, PMX_I16: np.int16, PMX_I32: np.int32, PMX_I64: np.int64, PMX_F32: np.float32,
+          PMX_F64: np.float64}
+_CODE_OF = {np.dtype(v): k for k, v in _NP_OF.items()}
+
+
+def dtype_code_np(dt) -> int:
+    """PMX element-type code of a numpy / torch dtype."""
+    s = str(dt).replace("torch.", "")
+    return _CODE_OF[np.dtype(s)]
+
+
+class Phantom:
+    """An open Phantom dump: its structure (parsed at open) and column reads into host arrays or
+    device tensors."""
+
+    def __init__(self, path: str):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.pmx_phantom_open(C.byref(h), os.fsencode(path))
+        if rc != PMX_OK:
+            _raise(rc, L.pmx_phantom_open_error().decode())
+        self._h = h
+        v = np.zeros(5, np.int64)
+        self._check(L.pmx_phantom_info(h, v.ctypes.data, 5))
+        self.int_bytes, self.real_bytes, nblocks, ncols, nvars = (int(x) for x in v)
+        buf = C.create_string_buffer(128)
+        self._check(L.pmx_phantom_file_identifier(h, buf, 128))
+        self.file_identifier = buf.value.decode(errors="replace")
+        self.block_sizes = []
+        for b in range(nblocks):
+            n = C.c_int64()
+            self._check(L.pmx_phantom_block_size(h, b, C.byref(n)))
+            self.block_sizes.append(int(n.value))
+        self.columns = []            # (name, block, dtype code) in file order
+        for i in range(ncols):
+            nm, blk, dt = C.create_string_buffer(17), C.c_int32(), C.c_int32()
+            self._check(L.pmx_phantom_column_info(h, i, nm, C.byref(blk), C.byref(dt)))
+            self.columns.append((nm.value.decode(errors="replace"), int(blk.value), int(dt.value)))
+        self.header = {}             # name -> numpy scalar of the stored type
+        for i in range(nvars):
+            nm, dt, fv, iv = C.create_string_buffer(24), C.c_int32(), C.c_double(), C.c_int64()
+            self._check(L.pmx_phantom_header_var(h, i, nm, C.byref(dt), C.byref(fv), C.byref(iv)))
+            t = _NP_OF[int(dt.value)]
+            val = t(fv.value) if int(dt.value) in (PMX_F32, PMX_F64) else t(iv.value)
+            self.header[nm.value.decode(errors="replace")] = val
+
+    def _check(self, rc):
+        if rc != PMX_OK:
+            _raise(rc, lib().pmx_phantom_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmx_phantom_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def read_column(self, col: int, out=None, dtype=None, keep=None):
+        """Column `col` into `out` (a numpy array, or a device tensor for a straight-to-HBM read),
+        or into a new host array of `dtype` (default: the stored type).  keep: increasing 0-based
+        rows to keep (int64), or None for all rows."""
+        name, blk, code = self.columns[col]
+        n = self.block_sizes[blk] if keep is None else len(keep)
+        if out is None:
+            out = np.empty(n, _NP_OF[code] if dtype is None else dtype)
+        if out.shape[0] != n:
+            raise DimensionMismatch(PMX_ERR_DIMENSION, f"column {name}: {n} rows, destination {out.shape[0]}")
+        mem = _mem_of([out])
+        kp = None if keep is None else np.ascontiguousarray(keep, np.int64)
+        self._check(lib().pmx_phantom_read_column(self._h, int(col), dtype_code_np(out.dtype), mem, _ptr(out),
+                                                  None if kp is None else kp.ctypes.data,
+                                                  0 if kp is None else int(kp.size)))
+        return out
+
+
+def density_from_h(h, m, hfact, dim=3, out=None):
+    """add_rho!'s _density on one column (host array or device tensor; m, hfact in h's type)."""
+    if out is None:
+        out = h.new_empty(h.shape) if not isinstance(h, np.ndarray) else np.empty_like(h)
+    mem = _mem_of([h, out])
+    rc = lib().pmx_phantom_density(_dtype_code(h), int(dim), mem, int(h.shape[0]), _ptr(h), float(m), float(hfact),
+                                   _ptr(out))
+    if rc != PMX_OK:
+        _raise(rc, "pmx_phantom_density")
+    return out

@@ -37,7 +37,9 @@ typedef enum pmx_status {
     PMX_ERR_STATE = -3,      /* no LBVH built in this context yet                            */
     PMX_ERR_HIP = -4,        /* HIP runtime failure (≙ ErrorException)                       */
     PMX_ERR_NOMEM = -5,      /* device allocation failed                                     */
-    PMX_ERR_CAPACITY = -6    /* an output buffer supplied by the caller is too small         */
+    PMX_ERR_CAPACITY = -6,   /* an output buffer supplied by the caller is too small         */
+    PMX_ERR_FORMAT = -7,     /* not a readable Phantom dump (≙ ErrorException of read_phantom) */
+    PMX_ERR_IO = -8          /* the file cannot be opened or read (≙ SystemError)            */
 } pmx_status;
 
 /* Element type of particle / sample columns (InterpolationInput{3,T}: T ∈ {Float32, Float64}). */
