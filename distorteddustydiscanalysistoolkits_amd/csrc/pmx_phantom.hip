@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -101,7 +102,22 @@ int fail(pmx_phantom* ph, int code, const std::string& msg) {
     return code;
 }
 
+bool pread_all(int fd, char* d, int64_t n, int64_t at) {
+    int64_t done = 0;
+    while (done < n) {
+        const ssize_t r = ::pread(fd, d + done, (size_t)(n - done), (off_t)(at + done));
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) return false;
+        done += r;
+    }
+    return true;
+}
+
+constexpr int64_t kSplitBytes = 4 << 20;   // reads of at least twice this are split over threads
+
 // Positioned reads over the file with a cursor (the reference reads sequentially from a stream).
+// A large read is split over up to 8 threads: one thread copying from the page cache reaches
+// only a fraction of the host's memory bandwidth.
 struct Cursor {
     int fd;
     int64_t pos;
@@ -109,12 +125,21 @@ struct Cursor {
     bool read(void* dst, int64_t n) {
         if (n < 0 || pos < 0 || pos + n > size) return false;
         char* d = static_cast<char*>(dst);
-        int64_t done = 0;
-        while (done < n) {
-            const ssize_t r = ::pread(fd, d + done, (size_t)(n - done), (off_t)(pos + done));
-            if (r < 0 && errno == EINTR) continue;
-            if (r <= 0) return false;
-            done += r;
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        const int parts = (int)std::min<int64_t>(std::min(8, hw), n / kSplitBytes);
+        if (parts < 2) {
+            if (!pread_all(fd, d, n, pos)) return false;
+        } else {
+            std::vector<std::thread> th;
+            std::vector<char> ok(parts, 0);
+            const int64_t step = (n + parts - 1) / parts;
+            for (int t = 0; t < parts; ++t) {
+                const int64_t a = t * step, b = std::min(n, a + step);
+                th.emplace_back([&, t, a, b] { ok[t] = pread_all(fd, d + a, b - a, pos + a); });
+            }
+            for (auto& x : th) x.join();
+            for (char v : ok)
+                if (!v) return false;
         }
         pos += n;
         return true;
@@ -502,16 +527,21 @@ int pmx_phantom_read_column(pmx_phantom* ph, int32_t col, int32_t out_dtype, int
         if (keep && k1 == k) continue;
         // with a sparse keep list, read only the span of this chunk's kept rows
         const int64_t s0 = keep ? keep[k] : r0, s1 = keep ? keep[k1 - 1] + 1 : r1;
-        void* raw = device ? ph->raw : host_raw.data();
+        // every row in the stored type: the file bytes go straight to their destination
+        const bool direct = !keep && c.dtype == out_dtype;
         cur.pos = c.offset + s0 * ib;
-        if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
         if (!device) {
-            written += convert_chunk(c.dtype, out_dtype, raw, s1 - s0, static_cast<char*>(dst) + written * ob, keep, k,
-                                     k1, s0);
+            void* raw = direct ? static_cast<char*>(dst) + written * ob : host_raw.data();
+            if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
+            written += direct ? s1 - s0
+                              : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, static_cast<char*>(dst) + written * ob,
+                                              keep, k, k1, s0);
         } else {
             const int b = (int)(chunk & 1);
             PH_HIP(ph, hipEventSynchronize(ph->done[b]));      // the copy that last used out[b] is done
-            const int64_t m = convert_chunk(c.dtype, out_dtype, raw, s1 - s0, ph->out[b], keep, k, k1, s0);
+            void* raw = direct ? ph->out[b] : ph->raw;
+            if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
+            const int64_t m = direct ? s1 - s0 : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, ph->out[b], keep, k, k1, s0);
             PH_HIP(ph, hipMemcpyAsync(static_cast<char*>(dst) + written * ob, ph->out[b], (size_t)(m * ob),
                                       hipMemcpyHostToDevice, ph->stream));
             PH_HIP(ph, hipEventRecord(ph->done[b], ph->stream));

@@ -80,19 +80,30 @@ def _torch_device(device):
     return torch.device(device)
 
 
-def _read(ph, cols, keep, device):
-    """Columns (index, name) of the open dump, keeping rows `keep`, into host arrays or tensors."""
+def _read(ph, cols, keep, device, have=None):
+    """Columns (index, name) of the open dump, keeping rows `keep`, into host arrays or tensors.
+    have: {column index: host array of the whole column} already read (uploaded, not re-read)."""
     out = {}
     for i, name in cols:
         _, blk, code = ph.columns[i]
         n = ph.block_sizes[blk] if keep is None else len(keep)
-        if device is None:
+        if have and i in have:
+            a = have[i] if keep is None else have[i][keep]
+            if isinstance(a, np.ndarray):
+                a = np.array(a, copy=True) if device is None else _to_device(a, device)
+            out[name] = a
+        elif device is None:
             out[name] = ph.read_column(i, keep=keep)
         else:
             import torch
             t = torch.empty(n, dtype=getattr(torch, np.dtype(_lib._NP_OF[code]).name), device=device)
             out[name] = ph.read_column(i, out=t, keep=keep)
     return out
+
+
+def _to_device(a, device):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
 
 
 def read_phantom(filename: str, separate_types: str = "sinks", ignore_inactive: bool = True, device=None):
@@ -119,15 +130,35 @@ def read_phantom(filename: str, separate_types: str = "sinks", ignore_inactive: 
         sinks_nonempty = bool(sinks) and n_sink > 0
 
         keep = None
+        have = {}
+        ih = main_idx.get("h")
         if ignore_inactive:
-            if "h" not in main_idx:
+            if ih is None:
                 raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "column name :h not found in the data frame")
-            h = ph.read_column(main_idx["h"])
-            keep = np.flatnonzero(h > 0).astype(np.int64)
+            if dev is not None:
+                # h straight into device memory, tested there; the host sees h only when some
+                # particles are inactive and the kept rows must be listed
+                import torch
+                hd = ph.read_column(ih, out=torch.empty(n_main, dtype=getattr(
+                    torch, np.dtype(_lib._NP_OF[ph.columns[ih][2]]).name), device=dev))
+                if bool((hd > 0).all()):
+                    have[ih] = hd
+                else:
+                    h = hd.cpu().numpy()
+                    have[ih] = h
+                    keep = np.flatnonzero(h > 0).astype(np.int64)
+            else:
+                h = ph.read_column(ih)
+                have[ih] = h
+                active = h > 0
+                # every particle active (the usual case): whole columns, no row list to build
+                keep = None if active.all() else np.flatnonzero(active).astype(np.int64)
 
         if separate_types == "all" and "itype" in main_idx:
             it = ph.read_column(main_idx["itype"])
             rows = np.arange(n_main, dtype=np.int64) if keep is None else keep
+            if ih in have and not isinstance(have[ih], np.ndarray):
+                have[ih] = have[ih].cpu().numpy()       # grouped rows are selected on the host
             types = np.unique(it[rows])
             if types.size > 1:
                 frames = []
@@ -139,7 +170,7 @@ def read_phantom(filename: str, separate_types: str = "sinks", ignore_inactive: 
                     params = dict(header)
                     params["mass"] = header[mass_key]
                     params["itype"] = itype
-                    frames.append(ParticleDataFrame(_read(ph, main, rows[it[rows] == v], dev), params))
+                    frames.append(ParticleDataFrame(_read(ph, main, rows[it[rows] == v], dev, have), params))
                 if sinks_nonempty:
                     frames.append(ParticleDataFrame(_read(ph, sinks, None, dev), header))
                 return frames
@@ -148,10 +179,10 @@ def read_phantom(filename: str, separate_types: str = "sinks", ignore_inactive: 
             params = dict(header)
             params["mass"] = None
             params["itype"] = math.nan
-            return [ParticleDataFrame(_read(ph, main, keep, dev), params),
+            return [ParticleDataFrame(_read(ph, main, keep, dev, have), params),
                     ParticleDataFrame(_read(ph, sinks, None, dev), header)]
 
-        cols = _read(ph, main, keep, dev)
+        cols = _read(ph, main, keep, dev, have)
         if sinks_nonempty:
             cols = _vcat_union(cols, _read(ph, sinks, None, dev))
         params = dict(header)

@@ -1,4 +1,4 @@
-"""Writes Phantom native binary dumps for the reader's tests (test infrastructure).
+"""Writes Phantom native binary dumps: synthetic benchmark inputs and the reader's test files.
 
 The layout is the one ParticleIO.jl's reader consumes (read_phantom.jl:20-207): every record is
 framed by a 4-byte tag before and after its payload (the payload length, as Fortran writes it);

@@ -1,7 +1,7 @@
 """The Phantom dump reader (pmx_phantom_*) and the ParticleIO mirror on host memory (no GPU).
 
 The reference's own test dump (ParticleIO.jl/test/testinput/testdumpfile_00000) is not in the
-repository, so the dumps are written by tests/phantom_writer.py from the record layout the
+repository, so the dumps are written by phantomdump.py from the record layout the
 reference reader consumes, with known contents; the checks follow the reference's io_tests.jl
 (columns x, y, h present, npart equal to the header's, dimension, finite time, header keys) plus
 the reader's format rules (type detection, duplicate-tag renaming, block 2 = sinks, inactive
@@ -14,9 +14,9 @@ import pytest
 
 from distorteddustydiscanalysistoolkits_amd import _lib, disc
 from distorteddustydiscanalysistoolkits_amd import particleio as PIO
+from distorteddustydiscanalysistoolkits_amd import phantomdump as W
 from distorteddustydiscanalysistoolkits_amd import partia as P
 
-from . import phantom_writer as W
 
 
 def _gas(n, seed=0, inactive=()):
