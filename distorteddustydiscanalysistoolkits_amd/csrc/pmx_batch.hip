@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
                 pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t,
                                                        P, Y, sp);
         } else {
-            lmap::lines_tile<K, 0>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut, U.inv_dq, U.out, U.ldo,
+            lmap::lines_tile<K, 0, true>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut, U.inv_dq, U.out, U.ldo,
                                    nullptr, t, L, ldyn);
         }
     }
@@ -262,7 +262,7 @@ int make_units(pmx_batch* b) {
                 U.inv_dq = 1.0 / c->lut_dq[b->kernel];
                 U.out = outbuf + (int64_t)base * nP;
                 U.ntiles = (int64_t)U.f.tx * U.f.ty;
-                lds = std::max(lds, sizeof(lmap::LineTileLDS) + lmap::lines_tile_lds(cnt, 0));
+                lds = std::max(lds, sizeof(lmap::LineTileLDS) + lmap::lines_tile_lds(cnt, 0, true));
             }
             list.push_back({weight, U});
         }

@@ -66,16 +66,22 @@ struct alignas(16) LineTileLDS {
     unsigned char w_own[64 * kTS];
 };
 
-__host__ __device__ inline size_t lines_tile_lds(int ncol, int mode) {
-    return sizeof(double) * (size_t)kCap * (mode == 0 ? 1 + ncol : 3) + sizeof(float) * 64 * (size_t)(mode == 0 ? ncol : 0);
+// Dynamic LDS of one tile: accumulators (and, for one-wavefront workgroups, the chunk's line
+// coordinates) and the particle fields of a sweep.
+__host__ __device__ inline size_t lines_tile_lds(int ncol, int mode, bool wg1 = false) {
+    return sizeof(double) * (size_t)kCap * ((wg1 ? 2 : 0) + (mode == 0 ? 1 + ncol : 3)) +
+           sizeof(float) * 64 * (size_t)(mode == 0 ? ncol : 0);
 }
 
-// One 8 x 8-cell tile of a line map (one wavefront; other wavefronts of the workgroup work on
-// other tiles, so only wavefront barriers are used).  `lut` is the kernel's Inorm table (in LDS
-// or global memory).  MODE 0: interpolation (FP64 weight sums).
-// MODE 1: neighbour digest per line (count, sum of sorted particle indices, sum of their
-// squares; unsigned 64-bit, wrapping).
-template <int K, int MODE>
+// One 8 x 8-cell tile of a line map (one wavefront).  `lut` is the kernel's Inorm table (in LDS
+// or global memory).  MODE 0: interpolation (FP64 weight sums).  MODE 1: neighbour digest per
+// line (count, sum of sorted particle indices, sum of their squares; unsigned 64-bit, wrapping).
+// WG1: the workgroup is this one wavefront (the batch kernel, whose table stays in global
+// memory): the chunk's line coordinates are staged in LDS, so the table has the vector L1 to
+// itself, and barriers are workgroup barriers.  Otherwise other wavefronts of the workgroup work
+// on other tiles: wavefront barriers only, and each line is read from global memory one pair
+// ahead (a global read waits on the vector-memory counter, not behind the LDS atomics).
+template <int K, int MODE, bool WG1 = false>
 __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& cat, const int32_t* __restrict__ cstart,
                                   const int32_t* __restrict__ perm, const double2* __restrict__ st_sorted,
                                   const double* __restrict__ o, int64_t nl, const float* __restrict__ lut,
@@ -86,19 +92,24 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     int32_t *w_p = L.w_p, *w_cc = L.w_cc;
     uint16_t* w_start = L.w_start;
     unsigned char* w_own = L.w_own;
-    const int lane = threadIdx.x & 63;
+    const int lane = WG1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    auto sync = [] {
+        if constexpr (WG1) __syncthreads();
+        else wave_sync();
+    };
     const int nacc = MODE == 0 ? 1 + cat.n : 3;
-    double* acc = dyn;
-    unsigned long long* dg = reinterpret_cast<unsigned long long*>(dyn);
-    float* w_A = reinterpret_cast<float*>(dyn + nacc * kCap);     // [ncol][64]
+    double2* ST = reinterpret_cast<double2*>(dyn);          // WG1: (s, t) of the chunk's lines
+    double* acc = dyn + (WG1 ? 2 * kCap : 0);
+    unsigned long long* dg = reinterpret_cast<unsigned long long*>(acc);
+    float* w_A = reinterpret_cast<float*>(acc + nacc * kCap);     // [ncol][64]
     const double Kv = support(K);
     const double xmax = Kv * inv_dq;       // support in table steps
     const int64_t n = T.n;
 
-    wave_sync();
+    sync();
     cs[lane] = cstart[tile * kCells + lane];
     if (lane == 0) cs[kCells] = cstart[(tile + 1) * kCells];
-    wave_sync();
+    sync();
     const int L0 = cs[0], L1 = cs[kCells];
     if (L0 >= L1) return;
     const int txi = (int)(tile % f.tx), tyi = (int)(tile / f.tx);
@@ -113,12 +124,13 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
     for (int C0 = L0; C0 < L1; C0 += kCap) {
         const int m = min(kCap, L1 - C0);
         for (int k = lane; k < kCap; k += 64) {
+            if (WG1 && k < m) ST[k] = st_sorted[C0 + k];
             for (int a = 0; a < nacc; ++a) {
                 if (MODE == 0) acc[a * kCap + k] = 0.0;
                 else dg[a * kCap + k] = 0ull;
             }
         }
-        wave_sync();
+        sync();
 
         auto need = [&](const Node32& nd) {
             double s0, s1, t0, t1;
@@ -186,7 +198,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             const int excl = incl - nrow;
             w_start[lane] = (uint16_t)excl;
             for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
-            wave_sync();
+            sync();
             for (int t = lane; t < total; t += 64) {
                 const int j = w_own[t];
                 const int cc = w_cc[j];
@@ -207,18 +219,18 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     if (cat.n > 1) A1 = w_A[64 + j];
                     // take the LDS values here: waited for at their first use inside the pair
                     // loop, the wait would also cover the accumulator atomics of every pair
-                    asm volatile("" ::"v"(wih), "v"(xscale), "v"(A0), "v"(A1));
+                    if constexpr (!WG1) asm volatile("" ::"v"(wih), "v"(xscale), "v"(A0), "v"(A1));
                 }
                 // lanes on the same row start at different columns (fewer LDS bank conflicts)
                 const int len = b - a;
                 int k = a + (lane % len);
-                // the next line's (s, t) is read one pair ahead
-                double2 Lnext = st_sorted[k];
+                // the next line's (s, t) is read one pair ahead (WG1: from LDS, in the pair)
+                double2 Lnext = WG1 ? make_double2(0.0, 0.0) : st_sorted[k];
                 for (int it = 0; it < len; ++it) {
                     const int kk = k - C0;
-                    const double2 L = Lnext;
+                    const double2 L = WG1 ? ST[kk] : Lnext;
                     k = (k + 1 == b) ? a : k + 1;
-                    Lnext = st_sorted[k];
+                    if constexpr (!WG1) Lnext = st_sorted[k];
                     const double ds = sb - L.x, dt = tb - L.y;
                     const double d2 = ds * ds + dt * dt;
                     if (d2 > r2hi) continue;
@@ -267,7 +279,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                     }
                 }
             }
-            wave_sync();
+            sync();
         };
         // Runs of leaves (contiguous in Morton order): the particles whose footprint box can
         // reach the tile are queued, and every 64 queued particles make one sweep, so that the
@@ -283,11 +295,11 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
             return !(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1);
         };
         auto flush = [&](int cnt) {
-            wave_sync();
+            sync();
             const int q = L.pend[lane];
             const int rest = npend - cnt;
             const int moved = lane < rest ? L.pend[cnt + lane] : 0;
-            wave_sync();
+            sync();
             if (lane < rest) L.pend[lane] = moved;
             npend = rest;
             sweep((int64_t)q, lane < cnt);
@@ -319,7 +331,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
         };
         walk(T, true, need, bucket, take);
         if (npend > 0) flush(npend);
-        wave_sync();
+        sync();
         for (int k = lane; k < m; k += 64) {
             const int64_t li = perm[C0 + k];
             if (MODE == 0) {
