@@ -116,7 +116,7 @@ bool pread_all(int fd, char* d, int64_t n, int64_t at) {
 constexpr int64_t kSplitBytes = 4 << 20;   // reads of at least twice this are split over threads
 
 // Positioned reads over the file with a cursor (the reference reads sequentially from a stream).
-// A large read is split over up to 8 threads: one thread copying from the page cache reaches
+// A large read is split over up to 16 threads: one thread copying from the page cache reaches
 // only a fraction of the host's memory bandwidth.
 struct Cursor {
     int fd;
@@ -126,7 +126,7 @@ struct Cursor {
         if (n < 0 || pos < 0 || pos + n > size) return false;
         char* d = static_cast<char*>(dst);
         const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-        const int parts = (int)std::min<int64_t>(std::min(8, hw), n / kSplitBytes);
+        const int parts = (int)std::min<int64_t>(std::min(16, hw), n / kSplitBytes);
         if (parts < 2) {
             if (!pread_all(fd, d, n, pos)) return false;
         } else {
