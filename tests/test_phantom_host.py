@@ -233,3 +233,30 @@ def test_add_rho_and_build_input(tmp_path):
     inp64, _ = PIO.build_input(gas, PIO.MassFromColumn("m64"))
     assert str(inp64.dtype) == "float64" and inp64.coord[0].dtype == np.float64
     assert np.array_equal(inp64.coord[0], gas["x"].astype(np.float64))
+
+
+def test_corrupted_dumps_fail_cleanly(tmp_path):
+    """Random byte corruptions and truncations of a valid dump either parse (and then read every
+    column) or raise FormatError / DimensionMismatch: never a crash or a hang."""
+    g = _gas(64)
+    good = open(_dump(tmp_path, g), "rb").read()
+    rng = np.random.default_rng(99)
+    p = tmp_path / "bad"
+    outcomes = {"ok": 0, "error": 0}
+    for trial in range(300):
+        b = bytearray(good)
+        if trial % 3 == 0:
+            b = b[:int(rng.integers(0, len(b)))]
+        else:
+            for _ in range(int(rng.integers(1, 6))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        p.write_bytes(bytes(b))
+        try:
+            with _lib.Phantom(str(p)) as ph:
+                for i, (_, blk, code) in enumerate(ph.columns):
+                    if 0 <= ph.block_sizes[blk] <= 1 << 20:
+                        ph.read_column(i)
+            outcomes["ok"] += 1
+        except (_lib.FormatError, _lib.DimensionMismatch, _lib.ArgumentError):
+            outcomes["error"] += 1
+    assert outcomes["error"] > 50 and sum(outcomes.values()) == 300
