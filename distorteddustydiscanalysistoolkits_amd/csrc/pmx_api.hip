@@ -502,6 +502,7 @@ int pmx_export_nodes(pmx_ctx* c, double* node_min, double* node_max, double* nod
     PMX_HIP(c, hipSetDevice(c->device));
     const int64_t n = c->tree.n;
     if (n < 2) return PMX_OK;
+    PMX_TRY(ensure_leaf64(c));                  // the FP64 nodes are refitted on first use
     std::vector<Node64> nodes((size_t)n);
     PMX_HIP(c, hipStreamSynchronize(c->stream));
     PMX_HIP(c, hipMemcpy(nodes.data(), c->nodes64.p, sizeof(Node64) * (size_t)n, hipMemcpyDeviceToHost));

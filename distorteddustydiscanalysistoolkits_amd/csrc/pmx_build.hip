@@ -603,9 +603,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, i
 // Round 0 starts from every leaf; later rounds from the list the previous round appended.
 // The FP32 copy of a node used by the FP32-compensated traversals: box rounded outward,
 // hmax rounded up (every FP32 node test stays conservative), plus its 0-based leaf run.
+// `nodes` is null when only the FP32 tree is built (the FP64 nodes are refitted on first use).
 __device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __restrict__ rlo,
                                   const int32_t* __restrict__ rhi, Node64* nodes, Node32* __restrict__ n32) {
-    nodes[i] = a;
+    if (nodes) nodes[i] = a;
     Node32 b;
     b.mnx = f32_down(a.mn[0]); b.mny = f32_down(a.mn[1]); b.mnz = f32_down(a.mn[2]);
     b.mxx = f32_up(a.mx[0]); b.mxy = f32_up(a.mx[1]); b.mxz = f32_up(a.mx[2]);
@@ -628,18 +629,30 @@ __device__ inline void combine_node(const double* __restrict__ cols64, const int
         const int64_t li = l - n;
         lmn[0] = lmx[0] = cols64[li]; lmn[1] = lmx[1] = cols64[n + li]; lmn[2] = lmx[2] = cols64[2 * n + li];
         hl = cols64[4 * n + li];
-    } else {
+    } else if (nodes) {
         const Node64 N = nodes[l];
         for (int k = 0; k < 3; ++k) { lmn[k] = N.mn[k]; lmx[k] = N.mx[k]; }
+        hl = N.hmax;
+    } else {
+        // FP32 tree only: the child's outward-rounded box.  Rounding is monotone, so the
+        // rounded min / max of the rounded children equal the rounding of the FP64 min / max
+        const Node32 N = n32[l];
+        lmn[0] = N.mnx; lmn[1] = N.mny; lmn[2] = N.mnz;
+        lmx[0] = N.mxx; lmx[1] = N.mxy; lmx[2] = N.mxz;
         hl = N.hmax;
     }
     if (r >= n) {
         const int64_t ri = r - n;
         rmn[0] = rmx[0] = cols64[ri]; rmn[1] = rmx[1] = cols64[n + ri]; rmn[2] = rmx[2] = cols64[2 * n + ri];
         hr = cols64[4 * n + ri];
-    } else {
+    } else if (nodes) {
         const Node64 N = nodes[r];
         for (int k = 0; k < 3; ++k) { rmn[k] = N.mn[k]; rmx[k] = N.mx[k]; }
+        hr = N.hmax;
+    } else {
+        const Node32 N = n32[r];
+        rmn[0] = N.mnx; rmn[1] = N.mny; rmn[2] = N.mnz;
+        rmx[0] = N.mxx; rmx[1] = N.mxy; rmx[2] = N.mxz;
         hr = N.hmax;
     }
     Node64 out;
@@ -823,6 +836,18 @@ __global__ void __launch_bounds__(kTailBlock) k_refit_tail(RefitTree R, int32_t*
     }
 }
 
+// Root (internal node 1) of the FP64 tree before its refit: the exact box is the particle
+// bounding box; hmax and the links are those of the FP32 root (hmax rounded up, replaced by
+// the FP64 refit before any FP64 traversal reads it).
+__global__ void k_root_box(const double* __restrict__ box, const Node32* __restrict__ n32, Node64* __restrict__ nodes) {
+    Node64 r;
+    for (int k = 0; k < 3; ++k) { r.mn[k] = box[k]; r.mx[k] = box[3 + k]; }
+    r.hmax = n32[1].hmax;
+    r.left = n32[1].left;
+    r.escape = n32[1].escape;
+    nodes[1] = r;
+}
+
 template <typename T>
 int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* y, const void* z,
                  int64_t n, double* part, int nparts, int* nan_count, double* box, uint64_t* codes,
@@ -865,7 +890,8 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
 // near the leaves), then the tail kernel for the rest; no host synchronisation.
 int g_refit_small = 32;
 
-int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
+// fp64: also write the FP64 nodes (otherwise only the FP32 tree of the compensated path).
+int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     hipStream_t s = c->stream;
     constexpr int kLaunchRounds = 10;
     PMX_TRY(ensure(c, c->scratch_c, 4 * (size_t)kMaxRounds + 64));
@@ -882,7 +908,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g) {
     R.rhi = c->range_hi.as<int32_t>();
     R.n = n;
     R.visited = c->visited.as<uint32_t>();
-    R.nodes = c->nodes64.as<Node64>();
+    R.nodes = fp64 ? c->nodes64.as<Node64>() : nullptr;
     R.n32 = c->nodes32.as<Node32>();
     const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
     hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.lr, R.escape, R.rlo, R.rhi, n,
@@ -1025,7 +1051,12 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
                            c->adj.as<int32_t>(), n, escape);
     }
     hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
-    if (n >= 2) PMX_TRY(refit_rounds(c, n, g));
+    if (n >= 2) {
+        // the FP32 tree only; the FP64 nodes follow on first use (ensure_leaf64), except the
+        // root's box, which is the particle bounding box of the Morton normalisation
+        PMX_TRY(refit_rounds(c, n, g, false));
+        hipLaunchKernelGGL(k_root_box, dim3(1), dim3(1), 0, s, box, c->nodes32.as<Node32>(), c->nodes64.as<Node64>());
+    }
     PMX_HIP(c, hipGetLastError());
     c->tree.n = n;
     c->tree.leaf64_ready = false;
@@ -1086,9 +1117,16 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
 
 namespace pmx {
 // Device address of the NaN-coordinate counter of the context's last build.
+// The plain FP64 path's structures, built on first use after a build: Leaf64 records and the
+// FP64 node boxes (a second bottom-up refit; the build refits the FP32 tree only).
 int ensure_leaf64(pmx_ctx* c) {
     if (c->tree.leaf64_ready) return PMX_OK;
     const int64_t n = c->tree.n;
+    if (n >= 2) {
+        const unsigned g = grid_for(n, kBlock, 16384);
+        hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, c->stream, c->visited.as<uint32_t>(), n + 2, 0u);
+        PMX_TRY(refit_rounds(c, n, g, true));
+    }
     PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)std::max<int64_t>(n, 1)));
     hipLaunchKernelGGL(k_pack_leaf, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, c->stream,
                        c->cols64.as<double>(), n, c->leaf64.as<Leaf64>());

@@ -43,7 +43,7 @@ struct Tree {
     int32_t kernel = 0;
     bool built = false;
     bool f32_ready = false;
-    bool leaf64_ready = false;    // Leaf64 records (plain FP64 path), packed on first use
+    bool leaf64_ready = false;    // plain FP64 path: Leaf64 records and FP64 node boxes, built on first use
 };
 
 }  // namespace pmx
