@@ -32,7 +32,7 @@
 
 namespace pmx {
 
-int g_bucket_points = 16;
+int g_bucket_points = 32;
 int g_bucket_lines = 16;
 
 namespace {
