@@ -22,6 +22,7 @@ PMX_MEM_HOST, PMX_MEM_DEVICE = 0, 1
 PMX_GATHER, PMX_SCATTER, PMX_SYMMETRIC = 0, 1, 2
 PMX_PREC_F64, PMX_PREC_F32C = 0, 1
 PMX_BUILD_WRITEBACK = 1
+PMX_LINE_INV_H2 = 0x100     # OR'd into a line call's precision: weight Inorm / h^2 (default Inorm / h)
 
 EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_build_lbvh",
            "pmx_interpolate_points", "pmx_interpolate_cartesian_grid", "pmx_interpolate_lines",

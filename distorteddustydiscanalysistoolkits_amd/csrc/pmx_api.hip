@@ -419,6 +419,8 @@ int pmx_interpolate_lines(pmx_ctx* c, int32_t strategy, int32_t precision, const
         return set_error(c, PMX_ERR_ARGUMENT,
                          "LineSamples_interpolation only supports itpScatter. Line-integrated samples do not have a "
                          "well-defined query smoothing length ha, so itpGather and itpSymmetric are not supported.");
+    c->line_inv_h2 = (precision & PMX_LINE_INV_H2) != 0;
+    precision &= ~PMX_LINE_INV_H2;
     if (precision != PMX_PREC_F64 && precision != PMX_PREC_F32C) return set_error(c, PMX_ERR_ARGUMENT, "unknown precision");
     FullCat fc;
     PMX_TRY(check_catalog(c, cat, true, fc));

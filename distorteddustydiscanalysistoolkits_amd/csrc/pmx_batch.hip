@@ -249,6 +249,7 @@ int make_units(pmx_batch* b) {
                 U.f = b->plan[j].f;
                 U.f.near = (double)g_tiled_near;
                 U.lcat.n = cnt;
+                U.lcat.h2 = 0;                  // batches weight by Inorm / h (the reference's)
                 for (int k = 0; k < cnt; ++k) {
                     U.lcat.fcol[k] = 1 + J.catalog.scalar_slots[base + k];
                     U.lcat.shep[k] = J.catalog.scalar_shepard ? (J.catalog.scalar_shepard[base + k] ? 1 : 0) : 1;

@@ -70,6 +70,7 @@ struct pmx_ctx {
     pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
     int ncu = 256;             // compute units of the device
+    bool line_inv_h2 = false;  // the current line call weights by Inorm / h^2 (PMX_LINE_INV_H2)
     int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
     pmx::DevBuf split;         // split-tile counters, item queue and slots (FP32c points)

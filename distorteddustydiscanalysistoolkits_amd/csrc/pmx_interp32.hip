@@ -303,6 +303,7 @@ struct LineCat32 {
     int n;
     int fcol[8];    // payload column of each output (1.. = quant)
     int shep[8];
+    int h2;         // weight Inorm / h^2 instead of Inorm / h
 };
 
 struct BLine {
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
                 const double invh = Y.invh64[j];
                 const double qq = sqrt(cx * cx + cy * cy + cz * cz) * invh;
                 const double I = qq >= Kv ? 0.0 : lin_lut_f64(qq, dq, lut);
-                const double w = Y.w64[j] * (invh * I);
+                const double w = Y.w64[j] * (cat.h2 ? (invh * I) * invh : invh * I);
                 S1 += w;
 #pragma unroll
                 for (int f = 0; f < 8; ++f)
@@ -598,6 +599,7 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
         LineCat32 cat{};
         cat.n = ncols - base < 8 ? ncols - base : 8;
         for (int j = 0; j < cat.n; ++j) { cat.fcol[j] = 1 + slots[base + j]; cat.shep[j] = shep[base + j]; }
+        cat.h2 = c->line_inv_h2;
         float* ob = out + (int64_t)base * ldo;
         switch (c->tree.kernel) {
         case kM4: launch_lines32<kM4>(c, cat, o, d, perm, nl, ob, ldo); break;

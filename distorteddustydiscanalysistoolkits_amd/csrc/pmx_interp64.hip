@@ -301,6 +301,7 @@ struct LineCat {
     int n;
     int slot[8];
     int shep[8];
+    int h2;         // weight Inorm / h^2 instead of Inorm / h
 };
 
 template <int K>
@@ -313,7 +314,8 @@ struct LineVisit {
     double acc[8];
     __device__ inline void operator()(int64_t li, double d2, double hb) {
         const double dr = sqrt(d2 > 0.0 ? d2 : 0.0);
-        const double W = lineW_f64(K, dr, hb, dq, lut);
+        double W = lineW_f64(K, dr, hb, dq, lut);
+        if (cat.h2) W = W * (1.0 / hb);
         const double w = P.m[li] * W / P.rho[li];
         S1 += w;
 #pragma unroll
@@ -523,6 +525,7 @@ int interp_lines_f64(pmx_ctx* c, int strategy, const int* slots, const int* shep
         LineCat cat{};
         cat.n = ncols - base < 8 ? ncols - base : 8;
         for (int j = 0; j < cat.n; ++j) { cat.slot[j] = slots[base + j]; cat.shep[j] = shep[base + j]; }
+        cat.h2 = c->line_inv_h2;
         double* ob = out + (int64_t)base * ldo;
         switch (c->tree.kernel) {
         case kM4: launch_lines_k<kM4>(c, cat, o, d, nl, ob, ldo); break;

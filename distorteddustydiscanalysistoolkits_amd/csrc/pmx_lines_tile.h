@@ -36,6 +36,7 @@ struct TileCat {
     int n;
     int fcol[kMaxCols];   // payload column of each output (1.. = quant)
     int shep[kMaxCols];
+    int h2;               // weight Inorm / h^2 instead of Inorm / h
 };
 
 // Extent of an outward-rounded FP32 box projected on a unit axis a.
@@ -180,7 +181,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                         if (MODE == 0) {
                             // (m/rho) / h and 1 / (h dq) of line_integrated_kernel_function
                             const double invh = T.ih64[p];
-                            w_wih[lane] = T.w64[p] * invh;
+                            w_wih[lane] = (T.w64[p] * invh) * (cat.h2 ? invh : 1.0);
                             w_xs[lane] = invh * inv_dq;
                             for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                         }

@@ -326,6 +326,7 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
         TileCat cat{};
         cat.n = std::min(kMaxCols, ncols - base);
         for (int j = 0; j < cat.n; ++j) { cat.fcol[j] = 1 + slots[base + j]; cat.shep[j] = shep[base + j]; }
+        cat.h2 = c->line_inv_h2;
         PMX_TRY(launch_tiled_k<0>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, out + (int64_t)base * ldo, ldo, nullptr,
                                   B.next_tile));
         if (ncols == 0) break;

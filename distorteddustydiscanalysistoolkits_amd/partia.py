@@ -344,7 +344,11 @@ def PointSamples_interpolation(backend, grid_template: PointSamples, input: Inte
 
 
 def LineSamples_interpolation(backend, grid_template: LineSamples, input: InterpolationInput,
-                              catalog: InterpolationCatalog, itp_strategy=itpScatter) -> GridBundle:
+                              catalog: InterpolationCatalog, itp_strategy=itpScatter, *,
+                              inv_h2: bool = False) -> GridBundle:
+    """line_integrated_scalar_interpolation (itpScatter).  inv_h2 (an extension, off by default):
+    weight each pair by Inorm(q) / h^2, the dimensionally correct column kernel, instead of the
+    reference's Inorm(q) / h (line_integrated_kernel.jl:132-137)."""
     if itp_strategy is not itpScatter:
         raise ArgumentError(_lib.PMX_ERR_ARGUMENT,
                             "LineSamples_interpolation only supports itpScatter. Line-integrated samples do not have a "
@@ -357,7 +361,8 @@ def LineSamples_interpolation(backend, grid_template: LineSamples, input: Interp
     outs = [np.empty(n, dt) for _ in range(catalog.L)]
     o = [_f64(c) for c in grid_template.origin]
     d = [_f64(c) for c in grid_template.direction]
-    ctx.interpolate_lines(itpScatter.code, backend.precision, catalog.struct(), o, d, outs)
+    prec = backend.precision | (_lib.PMX_LINE_INV_H2 if inv_h2 else 0)
+    ctx.interpolate_lines(itpScatter.code, prec, catalog.struct(), o, d, outs)
     grids = [LineSamples(v, grid_template.origin, grid_template.direction) for v in outs]
     return GridBundle(grids, catalog.ordered_names)
 

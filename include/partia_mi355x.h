@@ -57,6 +57,11 @@ enum { PMX_GATHER = 0, PMX_SCATTER = 1, PMX_SYMMETRIC = 2 };
  *                   sample tiles, wavefront-cooperative traversal and LDS-staged candidates;
  *                   neighbour decisions are exact FP64 decisions. */
 enum { PMX_PREC_F64 = 0, PMX_PREC_F32C = 1 };
+/* Flag OR'd into the precision argument of pmx_interpolate_lines / pmx_line_samples_interpolation:
+ * the line-integrated weight is Inorm(q) / h^2, the dimensionally correct column kernel, instead
+ * of the reference's Inorm(q) / h (line_integrated_kernel.jl:132-137).  Off by default (parity);
+ * batches (pmx_batch_*) always use 1 / h. */
+enum { PMX_LINE_INV_H2 = 0x100 };
 /* pmx_build_lbvh flags */
 enum { PMX_BUILD_WRITEBACK = 1 };   /* permute the caller's host columns into Morton order */
 
