@@ -665,11 +665,22 @@ def cpu_baseline(w, nsample):
     else:
         tree.point_samples(w.kname, "symmetric", w.cat_d, *[v[sel] for v in w.samples], nthreads=nthreads)
     ti = time.perf_counter() - t
+    # the same sample kernel on one thread, over every 8th sample of the subset
+    sel1 = sel[::8]
+    t = time.perf_counter()
+    if w.kind == "lines":
+        tree.line_samples(w.kname, "scatter", [0, 1], [0, 1], [v[sel1] for v in o], [v[sel1] for v in dv],
+                          nthreads=1)
+    else:
+        tree.point_samples(w.kname, "symmetric", w.cat_d, *[v[sel1] for v in w.samples], nthreads=1)
+    t1 = time.perf_counter() - t
     total = tb + ti * P / sel.size
     return {"value": P / total, "unit": "grid points/s", "cores": nthreads, "kind": "port",
             "sample": f"full LBVH build of {cols[0].size} particles ({tb:.2f} s) + {sel.size} of {P} samples "
                       f"({ti:.2f} s), extrapolated to all samples",
-            "build_particles_per_s": cols[0].size / tb, "interp_points_per_s": sel.size / ti}
+            "build_particles_per_s": cols[0].size / tb, "interp_points_per_s": sel.size / ti,
+            "interp_points_per_s_1thread": sel1.size / t1,
+            "sample_1thread": f"{sel1.size} samples on one thread ({t1:.2f} s)"}
 
 
 if __name__ == "__main__":
