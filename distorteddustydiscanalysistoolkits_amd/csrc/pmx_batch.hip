@@ -222,7 +222,7 @@ int make_units(pmx_batch* b) {
         const int per = J.kind == PMX_JOB_POINTS ? 2 : lmap::kMaxCols;
         float* outbuf = c->outbuf.as<float>();
         for (int base = 0; base < std::max(ns, 1); base += per) {
-            BatchUnit U;
+            BatchUnit U{};
             std::memset(&U, 0, sizeof(U));
             U.kind = J.kind;
             U.strategy = J.strategy;
