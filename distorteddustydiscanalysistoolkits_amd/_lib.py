@@ -242,8 +242,8 @@ class Context:
         self.ncol = 0
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().pmx_destroy(self._h)
+        if getattr(self, "_h", None) and _lib is not None:     # (module globals are gone at exit)
+            _lib.pmx_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -473,8 +473,8 @@ class Batch:
                     split_budget=int(v[8]))
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().pmx_batch_destroy(self._h)
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.pmx_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -536,8 +536,8 @@ class Phantom:
             _raise(rc, lib().pmx_phantom_last_error(self._h).decode())
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().pmx_phantom_close(self._h)
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.pmx_phantom_close(self._h)
             self._h = None
 
     def __del__(self):

@@ -33,7 +33,7 @@
 namespace pmx {
 
 int g_bucket_points = 32;
-int g_bucket_lines = 16;
+int g_bucket_lines = 64;
 
 namespace {
 
