@@ -15,6 +15,7 @@ const PMX_F32, PMX_F64 = Int32(0), Int32(1)
 const PMX_MEM_HOST = Int32(0)
 const PMX_PREC_F64, PMX_PREC_F32C = Int32(0), Int32(1)
 const PMX_BUILD_WRITEBACK = Int32(1)
+const PMX_LINE_INV_H2 = Int32(0x100)   # line weight Inorm / h^2 (an extension; default Inorm / h)
 
 struct PmxParticles
     dtype::Int32; mem::Int32; n::Int64
@@ -120,7 +121,7 @@ end
 
 function Partia.LineSamples_interpolation(backend::MI355XComputeBackend, grid_template::LineSamples{3,TF},
         input::InterpolationInput{3,TF}, catalog::InterpolationCatalog{3,N,0,0,0,N},
-        itp_strategy::Type{ITP} = itpScatter) where {N,TF<:AbstractFloat,ITP<:AbstractInterpolationStrategy}
+        itp_strategy::Type{ITP} = itpScatter; inv_h2::Bool = false) where {N,TF<:AbstractFloat,ITP<:AbstractInterpolationStrategy}
     itp_strategy === itpScatter || throw(ArgumentError(
         "LineSamples_interpolation only supports itpScatter. " *
         "Line-integrated samples do not have a well-defined query smoothing length ha, " *
@@ -141,7 +142,8 @@ function Partia.LineSamples_interpolation(backend::MI355XComputeBackend, grid_te
             out = Ref(PmxOutput(_dtype(TF), PMX_MEM_HOST, pointer(outp)))
             rc = ccall((:pmx_line_samples_interpolation, libpartia_mi355x), Cint,
                        (Ptr{Cvoid}, Ref{PmxParticles}, Int32, Int32, Ref{PmxCatalog}, Ref{PmxLines}, Ref{PmxOutput}),
-                       ctx, p, _strategy(itp_strategy), _precision(backend), cat, ln, out)
+                       ctx, p, _strategy(itp_strategy), _precision(backend) | (inv_h2 ? PMX_LINE_INV_H2 : Int32(0)),
+                       cat, ln, out)
             _check(ctx, rc)
         end
     end
