@@ -74,11 +74,6 @@ __global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ 
         out[i] = (double)in[i];
 }
 
-struct FullCat {
-    std::vector<int> ss, sh, gs, ds, cs;
-    int L() const { return (int)(ss.size() + 3 * gs.size() + ds.size() / 3 + cs.size()); }
-};
-
 int check_catalog(pmx_ctx* c, const pmx_catalog* cat, bool lines, FullCat& fc) {
     if (!cat) return set_error(c, PMX_ERR_ARGUMENT, "catalog: null pointer");
     const int ncol = c->tree.ncol;
@@ -112,6 +107,8 @@ int check_catalog(pmx_ctx* c, const pmx_catalog* cat, bool lines, FullCat& fc) {
     return PMX_OK;
 }
 
+}  // namespace
+
 // Split a catalog into chunks the fixed-size kernels hold (<= 4 scalars, 2 gradients,
 // 2 divergences, 2 curls each); every chunk writes its outputs at their catalog positions.
 std::vector<CatChunk> make_chunks(const FullCat& fc) {
@@ -137,6 +134,8 @@ std::vector<CatChunk> make_chunks(const FullCat& fc) {
     }
     return out;
 }
+
+namespace {
 
 int check_points(pmx_ctx* c, const pmx_points* pts) {
     if (!pts) return set_error(c, PMX_ERR_ARGUMENT, "points: null pointer");

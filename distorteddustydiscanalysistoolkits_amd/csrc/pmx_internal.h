@@ -127,6 +127,12 @@ struct CatChunk {
     int cs[6];
     int os, og, od, oc;   // catalog positions of this chunk's first scalar/gradient/div/curl output
 };
+// A whole catalog as 0-based slot lists (div / curl slots three per field).
+struct FullCat {
+    std::vector<int> ss, sh, gs, ds, cs;
+    int L() const { return (int)(ss.size() + 3 * gs.size() + ds.size() / 3 + cs.size()); }
+};
+std::vector<CatChunk> make_chunks(const FullCat& fc);
 int interp_points_f64(pmx_ctx* c, int strategy, const CatChunk& cat, const double* qx,
                       const double* qy, const double* qz, int64_t npts, double* out, int64_t ldo,
                       const double* ha_in, double* ha_out);

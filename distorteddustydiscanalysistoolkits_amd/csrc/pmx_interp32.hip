@@ -535,9 +535,10 @@ int prepare_f32(pmx_ctx* c) {
     return PMX_OK;
 }
 
-int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32_t* perm,
-                       const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo) {
-    // split the chunk into sub-chunks of <= 2 scalars, 1 gradient, 1 divergence, 1 curl
+// The launches of one catalog chunk: sub-chunks of <= 2 scalars, 1 gradient, 1 divergence and
+// 1 curl, each with its derivative fields and the LDS rows of the sums it holds.
+std::vector<PointCat> point_cats(const CatChunk& ch) {
+    std::vector<PointCat> cats;
     const int nsub = std::max({(ch.ns + 1) / 2, ch.ng, ch.nd, ch.nc, 1});
     for (int k = 0; k < nsub; ++k) {
         PointCat cat{};
@@ -578,6 +579,14 @@ int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32
         }
         cat.nacc = rows;
         if (cat.ns + cat.ng + cat.nd + cat.nc == 0 && k > 0) continue;
+        cats.push_back(cat);
+    }
+    return cats;
+}
+
+int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& ch, const int32_t* perm,
+                       const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo) {
+    for (const PointCat& cat : point_cats(ch)) {
         PointSplit sp;
         PMX_TRY(point_split(c, npts, strategy, &sp));
         switch (c->tree.kernel) {

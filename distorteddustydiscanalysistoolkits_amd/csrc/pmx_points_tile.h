@@ -776,4 +776,8 @@ __device__ inline void points_finish(const PointCat& cat, const int32_t* __restr
 }
 
 }  // namespace pts
+
+// The derivative / scalar launches one catalog chunk is split into (pmx_interp32.hip).
+std::vector<pts::PointCat> point_cats(const CatChunk& ch);
+
 }  // namespace pmx
