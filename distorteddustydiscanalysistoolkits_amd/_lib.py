@@ -414,7 +414,9 @@ class Batch:
     jobs: list of dicts with keys
       cols (x, y, z, m, h, rho), quant (list), kernel, kind ("points" | "lines"),
       strategy, samples ((qx, qy, qz) or ((ox, oy, oz), (dx, dy, dz))), scalar_slots,
-      scalar_shepard, out (list of FP32 device tensors, one per scalar)."""
+      scalar_shepard, grad_slots / div_slots / curl_slots (point jobs only, optional), out (list
+      of FP32 device tensors, one per output column in catalog order: scalars, 3 per gradient,
+      1 per divergence, 3 per curl)."""
 
     def __init__(self, jobs, device: int = 0):
         self._keep = []
@@ -426,8 +428,13 @@ class Batch:
             part = Particles(_dtype_code(cols[0]), _mem_of(cols + quant), int(cols[0].shape[0]),
                              *[_ptr(c) for c in cols], len(quant), C.cast(qp, C.POINTER(C.c_void_p)),
                              int(J.get("kernel", 0)))
-            cat = catalog_struct(scalar_slots=J["scalar_slots"], scalar_shepard=J.get("scalar_shepard"))
+            cat = catalog_struct(scalar_slots=J.get("scalar_slots", ()), scalar_shepard=J.get("scalar_shepard"),
+                                 grad_slots=J.get("grad_slots", ()), div_slots=J.get("div_slots", ()),
+                                 curl_slots=J.get("curl_slots", ()))
             outs = list(J["out"])
+            L = cat.n_scalar + 3 * cat.n_grad + cat.n_div + 3 * cat.n_curl
+            if len(outs) != L:
+                raise DimensionMismatch(PMX_ERR_DIMENSION, f"batch job {j}: {len(outs)} output columns for {L} outputs")
             op = (C.c_void_p * max(1, len(outs)))(*[_ptr(o) for o in outs])
             out = Output(PMX_F32, PMX_MEM_DEVICE, C.cast(op, C.POINTER(C.c_void_p)))
             job = arr[j]

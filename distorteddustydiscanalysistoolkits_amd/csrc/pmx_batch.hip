@@ -24,10 +24,12 @@
 namespace pmx {
 namespace {
 
-// One work unit: a job (or a group of its scalar columns) and its tile range in the queue.
+// One work unit: a job (or one sub-catalog of it: <= 2 scalars, 1 gradient, 1 divergence and
+// 1 curl for samples, <= 8 scalars for lines) and its tile range in the queue.
 struct BatchUnit {
     int kind;                  // PMX_JOB_POINTS / PMX_JOB_LINES
     int strategy;
+    int deriv;                 // points: the sub-catalog holds derivative terms (dynamic tile layout)
     T32 T;
     pts::PointCat pcat;
     const double* qx;
@@ -50,10 +52,19 @@ struct BatchUnit {
     int64_t ntiles;
 };
 
-template <int K>
+// Sample tiles of derivative sub-catalogs: the tile layout is sized by the unit's catalog.
+template <int K, int S>
+__device__ void batch_deriv_tile(const BatchUnit& U, int64_t t, pts::BPos& P, void* dyn, pts::PointSplit sp) {
+    pts::BTile<true> Y(dyn, U.pcat.nder, U.pcat.nacc);
+    pts::points_tile<K, S, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y, sp);
+}
+
+// DERIV: the batch holds derivative units (an instantiation of its own, launched only then, so
+// that scalar batches keep the smaller register budget).
+template <int K, bool DERIV>
 __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ units, int nunits, int64_t total,
                                               unsigned long long* __restrict__ counter, pts::PointSplit split) {
-    // dynamic LDS shared by the two tile kinds: points {BPos, BTile}, lines {LineTileLDS, dyn}
+    // dynamic LDS shared by the tile kinds: points {BPos, BTile}, lines {LineTileLDS, dyn}
     extern __shared__ double2 lds2[];             // 16-byte aligned dynamic LDS
     double* lds = reinterpret_cast<double*>(lds2);
     __shared__ long long item_s;
@@ -78,6 +89,15 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
         if (U.kind == PMX_JOB_POINTS) {
             pts::PointSplit sp = split;
             sp.unit = lo;
+            if constexpr (DERIV) {
+                if (U.deriv) {
+                    void* dyn = reinterpret_cast<char*>(lds) + sizeof(pts::BPos);
+                    if (U.strategy == kGather) batch_deriv_tile<K, kGather>(U, t, P, dyn, sp);
+                    else if (U.strategy == kScatter) batch_deriv_tile<K, kScatter>(U, t, P, dyn, sp);
+                    else batch_deriv_tile<K, kSymmetric>(U, t, P, dyn, sp);
+                    continue;
+                }
+            }
             if (U.strategy == kGather)
                 pts::points_tile<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y, sp);
             else if (U.strategy == kScatter)
@@ -93,19 +113,52 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
 }
 
 // Items of split point tiles of every job (after k_batch), then the split tiles' outputs.
+// The work unit of a queued split item.
+__device__ inline const BatchUnit& U_of(const BatchUnit* __restrict__ units, const pts::PointSplit& sp,
+                                        unsigned int item) {
+    return units[uniform(sp.slot_unit[uniform(sp.items[item].x)])];
+}
+
+// One queued item of a split scalar tile.
 template <int K>
+__device__ void batch_item(const BatchUnit& U, const pts::PointSplit& sp, unsigned int item, pts::BPos& P,
+                           pts::BTile<false>& Y) {
+    if (U.strategy == kGather)
+        pts::points_item<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
+    else
+        pts::points_item<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
+}
+
+// DERIV: dynamic LDS of max(sizeof(BTile<false>), the largest derivative tile) bytes, shared by
+// the two tile layouts; scalar batches keep the static layout.
+template <int K, bool DERIV>
 __global__ void __launch_bounds__(64) k_batch_items(const BatchUnit* __restrict__ units, pts::PointSplit sp) {
     __shared__ pts::BPos P;
-    __shared__ pts::BTile<false> Y;
     const unsigned int nitems = min(sp.counters[0], (unsigned int)sp.cap_items);
-    for (;;) {
-        const unsigned int item = pts::lane0_atomic_add(&sp.counters[2], 1u);
-        if (item >= nitems) break;
-        const BatchUnit& U = units[uniform(sp.slot_unit[uniform(sp.items[item].x)])];
-        if (U.strategy == kGather)
-            pts::points_item<K, kGather, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
-        else
-            pts::points_item<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Y);
+    if constexpr (DERIV) {
+        extern __shared__ double2 dyn2[];
+        for (;;) {
+            const unsigned int item = pts::lane0_atomic_add(&sp.counters[2], 1u);
+            if (item >= nitems) break;
+            const BatchUnit& U = U_of(units, sp, item);
+            if (U.deriv) {
+                pts::BTile<true> Yd(dyn2, U.pcat.nder, U.pcat.nacc);
+                if (U.strategy == kGather)
+                    pts::points_item<K, kGather, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Yd);
+                else
+                    pts::points_item<K, kSymmetric, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P,
+                                                         Yd);
+            } else {
+                batch_item<K>(U, sp, item, P, *reinterpret_cast<pts::BTile<false>*>(dyn2));
+            }
+        }
+    } else {
+        __shared__ pts::BTile<false> Y;
+        for (;;) {
+            const unsigned int item = pts::lane0_atomic_add(&sp.counters[2], 1u);
+            if (item >= nitems) break;
+            batch_item<K>(U_of(units, sp, item), sp, item, P, Y);
+        }
     }
 }
 
@@ -113,7 +166,8 @@ __global__ void __launch_bounds__(64) k_batch_finish(const BatchUnit* __restrict
     const int nslots = (int)min(sp.counters[1], (unsigned int)sp.cap_slots);
     for (int split = blockIdx.x; split < nslots; split += gridDim.x) {
         const BatchUnit& U = units[sp.slot_unit[split]];
-        pts::points_finish<false>(U.pcat, U.qperm, U.npts, U.out, U.ldo, sp, split);
+        if (U.deriv) pts::points_finish<true>(U.pcat, U.qperm, U.npts, U.out, U.ldo, sp, split);
+        else pts::points_finish<false>(U.pcat, U.qperm, U.npts, U.out, U.ldo, sp, split);
     }
 }
 
@@ -145,6 +199,8 @@ struct pmx_batch {
     int split_budget = 8192, split_item = 4096;
     int64_t total_tiles = 0;
     size_t lds_bytes = 0;
+    size_t lds_items = 0;                         // dynamic LDS of the split-item kernel
+    bool deriv = false;                           // some unit holds derivative terms
     int ncu = 256;
     // launch shape (autotuned)
     int waves_per_cu = 16, bucket_points = 16, bucket_lines = 512, heavy_first = 1;
@@ -204,6 +260,22 @@ int make_split(pmx_batch* b) {
     return PMX_OK;
 }
 
+// Output columns of a catalog: scalars, 3 per gradient, 1 per divergence, 3 per curl.
+int out_columns(const pmx_catalog& cat) { return cat.n_scalar + 3 * cat.n_grad + cat.n_div + 3 * cat.n_curl; }
+
+// A job's catalog as slot lists (validated at creation; a gradient slot of -1 is rho's).
+FullCat full_catalog(const pmx_catalog& cat) {
+    FullCat fc;
+    for (int j = 0; j < cat.n_scalar; ++j) {
+        fc.ss.push_back(cat.scalar_slots[j]);
+        fc.sh.push_back(cat.scalar_shepard ? (cat.scalar_shepard[j] ? 1 : 0) : 1);
+    }
+    for (int j = 0; j < cat.n_grad; ++j) fc.gs.push_back(cat.grad_slots[j]);
+    for (int j = 0; j < 3 * cat.n_div; ++j) fc.ds.push_back(cat.div_slots[j]);
+    for (int j = 0; j < 3 * cat.n_curl; ++j) fc.cs.push_back(cat.curl_slots[j]);
+    return fc;
+}
+
 // Work units from the jobs with the current launch shape; tile ranges in queue order.
 int make_units(pmx_batch* b) {
     b->units.clear();
@@ -212,59 +284,72 @@ int make_units(pmx_batch* b) {
         b->split.item_max = std::max(b->split_item, 2 * b->bucket_points);
     }
     size_t lds = std::max<size_t>(sizeof(pts::BPos) + sizeof(pts::BTile<false>), sizeof(lmap::LineTileLDS) + 64);
+    size_t lds_items = sizeof(pts::BTile<false>);
+    b->deriv = false;
     std::vector<std::pair<double, BatchUnit>> list;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const pmx_batch_job& J = b->jobs[j];
         pmx_ctx* c = b->ctx[j];
         const int64_t nP = J.kind == PMX_JOB_POINTS ? J.points.n : J.lines.n;
         const double weight = (double)c->tree.n;        // heavier snapshots first
-        const int ns = J.catalog.n_scalar;
-        const int per = J.kind == PMX_JOB_POINTS ? 2 : lmap::kMaxCols;
         float* outbuf = c->outbuf.as<float>();
-        for (int base = 0; base < std::max(ns, 1); base += per) {
+        auto unit = [&] {
             BatchUnit U{};
             std::memset(&U, 0, sizeof(U));
             U.kind = J.kind;
             U.strategy = J.strategy;
             U.out = outbuf;
             U.ldo = nP;
+            return U;
+        };
+        if (J.kind == PMX_JOB_POINTS) {
+            // the per-call split: chunks of the whole catalog, then their sub-catalogs
+            for (const CatChunk& ch : make_chunks(full_catalog(J.catalog)))
+                for (const pts::PointCat& cat : point_cats(ch)) {
+                    BatchUnit U = unit();
+                    U.T = tree32(c, b->bucket_points);
+                    U.pcat = cat;
+                    U.deriv = cat.ng + cat.nd + cat.nc > 0;
+                    U.qx = J.points.x;
+                    U.qy = J.points.y;
+                    U.qz = J.points.z;
+                    U.qperm = b->qperm[j];
+                    U.npts = nP;
+                    U.ntiles = (nP + 63) / 64;
+                    if (U.deriv) {
+                        const size_t dl = pts::deriv_tile_lds(cat.nder, cat.nacc);
+                        lds = std::max(lds, sizeof(pts::BPos) + dl);
+                        lds_items = std::max(lds_items, dl);
+                        b->deriv = true;
+                    }
+                    list.push_back({weight, U});
+                }
+            continue;
+        }
+        const int ns = J.catalog.n_scalar;
+        const int per = lmap::kMaxCols;
+        for (int base = 0; base < std::max(ns, 1); base += per) {
+            BatchUnit U = unit();
             const int cnt = std::min(per, ns - base);
-            if (J.kind == PMX_JOB_POINTS) {
-                U.T = tree32(c, b->bucket_points);
-                U.pcat.ns = cnt;
-                for (int k = 0; k < cnt; ++k) {
-                    U.pcat.scol[k] = 1 + J.catalog.scalar_slots[base + k];
-                    U.pcat.sh[k] = J.catalog.scalar_shepard ? (J.catalog.scalar_shepard[base + k] ? 1 : 0) : 1;
-                }
-                U.pcat.os = base;
-                U.qx = J.points.x;
-                U.qy = J.points.y;
-                U.qz = J.points.z;
-                U.qperm = b->qperm[j];
-                U.npts = nP;
-                U.ntiles = (nP + 63) / 64;
-            } else {
-                U.T = tree32(c, 64);
-                U.T.bucket = b->bucket_lines;
-                U.f = b->plan[j].f;
-                U.f.near = (double)g_tiled_near;
-                U.lcat.n = cnt;
-                U.lcat.h2 = 0;                  // batches weight by Inorm / h (the reference's)
-                for (int k = 0; k < cnt; ++k) {
-                    U.lcat.fcol[k] = 1 + J.catalog.scalar_slots[base + k];
-                    U.lcat.shep[k] = J.catalog.scalar_shepard ? (J.catalog.scalar_shepard[base + k] ? 1 : 0) : 1;
-                }
-                U.cstart = b->plan[j].cstart;
-                U.lperm = b->plan[j].perm;
-                U.sts = b->plan[j].sts;
-                U.o = b->lines_o[j];
-                U.nl = nP;
-                U.lut = c->lutf.as<float>() + (size_t)b->kernel * kLutN;
-                U.inv_dq = 1.0 / c->lut_dq[b->kernel];
-                U.out = outbuf + (int64_t)base * nP;
-                U.ntiles = (int64_t)U.f.tx * U.f.ty;
-                lds = std::max(lds, sizeof(lmap::LineTileLDS) + lmap::lines_tile_lds(cnt, 0, true));
+            U.T = tree32(c, 64);
+            U.T.bucket = b->bucket_lines;
+            U.f = b->plan[j].f;
+            U.f.near = (double)g_tiled_near;
+            U.lcat.n = cnt;
+            for (int k = 0; k < cnt; ++k) {
+                U.lcat.fcol[k] = 1 + J.catalog.scalar_slots[base + k];
+                U.lcat.shep[k] = J.catalog.scalar_shepard ? (J.catalog.scalar_shepard[base + k] ? 1 : 0) : 1;
             }
+            U.cstart = b->plan[j].cstart;
+            U.lperm = b->plan[j].perm;
+            U.sts = b->plan[j].sts;
+            U.o = b->lines_o[j];
+            U.nl = nP;
+            U.lut = c->lutf.as<float>() + (size_t)b->kernel * kLutN;
+            U.inv_dq = 1.0 / c->lut_dq[b->kernel];
+            U.out = outbuf + (int64_t)base * nP;
+            U.ntiles = (int64_t)U.f.tx * U.f.ty;
+            lds = std::max(lds, sizeof(lmap::LineTileLDS) + lmap::lines_tile_lds(cnt, 0, true));
             list.push_back({weight, U});
         }
     }
@@ -278,6 +363,7 @@ int make_units(pmx_batch* b) {
     }
     b->total_tiles = t0;
     b->lds_bytes = (lds + 15) & ~(size_t)15;
+    b->lds_items = (lds_items + 15) & ~(size_t)15;
     pmx_ctx* c0 = b->ctx[0];
     if (ensure(c0, b->units_dev, sizeof(BatchUnit) * std::max<size_t>(b->units.size(), 1)) != PMX_OK)
         return bfail(b, PMX_ERR_NOMEM, "batch: unit table allocation failed");
@@ -290,10 +376,13 @@ void launch_batch(pmx_batch* b, int64_t grid) {
     hipStream_t s = b->stream;
     const BatchUnit* U = b->units_dev.as<BatchUnit>();
     const pts::PointSplit& sp = b->split;
-    hipLaunchKernelGGL(k_batch<K>, dim3(grid), dim3(64), b->lds_bytes, s, U, (int)b->units.size(), b->total_tiles,
+    auto kb = b->deriv ? k_batch<K, true> : k_batch<K, false>;
+    auto ki = b->deriv ? k_batch_items<K, true> : k_batch_items<K, false>;
+    const size_t lds_items = b->deriv ? b->lds_items : 0;
+    hipLaunchKernelGGL(kb, dim3(grid), dim3(64), b->lds_bytes, s, U, (int)b->units.size(), b->total_tiles,
                        b->counter.as<unsigned long long>(), sp);
     if (sp.budget <= 0) return;
-    hipLaunchKernelGGL(k_batch_items<K>, dim3(b->ncu * 32), dim3(64), 0, s, U, sp);
+    hipLaunchKernelGGL(ki, dim3(b->ncu * 32), dim3(64), lds_items, s, U, sp);
     hipLaunchKernelGGL(k_batch_finish, dim3(std::min(sp.cap_slots, 4096)), dim3(64), 0, s, U, sp);
 }
 
@@ -317,7 +406,7 @@ int enqueue_interpolation(pmx_batch* b) {
         const pmx_batch_job& J = b->jobs[j];
         const int64_t nP = J.kind == PMX_JOB_POINTS ? J.points.n : J.lines.n;
         const float* src = b->ctx[j]->outbuf.as<float>();
-        for (int k = 0; k < J.catalog.n_scalar; ++k)
+        for (size_t k = 0; k < b->out_cols[j].size(); ++k)
             PMXB_HIP(b, hipMemcpyAsync(b->out_cols[j][k], src + (int64_t)k * nP, 4 * (size_t)nP, hipMemcpyDeviceToDevice, s));
     }
     return PMX_OK;
@@ -394,11 +483,28 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
             return bfail(b, PMX_ERR_ARGUMENT, tag + "LineSamples_interpolation only supports itpScatter.");
         if (J.kind == PMX_JOB_POINTS && (J.strategy < 0 || J.strategy > 2))
             return bfail(b, PMX_ERR_ARGUMENT, tag + "unknown interpolation strategy");
-        if (J.catalog.n_grad || J.catalog.n_div || J.catalog.n_curl || J.catalog.n_scalar < 1)
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "batches take scalar catalogs only");
-        for (int k = 0; k < J.catalog.n_scalar; ++k)
-            if (J.catalog.scalar_slots[k] < 0 || J.catalog.scalar_slots[k] >= J.particles.ncol)
-                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: scalar slot out of range");
+        const pmx_catalog& Cg = J.catalog;
+        if (Cg.n_scalar < 0 || Cg.n_grad < 0 || Cg.n_div < 0 || Cg.n_curl < 0)
+            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: negative count");
+        if (J.kind == PMX_JOB_LINES && (Cg.n_grad || Cg.n_div || Cg.n_curl))
+            return bfail(b, PMX_ERR_ARGUMENT,
+                         tag + "LineSamples_interpolation: catalog must hold scalars only (InterpolationCatalog{3,N,0,0,0,N})");
+        if (out_columns(Cg) < 1) return bfail(b, PMX_ERR_ARGUMENT, tag + "catalog: no quantities");
+        const int nc = J.particles.ncol;
+        auto bad = [nc](const int* v, int n, bool rho_ok) {
+            if (n > 0 && !v) return true;
+            for (int k = 0; k < n; ++k)
+                if (!((v[k] >= 0 && v[k] < nc) || (rho_ok && v[k] == -1))) return true;
+            return false;
+        };
+        if (bad(Cg.scalar_slots, Cg.n_scalar, false))
+            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: scalar slot out of range");
+        if (bad(Cg.grad_slots, Cg.n_grad, true))
+            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: gradient slot out of range");
+        if (bad(Cg.div_slots, 3 * Cg.n_div, false))
+            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: divergence slot out of range");
+        if (bad(Cg.curl_slots, 3 * Cg.n_curl, false))
+            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: curl slot out of range");
         const pmx_output& O = J.out;
         if (O.dtype != PMX_F32 || O.mem != PMX_MEM_DEVICE || !O.cols)
             return bfail(b, PMX_ERR_ARGUMENT, tag + "outputs must be FP32 device columns");
@@ -414,7 +520,7 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
     b->out_cols.resize(njobs);
     for (int j = 0; j < njobs; ++j) {
         pmx_batch_job& J = b->jobs[j];
-        b->out_cols[j].assign(J.out.cols, J.out.cols + J.catalog.n_scalar);
+        b->out_cols[j].assign(J.out.cols, J.out.cols + out_columns(J.catalog));
         pmx_ctx* c = nullptr;
         const int rc = pmx_create(&c, device_ordinal);
         if (rc != PMX_OK) return bfail(b, rc, "batch: context creation failed");
@@ -423,7 +529,7 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
         int r = build_lbvh(c, &J.particles, 0, nullptr, nullptr);
         if (r != PMX_OK) return ctx_fail(b, c, r, j);
         const int64_t nP = J.kind == PMX_JOB_POINTS ? J.points.n : J.lines.n;
-        if (ensure(c, c->outbuf, 4 * (size_t)nP * J.catalog.n_scalar) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
+        if (ensure(c, c->outbuf, 4 * (size_t)nP * out_columns(J.catalog)) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
         if (J.kind == PMX_JOB_POINTS) {
             int32_t* perm = nullptr;
             r = sort_queries(c, J.points.x, J.points.y, J.points.z, nP, &perm);

@@ -36,7 +36,6 @@ struct TileCat {
     int n;
     int fcol[kMaxCols];   // payload column of each output (1.. = quant)
     int shep[kMaxCols];
-    int h2;               // weight Inorm / h^2 instead of Inorm / h
 };
 
 // Extent of an outward-rounded FP32 box projected on a unit axis a.
@@ -82,8 +81,10 @@ __host__ __device__ inline size_t lines_tile_lds(int ncol, int mode, bool wg1 = 
 // itself, and barriers are workgroup barriers.  Otherwise other wavefronts of the workgroup work
 // on other tiles: wavefront barriers only, and each line is read from global memory one pair
 // ahead (a global read waits on the vector-memory counter, not behind the LDS atomics).
-template <int K, int MODE, bool WG1 = false>
-__device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& cat, const int32_t* __restrict__ cstart,
+// Always inlined: it is called from several kernels of one file, and an out-of-line call
+// makes each caller's register allocation conservative.
+template <int K, int MODE, bool WG1 = false, bool H2 = false>
+__device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, const Frame& f, const TileCat& cat, const int32_t* __restrict__ cstart,
                                   const int32_t* __restrict__ perm, const double2* __restrict__ st_sorted,
                                   const double* __restrict__ o, int64_t nl, const float* __restrict__ lut,
                                   double inv_dq, float* __restrict__ out, int64_t ldo, uint64_t* __restrict__ digest,
@@ -181,7 +182,7 @@ __device__ inline void lines_tile(const T32& T, const Frame& f, const TileCat& c
                         if (MODE == 0) {
                             // (m/rho) / h and 1 / (h dq) of line_integrated_kernel_function
                             const double invh = T.ih64[p];
-                            w_wih[lane] = (T.w64[p] * invh) * (cat.h2 ? invh : 1.0);
+                            w_wih[lane] = H2 ? (T.w64[p] * invh) * invh : T.w64[p] * invh;
                             w_xs[lane] = invh * inv_dq;
                             for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                         }

@@ -106,7 +106,9 @@ __global__ void __launch_bounds__(256) k_line_cells(const uint64_t* __restrict__
 // wavefronts of a workgroup stay busy until the map is done.
 constexpr int kTileWaves = 8;
 
-template <int K, int MODE>
+// H2: weight Inorm / h^2 instead of Inorm / h (a compile-time choice, so the default kernel
+// carries no test of it).
+template <int K, int MODE, bool H2 = false>
 __global__ void __launch_bounds__(64 * kTileWaves) k_lines_tiled(T32 T, Frame f, TileCat cat,
                                                                  const int32_t* __restrict__ cstart,
                                                                  const int32_t* __restrict__ perm,
@@ -129,42 +131,42 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_lines_tiled(T32 T, Frame f,
         if ((threadIdx.x & 63) == 0) t = atomicAdd(next_tile, 1u);
         const int64_t tile = (int64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0));
         if (tile >= ntiles) break;
-        lines_tile<K, MODE>(T, f, cat, cstart, perm, st_sorted, o, nl, lut_s, inv_dq, out, ldo, digest, tile,
-                            Ls[w], dyn);
+        lines_tile<K, MODE, false, H2>(T, f, cat, cstart, perm, st_sorted, o, nl, lut_s, inv_dq, out, ldo, digest, tile,
+                                       Ls[w], dyn);
     }
 }
 
-template <int K, int MODE>
+template <int K, int MODE, bool H2>
 int launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
                  const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest,
                  unsigned* next_tile) {
     const int64_t ntiles = (int64_t)f.tx * f.ty;
     const size_t lds = lines_tile_lds(cat.n, MODE) * kTileWaves;
     int per_cu = 0;
-    PMX_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lines_tiled<K, MODE>, 64 * kTileWaves, lds));
+    PMX_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lines_tiled<K, MODE, H2>, 64 * kTileWaves, lds));
     const int64_t resident = (int64_t)std::max(1, per_cu) * c->ncu;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(resident, (ntiles + kTileWaves - 1) / kTileWaves));
     T32 T = tree32(c, 64);
     T.bucket = g_tiled_bucket;
     PMX_HIP(c, hipMemsetAsync(next_tile, 0, sizeof(unsigned), c->stream));
-    hipLaunchKernelGGL((k_lines_tiled<K, MODE>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat, cells,
+    hipLaunchKernelGGL((k_lines_tiled<K, MODE, H2>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat, cells,
                        perm, sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out, ldo,
                        digest, next_tile);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
 }
 
-template <int MODE>
+template <int MODE, bool H2 = false>
 int launch_tiled_k(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* cells, const int32_t* perm,
                    const double2* sts, const double* o, int64_t nl, float* out, int64_t ldo, uint64_t* digest,
                    unsigned* next_tile) {
     switch (c->tree.kernel) {
-    case kM4: return launch_tiled<kM4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
-    case kM5: return launch_tiled<kM5, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
-    case kM6: return launch_tiled<kM6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
-    case kC2: return launch_tiled<kC2, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
-    case kC4: return launch_tiled<kC4, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
-    default: return launch_tiled<kC6, MODE>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kM4: return launch_tiled<kM4, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kM5: return launch_tiled<kM5, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kM6: return launch_tiled<kM6, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kC2: return launch_tiled<kC2, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    case kC4: return launch_tiled<kC4, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
+    default: return launch_tiled<kC6, MODE, H2>(c, f, cat, cells, perm, sts, o, nl, out, ldo, digest, next_tile);
     }
 }
 
@@ -326,9 +328,11 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
         TileCat cat{};
         cat.n = std::min(kMaxCols, ncols - base);
         for (int j = 0; j < cat.n; ++j) { cat.fcol[j] = 1 + slots[base + j]; cat.shep[j] = shep[base + j]; }
-        cat.h2 = c->line_inv_h2;
-        PMX_TRY(launch_tiled_k<0>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, out + (int64_t)base * ldo, ldo, nullptr,
-                                  B.next_tile));
+        float* ob = out + (int64_t)base * ldo;
+        if (c->line_inv_h2)
+            PMX_TRY((launch_tiled_k<0, true>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, ob, ldo, nullptr, B.next_tile)));
+        else
+            PMX_TRY((launch_tiled_k<0, false>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, ob, ldo, nullptr, B.next_tile)));
         if (ncols == 0) break;
     }
     PMX_HIP(c, hipGetLastError());

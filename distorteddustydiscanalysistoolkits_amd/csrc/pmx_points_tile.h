@@ -433,7 +433,7 @@ struct PointWalk<K, S, false> {
 
     // a run of leaves lo .. lo+cnt-1: stage, test every particle against every lane's sample,
     // queue the accepted pairs and process them 64 at a time
-    __device__ void run(int lo, int cnt) {
+    __device__ __attribute__((always_inline)) void run(int lo, int cnt) {
         const int64_t n = T.n;
         stage_pos(T, P, lo, cnt);
         if (lane < cnt) {
@@ -589,7 +589,7 @@ struct PointWalk<K, S, true> {
         __syncthreads();
     }
 
-    __device__ void run(int lo, int cnt) {
+    __device__ __attribute__((always_inline)) void run(int lo, int cnt) {
         const int64_t n = T.n;
         stage_pos(T, P, lo, cnt);
         if (lane < cnt) {

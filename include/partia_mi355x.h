@@ -245,7 +245,9 @@ int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
  * one LBVH per job resident, builds all of them, then runs every job's sample tiles in one
  * persistent kernel fed by a device work queue; builds and kernel can be captured into one
  * hipGraph and replayed, and pmx_batch_tune picks launch shapes by timing them.
- * Restrictions: the FP32-compensated path, scalar catalogs, one SPH kernel for all jobs,
+ * Point jobs take any catalog (scalars, gradients, divergences, curls), split as in
+ * pmx_interpolate_points; line jobs take scalars only (as pmx_interpolate_lines).
+ * Restrictions: the FP32-compensated path, one SPH kernel for all jobs,
  * particle columns / samples / outputs in device memory (FP32 outputs), line jobs must be one
  * parallel unit-direction line set each.  Job buffers are borrowed for the batch's lifetime. */
 typedef struct pmx_batch pmx_batch;
@@ -256,8 +258,9 @@ typedef struct pmx_batch_job {
     int32_t strategy;          /* points: PMX_GATHER / PMX_SCATTER / PMX_SYMMETRIC; lines: PMX_SCATTER */
     pmx_points points;         /* kind PMX_JOB_POINTS; mem = PMX_MEM_DEVICE */
     pmx_lines lines;           /* kind PMX_JOB_LINES; mem = PMX_MEM_DEVICE */
-    pmx_catalog catalog;       /* scalars only */
-    pmx_output out;            /* dtype PMX_F32, mem = PMX_MEM_DEVICE, one column per scalar */
+    pmx_catalog catalog;       /* points: any catalog; lines: scalars only */
+    pmx_output out;            /* dtype PMX_F32, mem = PMX_MEM_DEVICE, one column per output in
+                                  catalog order (n_scalar + 3 n_grad + n_div + 3 n_curl) */
 } pmx_batch_job;
 /* Plans the batch: one build per job (sizes every buffer), sample ordering, line binning. */
 int pmx_batch_create(pmx_batch **out, int32_t device_ordinal, int32_t njobs, const pmx_batch_job *jobs);
