@@ -303,7 +303,6 @@ struct LineCat32 {
     int n;
     int fcol[8];    // payload column of each output (1.. = quant)
     int shep[8];
-    int h2;         // weight Inorm / h^2 instead of Inorm / h
 };
 
 struct BLine {
@@ -312,7 +311,8 @@ struct BLine {
     float f[8][kMaxBucket];
 };
 
-template <int K>
+// H2: weight Inorm / h^2 instead of Inorm / h (compile-time, as in the tiled line kernel).
+template <int K, bool H2>
 __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const double* __restrict__ o,
                                                    const double* __restrict__ dvec, const int32_t* __restrict__ perm,
                                                    int64_t nl, const double* __restrict__ lut, double dq,
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
                 const double invh = Y.invh64[j];
                 const double qq = sqrt(cx * cx + cy * cy + cz * cz) * invh;
                 const double I = qq >= Kv ? 0.0 : lin_lut_f64(qq, dq, lut);
-                const double w = Y.w64[j] * (cat.h2 ? (invh * I) * invh : invh * I);
+                const double w = Y.w64[j] * (H2 ? (invh * I) * invh : invh * I);
                 S1 += w;
 #pragma unroll
                 for (int f = 0; f < 8; ++f)
@@ -483,7 +483,8 @@ int point_split(pmx_ctx* c, int64_t npts, int strategy, PointSplit* sp) {
 template <int K>
 void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const double* d, const int32_t* perm,
                     int64_t nl, float* out, int64_t ldo) {
-    hipLaunchKernelGGL(k_lines32<K>, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines), cat, o, d,
+    auto kern = c->line_inv_h2 ? k_lines32<K, true> : k_lines32<K, false>;
+    hipLaunchKernelGGL(kern, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines), cat, o, d,
                        perm, nl, c->lut64.as<double>() + (size_t)K * kLutN, c->lut_dq[K], out, ldo);
 }
 
@@ -608,7 +609,6 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
         LineCat32 cat{};
         cat.n = ncols - base < 8 ? ncols - base : 8;
         for (int j = 0; j < cat.n; ++j) { cat.fcol[j] = 1 + slots[base + j]; cat.shep[j] = shep[base + j]; }
-        cat.h2 = c->line_inv_h2;
         float* ob = out + (int64_t)base * ldo;
         switch (c->tree.kernel) {
         case kM4: launch_lines32<kM4>(c, cat, o, d, perm, nl, ob, ldo); break;
