@@ -111,6 +111,16 @@ def test_deriv_batch_eager_and_graph(deriv_setup):
         _compare(jobs, specs)
 
 
+def test_deriv_batch_tuned(deriv_setup):
+    """The autotuner re-plans the units (derivative tile LDS included) for every candidate shape."""
+    b, jobs, specs, _ = deriv_setup
+    b.tune()
+    print("[batch deriv] tuned", b.info(), flush=True)
+    _clear(jobs)
+    b.run(graph=True)
+    _compare(jobs, specs)
+
+
 def test_deriv_batch_against_oracle(deriv_setup):
     b, jobs, specs, host = deriv_setup
     _clear(jobs)
