@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libpartia_mi355x.so")
 
 PMX_OK, PMX_ERR_ARGUMENT, PMX_ERR_DIMENSION, PMX_ERR_STATE, PMX_ERR_HIP, PMX_ERR_NOMEM, PMX_ERR_CAPACITY = (
     0, -1, -2, -3, -4, -5, -6)
-PMX_ERR_FORMAT, PMX_ERR_IO = -7, -8
+PMX_ERR_FORMAT, PMX_ERR_IO, PMX_ERR_INTERNAL = -7, -8, -9
 PMX_I8, PMX_I16, PMX_I32, PMX_I64 = 10, 11, 12, 13
 PMX_F32, PMX_F64 = 0, 1
 PMX_MEM_HOST, PMX_MEM_DEVICE = 0, 1

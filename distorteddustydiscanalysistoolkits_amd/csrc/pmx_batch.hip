@@ -210,6 +210,17 @@ struct pmx_batch {
     double info[4] = {0, 0, 0, 0};
 };
 
+namespace pmx {
+void note_exception(pmx_batch* b, const char* msg) noexcept {
+    if (!b) return;
+    try {
+        b->err = msg;
+    } catch (...) {
+    }
+}
+void note_exception(const pmx_batch* b, const char* msg) noexcept { note_exception(const_cast<pmx_batch*>(b), msg); }
+}  // namespace pmx
+
 namespace {
 
 int bfail(pmx_batch* b, int code, const std::string& msg) {
@@ -334,7 +345,7 @@ int make_units(pmx_batch* b) {
             U.T = tree32(c, 64);
             U.T.bucket = b->bucket_lines;
             U.f = b->plan[j].f;
-            U.f.near = (double)g_tiled_near;
+            U.f.near = (double)c->tune.tiled_near;
             U.lcat.n = cnt;
             for (int k = 0; k < cnt; ++k) {
                 U.lcat.fcol[k] = 1 + J.catalog.scalar_slots[base + k];
@@ -459,204 +470,214 @@ int time_interpolation(pmx_batch* b, double* ms) {
 extern "C" {
 
 int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, const pmx_batch_job* jobs) {
-    if (!out) return PMX_ERR_ARGUMENT;
-    *out = nullptr;
-    pmx_batch* b = new (std::nothrow) pmx_batch();
-    if (!b) return PMX_ERR_NOMEM;
-    *out = b;
-    if (njobs < 1 || !jobs) return bfail(b, PMX_ERR_ARGUMENT, "batch: needs at least one job");
-    b->device = device_ordinal;
-    PMXB_HIP(b, hipSetDevice(device_ordinal));
-    PMXB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    for (auto& e : b->ev) PMXB_HIP(b, hipEventCreate(&e));
-    hipDeviceProp_t prop;
-    PMXB_HIP(b, hipGetDeviceProperties(&prop, device_ordinal));
-    b->ncu = std::max(1, prop.multiProcessorCount);
-    b->kernel = jobs[0].particles.kernel;
-    for (int j = 0; j < njobs; ++j) {
-        const pmx_batch_job& J = jobs[j];
-        const std::string tag = "batch job " + std::to_string(j) + ": ";
-        if (J.particles.kernel != b->kernel) return bfail(b, PMX_ERR_ARGUMENT, tag + "all jobs must use one SPH kernel");
-        if (J.particles.mem != PMX_MEM_DEVICE) return bfail(b, PMX_ERR_ARGUMENT, tag + "particle columns must be device memory");
-        if (J.kind != PMX_JOB_POINTS && J.kind != PMX_JOB_LINES) return bfail(b, PMX_ERR_ARGUMENT, tag + "unknown kind");
-        if (J.kind == PMX_JOB_LINES && J.strategy != PMX_SCATTER)
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "LineSamples_interpolation only supports itpScatter.");
-        if (J.kind == PMX_JOB_POINTS && (J.strategy < 0 || J.strategy > 2))
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "unknown interpolation strategy");
-        const pmx_catalog& Cg = J.catalog;
-        if (Cg.n_scalar < 0 || Cg.n_grad < 0 || Cg.n_div < 0 || Cg.n_curl < 0)
-            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: negative count");
-        if (J.kind == PMX_JOB_LINES && (Cg.n_grad || Cg.n_div || Cg.n_curl))
-            return bfail(b, PMX_ERR_ARGUMENT,
-                         tag + "LineSamples_interpolation: catalog must hold scalars only (InterpolationCatalog{3,N,0,0,0,N})");
-        if (out_columns(Cg) < 1) return bfail(b, PMX_ERR_ARGUMENT, tag + "catalog: no quantities");
-        const int nc = J.particles.ncol;
-        auto bad = [nc](const int* v, int n, bool rho_ok) {
-            if (n > 0 && !v) return true;
-            for (int k = 0; k < n; ++k)
-                if (!((v[k] >= 0 && v[k] < nc) || (rho_ok && v[k] == -1))) return true;
-            return false;
-        };
-        if (bad(Cg.scalar_slots, Cg.n_scalar, false))
-            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: scalar slot out of range");
-        if (bad(Cg.grad_slots, Cg.n_grad, true))
-            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: gradient slot out of range");
-        if (bad(Cg.div_slots, 3 * Cg.n_div, false))
-            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: divergence slot out of range");
-        if (bad(Cg.curl_slots, 3 * Cg.n_curl, false))
-            return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: curl slot out of range");
-        const pmx_output& O = J.out;
-        if (O.dtype != PMX_F32 || O.mem != PMX_MEM_DEVICE || !O.cols)
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "outputs must be FP32 device columns");
-        if (J.kind == PMX_JOB_POINTS && (J.points.mem != PMX_MEM_DEVICE || J.points.n < 1))
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "points must be device memory, n >= 1");
-        if (J.kind == PMX_JOB_LINES && (J.lines.mem != PMX_MEM_DEVICE || J.lines.n < 1))
-            return bfail(b, PMX_ERR_ARGUMENT, tag + "lines must be device memory, n >= 1");
-    }
-    b->jobs.assign(jobs, jobs + njobs);
-    b->qperm.assign(njobs, nullptr);
-    b->plan.resize(njobs);
-    b->lines_o.assign(njobs, nullptr);
-    b->out_cols.resize(njobs);
-    for (int j = 0; j < njobs; ++j) {
-        pmx_batch_job& J = b->jobs[j];
-        b->out_cols[j].assign(J.out.cols, J.out.cols + out_columns(J.catalog));
-        pmx_ctx* c = nullptr;
-        const int rc = pmx_create(&c, device_ordinal);
-        if (rc != PMX_OK) return bfail(b, rc, "batch: context creation failed");
-        b->ctx.push_back(c);
-        // plan: a full (synchronous) build sizes every buffer the captured builds reuse
-        int r = build_lbvh(c, &J.particles, 0, nullptr, nullptr);
-        if (r != PMX_OK) return ctx_fail(b, c, r, j);
-        const int64_t nP = J.kind == PMX_JOB_POINTS ? J.points.n : J.lines.n;
-        if (ensure(c, c->outbuf, 4 * (size_t)nP * out_columns(J.catalog)) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
-        if (J.kind == PMX_JOB_POINTS) {
-            int32_t* perm = nullptr;
-            r = sort_queries(c, J.points.x, J.points.y, J.points.z, nP, &perm);
-            if (r != PMX_OK) return ctx_fail(b, c, r, j);
-            b->qperm[j] = perm;
-        } else {
-            if (ensure(c, c->qbuf, 48 * (size_t)nP) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
-            double* q = c->qbuf.as<double>();
-            const double* src[6] = {J.lines.ox, J.lines.oy, J.lines.oz, J.lines.dx, J.lines.dy, J.lines.dz};
-            for (int k = 0; k < 6; ++k)
-                PMXB_HIP(b, hipMemcpyAsync(q + k * nP, src[k], 8 * (size_t)nP, hipMemcpyDeviceToDevice, c->stream));
-            bool ok = false;
-            r = lines_plan(c, q, q + 3 * nP, nP, &b->plan[j], &ok);
-            if (r != PMX_OK) return ctx_fail(b, c, r, j);
-            if (!ok) return bfail(b, PMX_ERR_ARGUMENT, "batch job " + std::to_string(j) +
-                                                           ": lines must be one parallel unit-direction set");
-            b->lines_o[j] = q;
+    return guard(nullptr, [&]() -> int {
+        if (!out) return PMX_ERR_ARGUMENT;
+        *out = nullptr;
+        pmx_batch* b = new (std::nothrow) pmx_batch();
+        if (!b) return PMX_ERR_NOMEM;
+        *out = b;
+        if (njobs < 1 || !jobs) return bfail(b, PMX_ERR_ARGUMENT, "batch: needs at least one job");
+        b->device = device_ordinal;
+        PMXB_HIP(b, hipSetDevice(device_ordinal));
+        PMXB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        for (auto& e : b->ev) PMXB_HIP(b, hipEventCreate(&e));
+        hipDeviceProp_t prop;
+        PMXB_HIP(b, hipGetDeviceProperties(&prop, device_ordinal));
+        b->ncu = std::max(1, prop.multiProcessorCount);
+        b->kernel = jobs[0].particles.kernel;
+        for (int j = 0; j < njobs; ++j) {
+            const pmx_batch_job& J = jobs[j];
+            const std::string tag = "batch job " + std::to_string(j) + ": ";
+            if (J.particles.kernel != b->kernel) return bfail(b, PMX_ERR_ARGUMENT, tag + "all jobs must use one SPH kernel");
+            if (J.particles.mem != PMX_MEM_DEVICE) return bfail(b, PMX_ERR_ARGUMENT, tag + "particle columns must be device memory");
+            if (J.kind != PMX_JOB_POINTS && J.kind != PMX_JOB_LINES) return bfail(b, PMX_ERR_ARGUMENT, tag + "unknown kind");
+            if (J.kind == PMX_JOB_LINES && J.strategy != PMX_SCATTER)
+                return bfail(b, PMX_ERR_ARGUMENT, tag + "LineSamples_interpolation only supports itpScatter.");
+            if (J.kind == PMX_JOB_POINTS && (J.strategy < 0 || J.strategy > 2))
+                return bfail(b, PMX_ERR_ARGUMENT, tag + "unknown interpolation strategy");
+            const pmx_catalog& Cg = J.catalog;
+            if (Cg.n_scalar < 0 || Cg.n_grad < 0 || Cg.n_div < 0 || Cg.n_curl < 0)
+                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: negative count");
+            if (J.kind == PMX_JOB_LINES && (Cg.n_grad || Cg.n_div || Cg.n_curl))
+                return bfail(b, PMX_ERR_ARGUMENT,
+                             tag + "LineSamples_interpolation: catalog must hold scalars only (InterpolationCatalog{3,N,0,0,0,N})");
+            if (out_columns(Cg) < 1) return bfail(b, PMX_ERR_ARGUMENT, tag + "catalog: no quantities");
+            const int nc = J.particles.ncol;
+            auto bad = [nc](const int* v, int n, bool rho_ok) {
+                if (n > 0 && !v) return true;
+                for (int k = 0; k < n; ++k)
+                    if (!((v[k] >= 0 && v[k] < nc) || (rho_ok && v[k] == -1))) return true;
+                return false;
+            };
+            if (bad(Cg.scalar_slots, Cg.n_scalar, false))
+                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: scalar slot out of range");
+            if (bad(Cg.grad_slots, Cg.n_grad, true))
+                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: gradient slot out of range");
+            if (bad(Cg.div_slots, 3 * Cg.n_div, false))
+                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: divergence slot out of range");
+            if (bad(Cg.curl_slots, 3 * Cg.n_curl, false))
+                return bfail(b, PMX_ERR_DIMENSION, tag + "catalog: curl slot out of range");
+            const pmx_output& O = J.out;
+            if (O.dtype != PMX_F32 || O.mem != PMX_MEM_DEVICE || !O.cols)
+                return bfail(b, PMX_ERR_ARGUMENT, tag + "outputs must be FP32 device columns");
+            if (J.kind == PMX_JOB_POINTS && (J.points.mem != PMX_MEM_DEVICE || J.points.n < 1))
+                return bfail(b, PMX_ERR_ARGUMENT, tag + "points must be device memory, n >= 1");
+            if (J.kind == PMX_JOB_LINES && (J.lines.mem != PMX_MEM_DEVICE || J.lines.n < 1))
+                return bfail(b, PMX_ERR_ARGUMENT, tag + "lines must be device memory, n >= 1");
         }
-        PMXB_HIP(b, hipStreamSynchronize(c->stream));
-    }
-    if (ensure(b->ctx[0], b->counter, 64) != PMX_OK) return bfail(b, PMX_ERR_NOMEM, "batch: counter allocation failed");
-    if (const char* v = std::getenv("PMX_SPLIT_BUDGET")) b->split_budget = std::max(0, std::atoi(v));
-    if (const char* v = std::getenv("PMX_SPLIT_ITEM")) b->split_item = std::max(64, std::atoi(v));
-    PMX_TRY(make_split(b));
-    PMX_TRY(make_units(b));
-    b->info[3] = (double)b->total_tiles;
-    return PMX_OK;
+        b->jobs.assign(jobs, jobs + njobs);
+        b->qperm.assign(njobs, nullptr);
+        b->plan.resize(njobs);
+        b->lines_o.assign(njobs, nullptr);
+        b->out_cols.resize(njobs);
+        for (int j = 0; j < njobs; ++j) {
+            pmx_batch_job& J = b->jobs[j];
+            b->out_cols[j].assign(J.out.cols, J.out.cols + out_columns(J.catalog));
+            pmx_ctx* c = nullptr;
+            const int rc = pmx_create(&c, device_ordinal);
+            if (rc != PMX_OK) return bfail(b, rc, "batch: context creation failed");
+            b->ctx.push_back(c);
+            // plan: a full (synchronous) build sizes every buffer the captured builds reuse
+            int r = build_lbvh(c, &J.particles, 0, nullptr, nullptr);
+            if (r != PMX_OK) return ctx_fail(b, c, r, j);
+            const int64_t nP = J.kind == PMX_JOB_POINTS ? J.points.n : J.lines.n;
+            if (ensure(c, c->outbuf, 4 * (size_t)nP * out_columns(J.catalog)) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
+            if (J.kind == PMX_JOB_POINTS) {
+                int32_t* perm = nullptr;
+                r = sort_queries(c, J.points.x, J.points.y, J.points.z, nP, &perm);
+                if (r != PMX_OK) return ctx_fail(b, c, r, j);
+                b->qperm[j] = perm;
+            } else {
+                if (ensure(c, c->qbuf, 48 * (size_t)nP) != PMX_OK) return ctx_fail(b, c, PMX_ERR_NOMEM, j);
+                double* q = c->qbuf.as<double>();
+                const double* src[6] = {J.lines.ox, J.lines.oy, J.lines.oz, J.lines.dx, J.lines.dy, J.lines.dz};
+                for (int k = 0; k < 6; ++k)
+                    PMXB_HIP(b, hipMemcpyAsync(q + k * nP, src[k], 8 * (size_t)nP, hipMemcpyDeviceToDevice, c->stream));
+                bool ok = false;
+                r = lines_plan(c, q, q + 3 * nP, nP, &b->plan[j], &ok);
+                if (r != PMX_OK) return ctx_fail(b, c, r, j);
+                if (!ok) return bfail(b, PMX_ERR_ARGUMENT, "batch job " + std::to_string(j) +
+                                                               ": lines must be one parallel unit-direction set");
+                b->lines_o[j] = q;
+            }
+            PMXB_HIP(b, hipStreamSynchronize(c->stream));
+        }
+        if (ensure(b->ctx[0], b->counter, 64) != PMX_OK) return bfail(b, PMX_ERR_NOMEM, "batch: counter allocation failed");
+        if (const char* v = std::getenv("PMX_SPLIT_BUDGET")) b->split_budget = std::max(0, std::atoi(v));
+        if (const char* v = std::getenv("PMX_SPLIT_ITEM")) b->split_item = std::max(64, std::atoi(v));
+        PMX_TRY(make_split(b));
+        PMX_TRY(make_units(b));
+        b->info[3] = (double)b->total_tiles;
+        return PMX_OK;
+    });
 }
 
 int pmx_batch_interpolate(pmx_batch* b) {
-    if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
-    PMXB_HIP(b, hipSetDevice(b->device));
-    double ms = 0;
-    PMX_TRY(time_interpolation(b, &ms));
-    b->info[2] = ms;
-    return PMX_OK;
+    return guard(b, [&]() -> int {
+        if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
+        PMXB_HIP(b, hipSetDevice(b->device));
+        double ms = 0;
+        PMX_TRY(time_interpolation(b, &ms));
+        b->info[2] = ms;
+        return PMX_OK;
+    });
 }
 
 int pmx_batch_run(pmx_batch* b, int32_t use_graph) {
-    if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
-    PMXB_HIP(b, hipSetDevice(b->device));
-    hipStream_t s = b->stream;
-    if (use_graph) {
-        if (!b->exec) {
-            PMXB_HIP(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_builds(b);
-            if (rc == PMX_OK) rc = enqueue_interpolation(b);
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(s, &g);
-            if (rc != PMX_OK) {
-                if (g) hipGraphDestroy(g);
-                return rc;
+    return guard(b, [&]() -> int {
+        if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
+        PMXB_HIP(b, hipSetDevice(b->device));
+        hipStream_t s = b->stream;
+        if (use_graph) {
+            if (!b->exec) {
+                PMXB_HIP(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_builds(b);
+                if (rc == PMX_OK) rc = enqueue_interpolation(b);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rc != PMX_OK) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) return bfail(b, PMX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+                b->graph = g;
+                PMXB_HIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             }
-            if (ec != hipSuccess) return bfail(b, PMX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-            b->graph = g;
-            PMXB_HIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            PMXB_HIP(b, hipEventRecord(b->ev[0], s));
+            PMXB_HIP(b, hipGraphLaunch(b->exec, s));
+            PMXB_HIP(b, hipEventRecord(b->ev[1], s));
+        } else {
+            PMXB_HIP(b, hipEventRecord(b->ev[0], s));
+            PMX_TRY(enqueue_builds(b));
+            PMXB_HIP(b, hipEventRecord(b->ev[2], s));
+            PMX_TRY(enqueue_interpolation(b));
+            PMXB_HIP(b, hipEventRecord(b->ev[1], s));
         }
-        PMXB_HIP(b, hipEventRecord(b->ev[0], s));
-        PMXB_HIP(b, hipGraphLaunch(b->exec, s));
-        PMXB_HIP(b, hipEventRecord(b->ev[1], s));
-    } else {
-        PMXB_HIP(b, hipEventRecord(b->ev[0], s));
-        PMX_TRY(enqueue_builds(b));
-        PMXB_HIP(b, hipEventRecord(b->ev[2], s));
-        PMX_TRY(enqueue_interpolation(b));
-        PMXB_HIP(b, hipEventRecord(b->ev[1], s));
-    }
-    PMXB_HIP(b, hipStreamSynchronize(s));
-    float t = 0;
-    PMXB_HIP(b, hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
-    b->info[0] = t;
-    if (!use_graph) {
-        float tb = 0, ti = 0;
-        PMXB_HIP(b, hipEventElapsedTime(&tb, b->ev[0], b->ev[2]));
-        PMXB_HIP(b, hipEventElapsedTime(&ti, b->ev[2], b->ev[1]));
-        b->info[1] = tb;
-        b->info[2] = ti;
-    }
-    return check_nans(b);
+        PMXB_HIP(b, hipStreamSynchronize(s));
+        float t = 0;
+        PMXB_HIP(b, hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+        b->info[0] = t;
+        if (!use_graph) {
+            float tb = 0, ti = 0;
+            PMXB_HIP(b, hipEventElapsedTime(&tb, b->ev[0], b->ev[2]));
+            PMXB_HIP(b, hipEventElapsedTime(&ti, b->ev[2], b->ev[1]));
+            b->info[1] = tb;
+            b->info[2] = ti;
+        }
+        return check_nans(b);
+    });
 }
 
 int pmx_batch_tune(pmx_batch* b) {
-    if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
-    PMXB_HIP(b, hipSetDevice(b->device));
-    // coordinate descent over the launch shape, each candidate timed once after a warm-up
-    struct Knob {
-        int* v;
-        std::vector<int> cand;
-    };
-    Knob knobs[] = {{&b->waves_per_cu, {4, 8, 16, 32}},
-                    {&b->bucket_lines, {128, 256, 512, 1024}},
-                    {&b->bucket_points, {8, 16, 32, 64}},
-                    {&b->heavy_first, {0, 1}},
-                    {&b->split_budget, {2048, 8192, 32768}}};
-    double best_ms = 0;
-    PMX_TRY(make_units(b));
-    PMX_TRY(time_interpolation(b, &best_ms));
-    PMX_TRY(time_interpolation(b, &best_ms));
-    for (Knob& k : knobs) {
-        const int keep = *k.v;
-        int best = keep;
-        for (int cand : k.cand) {
-            if (cand == keep) continue;
-            *k.v = cand;
-            PMX_TRY(make_units(b));
-            double ms = 0;
-            PMX_TRY(time_interpolation(b, &ms));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = cand;
+    return guard(b, [&]() -> int {
+        if (!b || b->ctx.empty()) return PMX_ERR_ARGUMENT;
+        PMXB_HIP(b, hipSetDevice(b->device));
+        // coordinate descent over the launch shape, each candidate timed once after a warm-up
+        struct Knob {
+            int* v;
+            std::vector<int> cand;
+        };
+        Knob knobs[] = {{&b->waves_per_cu, {4, 8, 16, 32}},
+                        {&b->bucket_lines, {128, 256, 512, 1024}},
+                        {&b->bucket_points, {8, 16, 32, 64}},
+                        {&b->heavy_first, {0, 1}},
+                        {&b->split_budget, {2048, 8192, 32768}}};
+        double best_ms = 0;
+        PMX_TRY(make_units(b));
+        PMX_TRY(time_interpolation(b, &best_ms));
+        PMX_TRY(time_interpolation(b, &best_ms));
+        for (Knob& k : knobs) {
+            const int keep = *k.v;
+            int best = keep;
+            for (int cand : k.cand) {
+                if (cand == keep) continue;
+                *k.v = cand;
+                PMX_TRY(make_units(b));
+                double ms = 0;
+                PMX_TRY(time_interpolation(b, &ms));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = cand;
+                }
             }
+            *k.v = best;
         }
-        *k.v = best;
-    }
-    PMX_TRY(make_units(b));
-    drop_graph(b);   // the captured graph holds the old launch shape
-    b->info[2] = best_ms;
-    return PMX_OK;
+        PMX_TRY(make_units(b));
+        drop_graph(b);   // the captured graph holds the old launch shape
+        b->info[2] = best_ms;
+        return PMX_OK;
+    });
 }
 
 int pmx_batch_info(const pmx_batch* b, double* vals, int32_t n) {
-    if (!b || !vals) return PMX_ERR_ARGUMENT;
-    const double all[9] = {b->info[0], b->info[1], b->info[2], b->info[3], (double)b->waves_per_cu,
-                           (double)b->bucket_points, (double)b->bucket_lines, (double)b->heavy_first,
-                           (double)b->split_budget};
-    for (int i = 0; i < n && i < 9; ++i) vals[i] = all[i];
-    return PMX_OK;
+    return guard(b, [&]() -> int {
+        if (!b || !vals) return PMX_ERR_ARGUMENT;
+        const double all[9] = {b->info[0], b->info[1], b->info[2], b->info[3], (double)b->waves_per_cu,
+                               (double)b->bucket_points, (double)b->bucket_lines, (double)b->heavy_first,
+                               (double)b->split_budget};
+        for (int i = 0; i < n && i < 9; ++i) vals[i] = all[i];
+        return PMX_OK;
+    });
 }
 
 const char* pmx_batch_last_error(const pmx_batch* b) { return b ? b->err.c_str() : "null batch"; }

@@ -864,10 +864,13 @@ int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* 
 }  // namespace
 
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
-                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt) {
+                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist) {
+    static_assert(kTile == 4096 && kBins == 256, "radix_hist_bytes assumes 4096-key tiles of 8-bit digits");
     const int ntiles = (int)((n + kTile - 1) / kTile);
-    PMX_TRY(ensure(c, c->hist, sizeof(uint32_t) * ((size_t)kBins * ntiles + kBins)));
-    uint32_t* hist = c->hist.as<uint32_t>();
+    if (!hist) {
+        PMX_TRY(ensure(c, c->hist, radix_hist_bytes(n)));
+        hist = c->hist.as<uint32_t>();
+    }
     uint32_t* totals = hist + (size_t)kBins * ntiles;
     uint64_t *kin = keys, *kout = keys_alt;
     int32_t *vin = vals, *vout = vals_alt;
@@ -888,7 +891,6 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
 
 // Refit: short-range nodes directly, then kLaunchRounds grid-wide rounds (the long lists
 // near the leaves), then the tail kernel for the rest; no host synchronisation.
-int g_refit_small = 32;
 
 // fp64: also write the FP64 nodes (otherwise only the FP32 tree of the compensated path).
 int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
@@ -912,7 +914,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     R.n32 = c->nodes32.as<Node32>();
     const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
     hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.lr, R.escape, R.rlo, R.rhi, n,
-                       R.visited, R.nodes, R.n32, std::min(g_refit_small, kRefitPad), listA, &counts[0]);
+                       R.visited, R.nodes, R.n32, std::min(c->tune.refit_small, kRefitPad), listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
         int32_t* cur = (round & 1) ? listB : listA;

@@ -3,11 +3,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/partia_mi355x.h"
 #include "pmx_device.h"
+
+struct pmx_phantom;   // Phantom dump reader (pmx_phantom.hip, include/partia_phantom.h)
 
 namespace pmx {
 
@@ -34,6 +38,22 @@ struct __align__(16) Node32 {
     int32_t lo;       // first leaf (0-based sorted index) of the node's range
     int32_t cnt;      // number of leaves in the range
     int32_t pad;
+};
+
+// Launch-shape knobs of a context, read from the environment when it is created:
+// PMX_LINES_TILED=0 sends every line set through the union walk; PMX_BUCKET_POINTS /
+// PMX_BUCKET_LINES set the leaf-run size the sample-tile walks hand over at once (1..64);
+// PMX_TILED_BUCKET the leaf-run size of the line-map tile walk (1..65536); PMX_TILED_NEAR how
+// close (in cells) a node's box must be to the tile for the walk to read it whole;
+// PMX_REFIT_SMALL the leaf-run length up to which the refit computes a node directly from its
+// leaves.
+struct Tuning {
+    int lines_tiled = 1;
+    int bucket_points = 32;
+    int bucket_lines = 64;
+    int tiled_bucket = 1024;
+    int tiled_near = 16;
+    int refit_small = 32;
 };
 
 struct Tree {
@@ -70,6 +90,7 @@ struct pmx_ctx {
     pmx::DevBuf visited, adj, range_hi, range_lo;
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
     int ncu = 256;             // compute units of the device
+    pmx::Tuning tune;          // launch-shape knobs (environment at creation)
     bool line_inv_h2 = false;  // the current line call weights by Inorm / h^2 (PMX_LINE_INV_H2)
     int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
@@ -84,6 +105,42 @@ struct pmx_ctx {
 namespace pmx {
 
 int set_error(pmx_ctx* c, int code, const std::string& msg);
+// The same for a plain text: copied into the context's reserved error buffer (no allocation for
+// texts that fit it).
+int set_error(pmx_ctx* c, int code, const char* msg);
+
+// No C++ exception crosses the C ABI: every int-returning entry point runs its body through
+// guard(owner, body).  std::bad_alloc becomes PMX_ERR_NOMEM, any other exception
+// PMX_ERR_INTERNAL, and the message goes to the owner's last-error text (a context, a batch, a
+// Phantom reader, the open-error string, or nothing).  The environment variable PMX_TEST_THROW
+// ("bad_alloc" or "exception") makes the guard throw before the body runs: the tests of this
+// conversion use it.
+void note_exception(pmx_ctx* c, const char* msg) noexcept;
+void note_exception(const pmx_ctx* c, const char* msg) noexcept;
+void note_exception(pmx_batch* b, const char* msg) noexcept;
+void note_exception(const pmx_batch* b, const char* msg) noexcept;
+void note_exception(pmx_phantom* ph, const char* msg) noexcept;
+void note_exception(const pmx_phantom* ph, const char* msg) noexcept;
+void note_exception(std::string* s, const char* msg) noexcept;
+inline void note_exception(std::nullptr_t, const char*) noexcept {}
+void throw_test_fault();
+
+template <class Owner, class Body>
+int guard(Owner owner, Body&& body) noexcept {
+    try {
+        throw_test_fault();
+        return body();
+    } catch (const std::bad_alloc&) {
+        note_exception(owner, "host memory allocation failed (std::bad_alloc)");
+        return PMX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        note_exception(owner, e.what());
+        return PMX_ERR_INTERNAL;
+    } catch (...) {
+        note_exception(owner, "unknown C++ exception");
+        return PMX_ERR_INTERNAL;
+    }
+}
 int hip_fail(pmx_ctx* c, hipError_t e, const char* what);
 int ensure(pmx_ctx* c, DevBuf& b, size_t bytes);
 
@@ -114,8 +171,28 @@ constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
                int64_t* order_out);
+// hist: the histogram buffer, 4 * 256 * (ceil(n / 4096) + 1) bytes (null: the context's).
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
-                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt);
+                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist = nullptr);
+inline size_t radix_hist_bytes(int64_t n) { return sizeof(uint32_t) * 256 * (size_t)((n + 4095) / 4096 + 1); }
+
+// Device work memory of one line-map call.  The per-call entry points take it from the
+// context's grow-only buffers (ws == nullptr); the no-allocation entry point passes regions of
+// its static scratch array instead.
+constexpr int kBinBlocks = 1024;                                        // blocks of the line projection
+constexpr size_t kBinPartBytes = sizeof(double) * 5 * kBinBlocks + 64;  // bounds, flag, tile counter
+struct LineWork {
+    double* st = nullptr;                  // (s, t) of each line origin: 2 nl doubles
+    double* part = nullptr;                // kBinPartBytes
+    uint64_t* keys0 = nullptr;             // sort pairs, nl each
+    int32_t* vals0 = nullptr;
+    uint64_t* keys1 = nullptr;
+    int32_t* vals1 = nullptr;
+    uint32_t* hist = nullptr;              // radix_hist_bytes(nl)
+    char* cells = nullptr;                 // cell starts (ncell + 1 int32, 16-aligned) then sorted (s, t)
+    size_t cells_bytes = 0;
+};
+
 
 // interpolation (pmx_interp64.hip / pmx_interp32.hip)
 struct CatChunk {
@@ -152,7 +229,8 @@ int traversal_stats_f64(pmx_ctx* c, int kind, int strategy, const double* a, con
 
 // Morton order of the samples (FP32-compensated path tiles); the permutation lives in the
 // context's scratch buffers until the next call that uses them.
-int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm);
+int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm,
+                 const LineWork* ws = nullptr);
 // Leaf64 records of the current tree from cols64 (the plain FP64 path packs them on first use).
 int ensure_leaf64(pmx_ctx* c);
 int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm,
@@ -162,10 +240,8 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
 int prepare_f32(pmx_ctx* c);
 // Parallel unit-direction line sets: particle-centric tiled path.  *used is false (and
 // nothing is written) when the lines do not qualify; the caller then uses the union walk.
-extern int g_lines_tiled;
-extern int g_bucket_points, g_bucket_lines, g_tiled_bucket, g_tiled_near, g_refit_small;
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
-                       int64_t nl, float* out, int64_t ldo, bool* used);
+                       int64_t nl, float* out, int64_t ldo, bool* used, const LineWork* ws = nullptr);
 int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, uint64_t* digest, bool* used);
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx);
 int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const double* qy, const double* qz,

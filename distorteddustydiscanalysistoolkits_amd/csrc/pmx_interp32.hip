@@ -32,8 +32,6 @@
 
 namespace pmx {
 
-int g_bucket_points = 32;
-int g_bucket_lines = 64;
 
 namespace {
 
@@ -427,7 +425,7 @@ void launch_points32_s(pmx_ctx* c, const PointCat& cat, const double* qx, const 
                        const int32_t* perm, int64_t npts, float* out, int64_t ldo,
                        const PointSplit& sp) {
     const dim3 g(tile_grid(npts)), b(kWave);
-    T32 T = tree32(c, g_bucket_points);
+    T32 T = tree32(c, c->tune.bucket_points);
     const bool split = S != kScatter && sp.budget > 0;
     const size_t lds = DERIV ? deriv_tile_lds(cat.nder, cat.nacc) : 0;
     if (split) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, sp.counters, 4);
@@ -464,7 +462,7 @@ int point_split(pmx_ctx* c, int64_t npts, int strategy, PointSplit* sp) {
     if (strategy == kScatter || c->split_budget <= 0) return PMX_OK;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
     sp->budget = c->split_budget;
-    sp->item_max = std::max(c->split_item, 2 * g_bucket_points);
+    sp->item_max = std::max(c->split_item, 2 * c->tune.bucket_points);
     sp->cap_items = kSplitItems;
     sp->cap_slots = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, kSplitSlots));
     const size_t b_cnt = 256, b_items = 8 * (size_t)sp->cap_items, b_tile = 4 * (size_t)sp->cap_slots,
@@ -484,7 +482,7 @@ template <int K>
 void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const double* d, const int32_t* perm,
                     int64_t nl, float* out, int64_t ldo) {
     auto kern = c->line_inv_h2 ? k_lines32<K, true> : k_lines32<K, false>;
-    hipLaunchKernelGGL(kern, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines), cat, o, d,
+    hipLaunchKernelGGL(kern, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines), cat, o, d,
                        perm, nl, c->lut64.as<double>() + (size_t)K * kLutN, c->lut_dq[K], out, ldo);
 }
 
@@ -508,18 +506,25 @@ T32 tree32(pmx_ctx* c, int bucket) {
 }
 
 // Morton order of the samples (queries): permutation in c->scratch_a or c->scratch_b
-int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm) {
-    PMX_TRY(ensure(c, c->scratch_a, 8 * (size_t)npts + 4 * (size_t)npts + 64));
-    PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)npts + 4 * (size_t)npts + 64));
-    uint64_t* k0 = c->scratch_a.as<uint64_t>();
-    int32_t* v0 = reinterpret_cast<int32_t*>(k0 + npts);
-    uint64_t* k1 = c->scratch_b.as<uint64_t>();
-    int32_t* v1 = reinterpret_cast<int32_t*>(k1 + npts);
+int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, int64_t npts, int32_t** perm,
+                 const LineWork* ws) {
+    uint64_t *k0, *k1;
+    int32_t *v0, *v1;
+    if (ws) {
+        k0 = ws->keys0; v0 = ws->vals0; k1 = ws->keys1; v1 = ws->vals1;
+    } else {
+        PMX_TRY(ensure(c, c->scratch_a, 8 * (size_t)npts + 4 * (size_t)npts + 64));
+        PMX_TRY(ensure(c, c->scratch_b, 8 * (size_t)npts + 4 * (size_t)npts + 64));
+        k0 = c->scratch_a.as<uint64_t>();
+        v0 = reinterpret_cast<int32_t*>(k0 + npts);
+        k1 = c->scratch_b.as<uint64_t>();
+        v1 = reinterpret_cast<int32_t*>(k1 + npts);
+    }
     const dim3 g(grid_for(npts, 256, 8192)), b(256);
     if (c->tree.n >= 2) {
         hipLaunchKernelGGL(k_query_keys, g, b, 0, c->stream, x, y, z, 0, npts, c->nodes64.as<Node64>(), k0, v0);
         bool alt = false;
-        PMX_TRY(radix_sort_pairs(c, k0, v0, npts, k1, v1, 0, 32, &alt));
+        PMX_TRY(radix_sort_pairs(c, k0, v0, npts, k1, v1, 0, 32, &alt, ws ? ws->hist : nullptr));
         *perm = alt ? v1 : v0;
     } else {
         hipLaunchKernelGGL(k_iota, g, b, 0, c->stream, k0, v0, npts);
@@ -626,7 +631,7 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
 
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx) {
     PMX_TRY(prepare_f32(c));
-    hipLaunchKernelGGL(k_nearest32, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_points), qx, qy,
+    hipLaunchKernelGGL(k_nearest32, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_points), qx, qy,
                        qz, npts, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
@@ -637,7 +642,7 @@ int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const doub
     PMX_TRY(prepare_f32(c));
     const dim3 g(tile_grid(npts)), b(kWave);
     const double Kv = support(c->tree.kernel);
-    T32 T = tree32(c, g_bucket_points);
+    T32 T = tree32(c, c->tune.bucket_points);
     if (strategy == kGather)
         hipLaunchKernelGGL(k_points32_nb<kGather>, g, b, 0, c->stream, T, Kv, qx, qy, qz, ha, npts, counts, offsets, idx);
     else if (strategy == kScatter)
@@ -652,7 +657,7 @@ int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const doub
 int neighbors_lines_f32c(pmx_ctx* c, const double* o, const double* d, int64_t nl, int64_t* counts,
                          const int64_t* offsets, int32_t* idx) {
     PMX_TRY(prepare_f32(c));
-    hipLaunchKernelGGL(k_lines32_nb, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, g_bucket_lines),
+    hipLaunchKernelGGL(k_lines32_nb, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines),
                        support(c->tree.kernel), o, d, nl, counts, offsets, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;

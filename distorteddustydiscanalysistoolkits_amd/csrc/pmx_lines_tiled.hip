@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "pmx_internal.h"
@@ -100,6 +101,11 @@ __global__ void __launch_bounds__(256) k_line_cells(const uint64_t* __restrict__
     }
 }
 
+// Clears counters and flags (a kernel rather than a memset node, so callers may capture it).
+__global__ void k_zero_counter(unsigned* __restrict__ p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
 // Workgroups of kTileWaves wavefronts share one LDS copy of the Inorm table: its reads are
 // per-lane gathers over 16 KB, which the vector L1 serves one cache line at a time while LDS
 // serves them in a few cycles.  Each wavefront takes tiles from a device counter, so the
@@ -147,8 +153,8 @@ int launch_tiled(pmx_ctx* c, const Frame& f, const TileCat& cat, const int32_t* 
     const int64_t resident = (int64_t)std::max(1, per_cu) * c->ncu;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(resident, (ntiles + kTileWaves - 1) / kTileWaves));
     T32 T = tree32(c, 64);
-    T.bucket = g_tiled_bucket;
-    PMX_HIP(c, hipMemsetAsync(next_tile, 0, sizeof(unsigned), c->stream));
+    T.bucket = c->tune.tiled_bucket;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, c->stream, next_tile, 1);
     hipLaunchKernelGGL((k_lines_tiled<K, MODE, H2>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat, cells,
                        perm, sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out, ldo,
                        digest, next_tile);
@@ -180,7 +186,8 @@ struct Binned {
     unsigned* next_tile;       // tile counter of the line-map kernel
 };
 
-int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok, Binned* out) {
+int bin_lines(pmx_ctx* c, const LineWork* ws, const double* o, const double* d, int64_t nl, bool* ok,
+              Binned* out) {
     *ok = false;
     if (nl < 1 || nl > (int64_t)INT32_MAX - kCap || c->tree.n < 1) return PMX_OK;
     // direction of line 0 and the particle box (root node, or the single leaf)
@@ -214,17 +221,26 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
         f.v[2] = d0[0] * f.u[1] - d0[1] * f.u[0];
     }
 
-    const int blocks = (int)std::min<int64_t>(1024, (nl + 255) / 256);
-    PMX_TRY(ensure(c, c->tl_st, sizeof(double) * 2 * (size_t)nl + 64));
-    PMX_TRY(ensure(c, c->tl_part, sizeof(double) * 5 * (size_t)blocks + sizeof(int) * 4));
-    int* flag = reinterpret_cast<int*>(c->tl_part.as<double>() + 5 * blocks);
-    PMX_HIP(c, hipMemsetAsync(flag, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_line_project, dim3(blocks), dim3(256), 0, c->stream, o, d, nl, f, c->tl_st.as<double>(),
-                       c->tl_part.as<double>(), flag);
+    const int blocks = (int)std::min<int64_t>(kBinBlocks, (nl + 255) / 256);
+    double* st;
+    double* partd;
+    if (ws) {
+        st = ws->st;
+        partd = ws->part;
+    } else {
+        PMX_TRY(ensure(c, c->tl_st, sizeof(double) * 2 * (size_t)nl + 64));
+        PMX_TRY(ensure(c, c->tl_part, kBinPartBytes));
+        st = c->tl_st.as<double>();
+        partd = c->tl_part.as<double>();
+    }
+    // after the partial bounds: the not-parallel flag, then the line-map kernel's tile counter
+    int* flag = reinterpret_cast<int*>(partd + 5 * kBinBlocks);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<unsigned*>(flag), 2);
+    hipLaunchKernelGGL(k_line_project, dim3(blocks), dim3(256), 0, c->stream, o, d, nl, f, st, partd, flag);
     PMX_HIP(c, hipGetLastError());
-    std::vector<double> part(5 * (size_t)blocks);
+    double part[5 * kBinBlocks];
     int hflag = 0;
-    PMX_HIP(c, hipMemcpyAsync(part.data(), c->tl_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP(c, hipMemcpyAsync(part, partd, sizeof(double) * 5 * (size_t)blocks, hipMemcpyDeviceToHost, c->stream));
     PMX_HIP(c, hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     PMX_HIP(c, hipStreamSynchronize(c->stream));
     if (hflag) return PMX_OK;
@@ -270,7 +286,7 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     f.cs = gx > 1 ? es / (double)(gx - 1) : std::max(es, 1.0);
     f.ct = gy > 1 ? et / (double)(gy - 1) : std::max(et, 1.0);
     if (!(f.cs > 0.0) || !(f.ct > 0.0)) return PMX_OK;
-    f.near = (double)g_tiled_near;
+    f.near = (double)c->tune.tiled_near;
     f.inv_cs = 1.0 / f.cs;
     f.inv_ct = 1.0 / f.ct;
     f.so = smin - 0.5 * f.cs;
@@ -283,22 +299,41 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
     int bits = 1;
     while (bits < 62 && ((int64_t)1 << bits) < ncell) ++bits;
 
-    PMX_TRY(ensure(c, c->scratch_a, 12 * (size_t)nl + 64));
-    PMX_TRY(ensure(c, c->scratch_b, 12 * (size_t)nl + 64));
-    PMX_TRY(ensure(c, c->tl_cells, sizeof(int32_t) * (size_t)(ncell + 1) + sizeof(double2) * (size_t)nl + 64));
-    uint64_t* k0 = c->scratch_a.as<uint64_t>();
-    int32_t* v0 = reinterpret_cast<int32_t*>(k0 + nl);
-    uint64_t* k1 = c->scratch_b.as<uint64_t>();
-    int32_t* v1 = reinterpret_cast<int32_t*>(k1 + nl);
+    const size_t cells_bytes = ((sizeof(int32_t) * (size_t)(ncell + 1) + 15) & ~(size_t)15) + sizeof(double2) * (size_t)nl;
+    uint64_t *k0, *k1;
+    int32_t *v0, *v1;
+    uint32_t* hist = nullptr;
+    char* cbuf;
+    if (ws) {
+        if (cells_bytes > ws->cells_bytes) {
+            char msg[192];
+            snprintf(msg, sizeof(msg),
+                     "line map: the line set needs %lld binning cells, beyond the scratch capacity (lines far from a "
+                     "square lattice)", (long long)ncell);
+            return set_error(c, PMX_ERR_CAPACITY, msg);
+        }
+        k0 = ws->keys0; v0 = ws->vals0; k1 = ws->keys1; v1 = ws->vals1;
+        hist = ws->hist;
+        cbuf = ws->cells;
+    } else {
+        PMX_TRY(ensure(c, c->scratch_a, 12 * (size_t)nl + 64));
+        PMX_TRY(ensure(c, c->scratch_b, 12 * (size_t)nl + 64));
+        PMX_TRY(ensure(c, c->tl_cells, cells_bytes + 64));
+        k0 = c->scratch_a.as<uint64_t>();
+        v0 = reinterpret_cast<int32_t*>(k0 + nl);
+        k1 = c->scratch_b.as<uint64_t>();
+        v1 = reinterpret_cast<int32_t*>(k1 + nl);
+        cbuf = c->tl_cells.as<char>();
+    }
     const dim3 g(grid_for(nl, 256, 8192)), b(256);
-    hipLaunchKernelGGL(k_line_keys, g, b, 0, c->stream, c->tl_st.as<double>(), nl, f, k0, v0);
+    hipLaunchKernelGGL(k_line_keys, g, b, 0, c->stream, st, nl, f, k0, v0);
     bool alt = false;
-    PMX_TRY(radix_sort_pairs(c, k0, v0, nl, k1, v1, 0, bits, &alt));
+    PMX_TRY(radix_sort_pairs(c, k0, v0, nl, k1, v1, 0, bits, &alt, hist));
     const uint64_t* ks = alt ? k1 : k0;
     const int32_t* perm = alt ? v1 : v0;
-    int32_t* cells = c->tl_cells.as<int32_t>();
-    double2* sts = reinterpret_cast<double2*>(c->tl_cells.as<char>() + ((sizeof(int32_t) * (size_t)(ncell + 1) + 15) & ~(size_t)15));
-    hipLaunchKernelGGL(k_line_cells, g, b, 0, c->stream, ks, perm, c->tl_st.as<double>(), nl, ncell, cells, sts);
+    int32_t* cells = reinterpret_cast<int32_t*>(cbuf);
+    double2* sts = reinterpret_cast<double2*>(cbuf + ((sizeof(int32_t) * (size_t)(ncell + 1) + 15) & ~(size_t)15));
+    hipLaunchKernelGGL(k_line_cells, g, b, 0, c->stream, ks, perm, st, nl, ncell, cells, sts);
     PMX_HIP(c, hipGetLastError());
     out->f = f;
     out->cells = cells;
@@ -311,17 +346,14 @@ int bin_lines(pmx_ctx* c, const double* o, const double* d, int64_t nl, bool* ok
 
 }  // namespace
 
-int g_lines_tiled = 1;
-int g_tiled_bucket = 1024;
-int g_tiled_near = 16;
 
 int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols, const double* o, const double* d,
-                       int64_t nl, float* out, int64_t ldo, bool* used) {
+                       int64_t nl, float* out, int64_t ldo, bool* used, const LineWork* ws) {
     *used = false;
-    if (!g_lines_tiled) return PMX_OK;
+    if (!c->tune.lines_tiled) return PMX_OK;
     bool ok = false;
     Binned B;
-    PMX_TRY(bin_lines(c, o, d, nl, &ok, &B));
+    PMX_TRY(bin_lines(c, ws, o, d, nl, &ok, &B));
     if (!ok) return PMX_OK;
     PMX_HIP(c, hipEventRecord(c->ev[8], c->stream));
     for (int base = 0; base < ncols || (ncols == 0 && base == 0); base += kMaxCols) {
@@ -342,7 +374,7 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
 
 int lines_plan(pmx_ctx* c, const double* o, const double* d, int64_t nl, lmap::LinePlan* plan, bool* ok) {
     Binned B;
-    PMX_TRY(bin_lines(c, o, d, nl, ok, &B));
+    PMX_TRY(bin_lines(c, nullptr, o, d, nl, ok, &B));
     if (*ok) {
         plan->f = B.f;
         plan->cstart = B.cells;
@@ -356,7 +388,7 @@ int line_digest_tiled(pmx_ctx* c, const double* o, const double* d, int64_t nl, 
     *used = false;
     bool ok = false;
     Binned B;
-    PMX_TRY(bin_lines(c, o, d, nl, &ok, &B));
+    PMX_TRY(bin_lines(c, nullptr, o, d, nl, &ok, &B));
     if (!ok) return PMX_OK;
     TileCat cat{};
     PMX_TRY(launch_tiled_k<1>(c, B.f, cat, B.cells, B.perm, B.sts, o, nl, nullptr, 0, digest, B.next_tile));

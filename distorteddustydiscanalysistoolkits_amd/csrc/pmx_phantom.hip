@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../include/partia_phantom.h"
+#include "pmx_internal.h"
 
 namespace {
 
@@ -92,6 +93,19 @@ struct pmx_phantom {
     void* out[2] = {nullptr, nullptr};   // pinned: converted chunk, double-buffered
     size_t raw_bytes = 0, out_bytes = 0;
 };
+
+namespace pmx {
+void note_exception(pmx_phantom* ph, const char* msg) noexcept {
+    if (!ph) return;
+    try {
+        ph->err = msg;
+    } catch (...) {
+    }
+}
+void note_exception(const pmx_phantom* ph, const char* msg) noexcept {
+    note_exception(const_cast<pmx_phantom*>(ph), msg);
+}
+}  // namespace pmx
 
 namespace {
 
@@ -380,43 +394,47 @@ void density_host(const T* h, int64_t n, T c, int dim, T* rho) {
 
 }  // namespace
 
+using pmx::guard;
+
 extern "C" {
 
 int pmx_phantom_open(pmx_phantom** out, const char* path) {
-    if (!out) return PMX_ERR_ARGUMENT;
-    *out = nullptr;
-    g_open_error.clear();
-    if (!path) {
-        g_open_error = "path is NULL";
-        return PMX_ERR_ARGUMENT;
-    }
-    const int fd = ::open(path, O_RDONLY | O_CLOEXEC);
-    if (fd < 0) {
-        g_open_error = std::string("SystemError: opening file \"") + path + "\": " + std::strerror(errno);
-        return PMX_ERR_IO;
-    }
-    struct stat st;
-    if (::fstat(fd, &st) != 0) {
-        g_open_error = std::string("SystemError: stat \"") + path + "\": " + std::strerror(errno);
-        ::close(fd);
-        return PMX_ERR_IO;
-    }
-    pmx_phantom* ph = new (std::nothrow) pmx_phantom();
-    if (!ph) {
-        ::close(fd);
-        g_open_error = "out of host memory";
-        return PMX_ERR_NOMEM;
-    }
-    ph->fd = fd;
-    ph->size = (int64_t)st.st_size;
-    const int rc = parse(ph);
-    if (rc != PMX_OK) {
-        g_open_error = ph->err;
-        pmx_phantom_close(ph);
-        return rc;
-    }
-    *out = ph;
-    return PMX_OK;
+    return guard(&g_open_error, [&]() -> int {
+        if (!out) return PMX_ERR_ARGUMENT;
+        *out = nullptr;
+        g_open_error.clear();
+        if (!path) {
+            g_open_error = "path is NULL";
+            return PMX_ERR_ARGUMENT;
+        }
+        const int fd = ::open(path, O_RDONLY | O_CLOEXEC);
+        if (fd < 0) {
+            g_open_error = std::string("SystemError: opening file \"") + path + "\": " + std::strerror(errno);
+            return PMX_ERR_IO;
+        }
+        struct stat st;
+        if (::fstat(fd, &st) != 0) {
+            g_open_error = std::string("SystemError: stat \"") + path + "\": " + std::strerror(errno);
+            ::close(fd);
+            return PMX_ERR_IO;
+        }
+        pmx_phantom* ph = new (std::nothrow) pmx_phantom();
+        if (!ph) {
+            ::close(fd);
+            g_open_error = "out of host memory";
+            return PMX_ERR_NOMEM;
+        }
+        ph->fd = fd;
+        ph->size = (int64_t)st.st_size;
+        const int rc = parse(ph);
+        if (rc != PMX_OK) {
+            g_open_error = ph->err;
+            pmx_phantom_close(ph);
+            return rc;
+        }
+        *out = ph;
+        return PMX_OK;
+    });
 }
 
 const char* pmx_phantom_open_error(void) { return g_open_error.c_str(); }
@@ -439,154 +457,168 @@ void pmx_phantom_close(pmx_phantom* ph) {
 const char* pmx_phantom_last_error(const pmx_phantom* ph) { return ph ? ph->err.c_str() : "reader is NULL"; }
 
 int pmx_phantom_info(const pmx_phantom* ph, int64_t* vals, int32_t n) {
-    if (!ph || !vals || n < 0) return PMX_ERR_ARGUMENT;
-    const int64_t v[5] = {ph->int_bytes, ph->real_bytes, (int64_t)ph->block_n.size(), (int64_t)ph->cols.size(),
-                          (int64_t)ph->vars.size()};
-    for (int i = 0; i < n && i < 5; ++i) vals[i] = v[i];
-    return PMX_OK;
+    return guard(ph, [&]() -> int {
+        if (!ph || !vals || n < 0) return PMX_ERR_ARGUMENT;
+        const int64_t v[5] = {ph->int_bytes, ph->real_bytes, (int64_t)ph->block_n.size(), (int64_t)ph->cols.size(),
+                              (int64_t)ph->vars.size()};
+        for (int i = 0; i < n && i < 5; ++i) vals[i] = v[i];
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_file_identifier(const pmx_phantom* ph, char* buf, int32_t cap) {
-    if (!ph || !buf || cap < 1) return PMX_ERR_ARGUMENT;
-    const size_t k = std::min((size_t)cap - 1, ph->ident.size());
-    std::memcpy(buf, ph->ident.data(), k);
-    buf[k] = 0;
-    return PMX_OK;
+    return guard(ph, [&]() -> int {
+        if (!ph || !buf || cap < 1) return PMX_ERR_ARGUMENT;
+        const size_t k = std::min((size_t)cap - 1, ph->ident.size());
+        std::memcpy(buf, ph->ident.data(), k);
+        buf[k] = 0;
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_block_size(const pmx_phantom* ph, int32_t block, int64_t* n) {
-    if (!ph || !n || block < 0 || block >= (int32_t)ph->block_n.size()) return PMX_ERR_ARGUMENT;
-    *n = ph->block_n[block];
-    return PMX_OK;
+    return guard(ph, [&]() -> int {
+        if (!ph || !n || block < 0 || block >= (int32_t)ph->block_n.size()) return PMX_ERR_ARGUMENT;
+        *n = ph->block_n[block];
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_column_info(const pmx_phantom* ph, int32_t col, char* name, int32_t* block, int32_t* dtype) {
-    if (!ph || col < 0 || col >= (int32_t)ph->cols.size()) return PMX_ERR_ARGUMENT;
-    const PhColumn& c = ph->cols[col];
-    if (name) {
-        const size_t k = std::min<size_t>(16, c.name.size());
-        std::memcpy(name, c.name.data(), k);
-        name[k] = 0;
-    }
-    if (block) *block = c.block;
-    if (dtype) *dtype = c.dtype;
-    return PMX_OK;
+    return guard(ph, [&]() -> int {
+        if (!ph || col < 0 || col >= (int32_t)ph->cols.size()) return PMX_ERR_ARGUMENT;
+        const PhColumn& c = ph->cols[col];
+        if (name) {
+            const size_t k = std::min<size_t>(16, c.name.size());
+            std::memcpy(name, c.name.data(), k);
+            name[k] = 0;
+        }
+        if (block) *block = c.block;
+        if (dtype) *dtype = c.dtype;
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_header_var(const pmx_phantom* ph, int32_t i, char* name, int32_t* dtype, double* fval, int64_t* ival) {
-    if (!ph || i < 0 || i >= (int32_t)ph->vars.size()) return PMX_ERR_ARGUMENT;
-    const PhVar& v = ph->vars[i];
-    if (name) {
-        const size_t k = std::min<size_t>(23, v.name.size());
-        std::memcpy(name, v.name.data(), k);
-        name[k] = 0;
-    }
-    if (dtype) *dtype = v.dtype;
-    if (fval) *fval = v.fval;
-    if (ival) *ival = v.ival;
-    return PMX_OK;
+    return guard(ph, [&]() -> int {
+        if (!ph || i < 0 || i >= (int32_t)ph->vars.size()) return PMX_ERR_ARGUMENT;
+        const PhVar& v = ph->vars[i];
+        if (name) {
+            const size_t k = std::min<size_t>(23, v.name.size());
+            std::memcpy(name, v.name.data(), k);
+            name[k] = 0;
+        }
+        if (dtype) *dtype = v.dtype;
+        if (fval) *fval = v.fval;
+        if (ival) *ival = v.ival;
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_read_column(pmx_phantom* ph, int32_t col, int32_t out_dtype, int32_t mem, void* dst,
                             const int64_t* keep, int64_t nkeep) {
-    if (!ph) return PMX_ERR_ARGUMENT;
-    ph->err.clear();
-    if (col < 0 || col >= (int32_t)ph->cols.size()) return fail(ph, PMX_ERR_ARGUMENT, "column index out of range");
-    const PhColumn& c = ph->cols[col];
-    const bool out_ok = out_dtype == PMX_F32 || out_dtype == PMX_F64 || (out_dtype >= PMX_I8 && out_dtype <= PMX_I64);
-    if (!out_ok) return fail(ph, PMX_ERR_ARGUMENT, "unknown output element type");
-    if (is_real(c.dtype) && !is_real(out_dtype))
-        return fail(ph, PMX_ERR_ARGUMENT, "InexactError: a real column cannot be read as an integer type");
-    if (mem != PMX_MEM_HOST && mem != PMX_MEM_DEVICE) return fail(ph, PMX_ERR_ARGUMENT, "mem must be host or device");
-    if (keep) {
-        if (nkeep < 0) return fail(ph, PMX_ERR_ARGUMENT, "nkeep < 0");
-        for (int64_t k = 0; k < nkeep; ++k)
-            if (keep[k] < 0 || keep[k] >= c.n || (k > 0 && keep[k] <= keep[k - 1]))
-                return fail(ph, PMX_ERR_ARGUMENT, "keep must hold strictly increasing rows of the column");
-    }
-    const int64_t rows_out = keep ? nkeep : c.n;
-    if (rows_out > 0 && !dst) return fail(ph, PMX_ERR_ARGUMENT, "destination is NULL");
-    if (rows_out == 0) return PMX_OK;
-    const int ib = dtype_bytes(c.dtype), ob = dtype_bytes(out_dtype);
-    const bool device = mem == PMX_MEM_DEVICE;
-    if (device) {
-        const int rc = ensure_device_state(ph);
-        if (rc != PMX_OK) return rc;
-    }
-    std::vector<char> host_raw;
-    if (!device) host_raw.resize((size_t)std::min<int64_t>(c.n, kChunkRows) * ib);
-    Cursor cur{ph->fd, 0, ph->size};
-    int64_t written = 0, k = 0, chunk = 0;
-    // rows to read: every row, or from the first to the last kept row
-    const int64_t first = keep ? keep[0] : 0, last = keep ? keep[nkeep - 1] + 1 : c.n;
-    for (int64_t r0 = first; r0 < last; r0 += kChunkRows, ++chunk) {
-        const int64_t r1 = std::min(last, r0 + kChunkRows);
-        int64_t k1 = k;
-        if (keep)
-            while (k1 < nkeep && keep[k1] < r1) ++k1;
-        if (keep && k1 == k) continue;
-        // with a sparse keep list, read only the span of this chunk's kept rows
-        const int64_t s0 = keep ? keep[k] : r0, s1 = keep ? keep[k1 - 1] + 1 : r1;
-        // every row in the stored type: the file bytes go straight to their destination
-        const bool direct = !keep && c.dtype == out_dtype;
-        cur.pos = c.offset + s0 * ib;
-        if (!device) {
-            void* raw = direct ? static_cast<char*>(dst) + written * ob : host_raw.data();
-            if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
-            written += direct ? s1 - s0
-                              : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, static_cast<char*>(dst) + written * ob,
-                                              keep, k, k1, s0);
-        } else {
-            const int b = (int)(chunk & 1);
-            PH_HIP(ph, hipEventSynchronize(ph->done[b]));      // the copy that last used out[b] is done
-            void* raw = direct ? ph->out[b] : ph->raw;
-            if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
-            const int64_t m = direct ? s1 - s0 : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, ph->out[b], keep, k, k1, s0);
-            PH_HIP(ph, hipMemcpyAsync(static_cast<char*>(dst) + written * ob, ph->out[b], (size_t)(m * ob),
-                                      hipMemcpyHostToDevice, ph->stream));
-            PH_HIP(ph, hipEventRecord(ph->done[b], ph->stream));
-            written += m;
+    return guard(ph, [&]() -> int {
+        if (!ph) return PMX_ERR_ARGUMENT;
+        ph->err.clear();
+        if (col < 0 || col >= (int32_t)ph->cols.size()) return fail(ph, PMX_ERR_ARGUMENT, "column index out of range");
+        const PhColumn& c = ph->cols[col];
+        const bool out_ok = out_dtype == PMX_F32 || out_dtype == PMX_F64 || (out_dtype >= PMX_I8 && out_dtype <= PMX_I64);
+        if (!out_ok) return fail(ph, PMX_ERR_ARGUMENT, "unknown output element type");
+        if (is_real(c.dtype) && !is_real(out_dtype))
+            return fail(ph, PMX_ERR_ARGUMENT, "InexactError: a real column cannot be read as an integer type");
+        if (mem != PMX_MEM_HOST && mem != PMX_MEM_DEVICE) return fail(ph, PMX_ERR_ARGUMENT, "mem must be host or device");
+        if (keep) {
+            if (nkeep < 0) return fail(ph, PMX_ERR_ARGUMENT, "nkeep < 0");
+            for (int64_t k = 0; k < nkeep; ++k)
+                if (keep[k] < 0 || keep[k] >= c.n || (k > 0 && keep[k] <= keep[k - 1]))
+                    return fail(ph, PMX_ERR_ARGUMENT, "keep must hold strictly increasing rows of the column");
         }
-        k = k1;
-    }
-    if (device) PH_HIP(ph, hipStreamSynchronize(ph->stream));
-    if (written != rows_out) return fail(ph, PMX_ERR_IO, "short read of column " + c.name);
-    return PMX_OK;
+        const int64_t rows_out = keep ? nkeep : c.n;
+        if (rows_out > 0 && !dst) return fail(ph, PMX_ERR_ARGUMENT, "destination is NULL");
+        if (rows_out == 0) return PMX_OK;
+        const int ib = dtype_bytes(c.dtype), ob = dtype_bytes(out_dtype);
+        const bool device = mem == PMX_MEM_DEVICE;
+        if (device) {
+            const int rc = ensure_device_state(ph);
+            if (rc != PMX_OK) return rc;
+        }
+        std::vector<char> host_raw;
+        if (!device) host_raw.resize((size_t)std::min<int64_t>(c.n, kChunkRows) * ib);
+        Cursor cur{ph->fd, 0, ph->size};
+        int64_t written = 0, k = 0, chunk = 0;
+        // rows to read: every row, or from the first to the last kept row
+        const int64_t first = keep ? keep[0] : 0, last = keep ? keep[nkeep - 1] + 1 : c.n;
+        for (int64_t r0 = first; r0 < last; r0 += kChunkRows, ++chunk) {
+            const int64_t r1 = std::min(last, r0 + kChunkRows);
+            int64_t k1 = k;
+            if (keep)
+                while (k1 < nkeep && keep[k1] < r1) ++k1;
+            if (keep && k1 == k) continue;
+            // with a sparse keep list, read only the span of this chunk's kept rows
+            const int64_t s0 = keep ? keep[k] : r0, s1 = keep ? keep[k1 - 1] + 1 : r1;
+            // every row in the stored type: the file bytes go straight to their destination
+            const bool direct = !keep && c.dtype == out_dtype;
+            cur.pos = c.offset + s0 * ib;
+            if (!device) {
+                void* raw = direct ? static_cast<char*>(dst) + written * ob : host_raw.data();
+                if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
+                written += direct ? s1 - s0
+                                  : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, static_cast<char*>(dst) + written * ob,
+                                                  keep, k, k1, s0);
+            } else {
+                const int b = (int)(chunk & 1);
+                PH_HIP(ph, hipEventSynchronize(ph->done[b]));      // the copy that last used out[b] is done
+                void* raw = direct ? ph->out[b] : ph->raw;
+                if (!cur.read(raw, (s1 - s0) * ib)) return fail(ph, PMX_ERR_IO, "read of column " + c.name + " failed");
+                const int64_t m = direct ? s1 - s0 : convert_chunk(c.dtype, out_dtype, raw, s1 - s0, ph->out[b], keep, k, k1, s0);
+                PH_HIP(ph, hipMemcpyAsync(static_cast<char*>(dst) + written * ob, ph->out[b], (size_t)(m * ob),
+                                          hipMemcpyHostToDevice, ph->stream));
+                PH_HIP(ph, hipEventRecord(ph->done[b], ph->stream));
+                written += m;
+            }
+            k = k1;
+        }
+        if (device) PH_HIP(ph, hipStreamSynchronize(ph->stream));
+        if (written != rows_out) return fail(ph, PMX_ERR_IO, "short read of column " + c.name);
+        return PMX_OK;
+    });
 }
 
 int pmx_phantom_density(int32_t dtype, int32_t dim, int32_t mem, int64_t n, const void* h, double m, double hfact,
                         void* rho) {
-    if ((dtype != PMX_F32 && dtype != PMX_F64) || (dim != 2 && dim != 3) || n < 0 ||
-        (mem != PMX_MEM_HOST && mem != PMX_MEM_DEVICE))
-        return PMX_ERR_ARGUMENT;
-    if (n == 0) return PMX_OK;
-    if (!h || !rho) return PMX_ERR_ARGUMENT;
-    // c = m * hfact^dim, left to right in TF (_density: m * hfact * hfact [* hfact])
-    if (dtype == PMX_F64) {
-        double c = m * hfact * hfact;
-        if (dim == 3) c = c * hfact;
-        if (mem == PMX_MEM_HOST) {
-            density_host(static_cast<const double*>(h), n, c, dim, static_cast<double*>(rho));
-            return PMX_OK;
+    return guard(nullptr, [&]() -> int {
+        if ((dtype != PMX_F32 && dtype != PMX_F64) || (dim != 2 && dim != 3) || n < 0 ||
+            (mem != PMX_MEM_HOST && mem != PMX_MEM_DEVICE))
+            return PMX_ERR_ARGUMENT;
+        if (n == 0) return PMX_OK;
+        if (!h || !rho) return PMX_ERR_ARGUMENT;
+        // c = m * hfact^dim, left to right in TF (_density: m * hfact * hfact [* hfact])
+        if (dtype == PMX_F64) {
+            double c = m * hfact * hfact;
+            if (dim == 3) c = c * hfact;
+            if (mem == PMX_MEM_HOST) {
+                density_host(static_cast<const double*>(h), n, c, dim, static_cast<double*>(rho));
+                return PMX_OK;
+            }
+            const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 16);
+            hipLaunchKernelGGL(k_density<double>, dim3(grid), dim3(256), 0, 0, static_cast<const double*>(h), n, c, dim,
+                               static_cast<double*>(rho));
+        } else {
+            const float mf = (float)m, hf = (float)hfact;
+            float c = mf * hf * hf;
+            if (dim == 3) c = c * hf;
+            if (mem == PMX_MEM_HOST) {
+                density_host(static_cast<const float*>(h), n, c, dim, static_cast<float*>(rho));
+                return PMX_OK;
+            }
+            const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 16);
+            hipLaunchKernelGGL(k_density<float>, dim3(grid), dim3(256), 0, 0, static_cast<const float*>(h), n, c, dim,
+                               static_cast<float*>(rho));
         }
-        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 16);
-        hipLaunchKernelGGL(k_density<double>, dim3(grid), dim3(256), 0, 0, static_cast<const double*>(h), n, c, dim,
-                           static_cast<double*>(rho));
-    } else {
-        const float mf = (float)m, hf = (float)hfact;
-        float c = mf * hf * hf;
-        if (dim == 3) c = c * hf;
-        if (mem == PMX_MEM_HOST) {
-            density_host(static_cast<const float*>(h), n, c, dim, static_cast<float*>(rho));
-            return PMX_OK;
-        }
-        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 16);
-        hipLaunchKernelGGL(k_density<float>, dim3(grid), dim3(256), 0, 0, static_cast<const float*>(h), n, c, dim,
-                           static_cast<float*>(rho));
-    }
-    if (hipGetLastError() != hipSuccess) return PMX_ERR_HIP;
-    if (hipDeviceSynchronize() != hipSuccess) return PMX_ERR_HIP;
-    return PMX_OK;
+        if (hipGetLastError() != hipSuccess) return PMX_ERR_HIP;
+        if (hipDeviceSynchronize() != hipSuccess) return PMX_ERR_HIP;
+        return PMX_OK;
+    });
 }
 
 }  // extern "C"

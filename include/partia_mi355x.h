@@ -39,7 +39,8 @@ typedef enum pmx_status {
     PMX_ERR_NOMEM = -5,      /* device allocation failed                                     */
     PMX_ERR_CAPACITY = -6,   /* an output buffer supplied by the caller is too small         */
     PMX_ERR_FORMAT = -7,     /* not a readable Phantom dump (≙ ErrorException of read_phantom) */
-    PMX_ERR_IO = -8          /* the file cannot be opened or read (≙ SystemError)            */
+    PMX_ERR_IO = -8,         /* the file cannot be opened or read (≙ SystemError)            */
+    PMX_ERR_INTERNAL = -9    /* a C++ exception inside the library, turned into a status      */
 } pmx_status;
 
 /* Element type of particle / sample columns (InterpolationInput{3,T}: T ∈ {Float32, Float64}). */

@@ -66,3 +66,28 @@ def test_line_table_matches_oracle_and_reference(kernel):
 
 def test_line_table_rejects_bad_kernel():
     assert _lib.lib().pmx_line_kernel_table(7, None, None) == _lib.PMX_ERR_ARGUMENT
+
+
+# ---------------------------------------------------------------------------------------
+# No C++ exception crosses the ABI: a std::bad_alloc becomes PMX_ERR_NOMEM and any other
+# exception PMX_ERR_INTERNAL, with the message kept.  PMX_TEST_THROW makes the entry points'
+# guard throw before the body runs (entry points that need no device here).
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("fault, code", [("bad_alloc", _lib.PMX_ERR_NOMEM), ("exception", _lib.PMX_ERR_INTERNAL)])
+def test_exceptions_become_status_codes(monkeypatch, tmp_path, fault, code):
+    lib = _lib.lib()
+    Q, In = np.empty(4096), np.empty(4096)
+    monkeypatch.setenv("PMX_TEST_THROW", fault)
+    assert lib.pmx_line_kernel_table(0, Q.ctypes.data, In.ctypes.data) == code
+    h = C.c_void_p()
+    assert lib.pmx_create(C.byref(h), 0) == code
+    assert not h.value
+    p = tmp_path / "dump"
+    p.write_bytes(b"\0" * 64)
+    assert lib.pmx_phantom_open(C.byref(h), str(p).encode()) == code
+    msg = lib.pmx_phantom_open_error()
+    assert (b"bad_alloc" in msg) if fault == "bad_alloc" else (b"injected exception" in msg)
+    assert lib.pmx_phantom_density(_lib.PMX_F64, 3, _lib.PMX_MEM_HOST, 4, Q.ctypes.data, 1.0, 1.2,
+                                   In.ctypes.data) == code
+    monkeypatch.delenv("PMX_TEST_THROW")
+    assert lib.pmx_line_kernel_table(0, Q.ctypes.data, In.ctypes.data) == _lib.PMX_OK
