@@ -26,6 +26,7 @@ PMX_LINE_INV_H2 = 0x100     # OR'd into a line call's precision: weight Inorm / 
 
 EXPORTS = ("pmx_create", "pmx_destroy", "pmx_last_error", "pmx_version", "pmx_build_lbvh",
            "pmx_interpolate_points", "pmx_interpolate_cartesian_grid", "pmx_interpolate_lines",
+           "pmx_interpolate_lines_noalloc",
            "pmx_point_samples_interpolation",
            "pmx_line_samples_interpolation", "pmx_export_tree", "pmx_export_nodes",
            "pmx_export_particles", "pmx_nearest", "pmx_point_neighbors", "pmx_line_neighbors",
@@ -132,6 +133,8 @@ def lib():
         L.pmx_build_lbvh.argtypes = [vp, C.POINTER(Particles), i32, vp, vp]
         L.pmx_interpolate_points.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Points), C.POINTER(Output)]
         L.pmx_interpolate_lines.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Lines), C.POINTER(Output)]
+        L.pmx_interpolate_lines_noalloc.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Lines),
+                                                    C.POINTER(Output)]
         L.pmx_interpolate_cartesian_grid.argtypes = [vp, i32, i32, C.POINTER(Catalog), C.POINTER(Axes),
                                                      C.POINTER(Output)]
         L.pmx_point_samples_interpolation.argtypes = [vp, C.POINTER(Particles), i32, i32, C.POINTER(Catalog),
@@ -313,6 +316,16 @@ class Context:
         out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
         self._check(lib().pmx_interpolate_lines(self._h, int(strategy), int(precision), C.byref(cat),
                                                 C.byref(ln), C.byref(out)))
+
+    def interpolate_lines_noalloc(self, strategy, precision, cat, origin, direction, out_cols):
+        """pmx_interpolate_lines_noalloc: device lines and outputs (torch tensors), no allocation."""
+        ln = self._lines(origin, direction)
+        mem = _mem_of(out_cols) if out_cols else PMX_MEM_HOST
+        dt = _dtype_code(out_cols[0]) if out_cols else PMX_F32
+        cp = (C.c_void_p * max(1, len(out_cols)))(*[_ptr(o) for o in out_cols])
+        out = Output(dt, mem, C.cast(cp, C.POINTER(C.c_void_p)))
+        self._check(lib().pmx_interpolate_lines_noalloc(self._h, int(strategy), int(precision), C.byref(cat),
+                                                        C.byref(ln), C.byref(out)))
 
     # ---- inspection ----
     def export_tree(self):

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpartia_mi355x.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["pmx_api.hip", "pmx_build.hip", "pmx_interp64.hip", "pmx_interp32.hip", "pmx_lines_tiled.hip", "pmx_batch.hip", "pmx_phantom.hip",
-           "pmx_lut.cpp"]
+           "pmx_noalloc.hip", "pmx_lut.cpp"]
 ARCH = os.environ.get("PMX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -295,6 +295,7 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
             return PMX_ERR_HIP;
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) return PMX_ERR_HIP;
+        if (noalloc_init_device(c) != PMX_OK) return PMX_ERR_HIP;
         std::vector<double> h64;
         upload_tables(c, h64);
         std::vector<float> hf(h64.size());

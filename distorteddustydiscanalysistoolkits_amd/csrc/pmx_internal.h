@@ -91,6 +91,7 @@ struct pmx_ctx {
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
     int ncu = 256;             // compute units of the device
     pmx::Tuning tune;          // launch-shape knobs (environment at creation)
+    void* noalloc_scratch = nullptr;   // this device's address of the no-allocation scratch array
     bool line_inv_h2 = false;  // the current line call weights by Inorm / h^2 (PMX_LINE_INV_H2)
     int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
@@ -193,6 +194,9 @@ struct LineWork {
     size_t cells_bytes = 0;
 };
 
+// The no-allocation scratch array of the context's device (pmx_noalloc.hip): its address on
+// this device, looked up once per device at context creation.
+int noalloc_init_device(pmx_ctx* c);
 
 // interpolation (pmx_interp64.hip / pmx_interp32.hip)
 struct CatChunk {

@@ -169,6 +169,19 @@ int pmx_interpolate_cartesian_grid(struct pmx_ctx *ctx, int32_t strategy, int32_
 int pmx_interpolate_lines(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
                           const pmx_catalog *cat, const pmx_lines *lines, const pmx_output *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * PMX_PREC_F32C path of pmx_interpolate_lines on the context's current LBVH, with every work
+ * buffer (staged lines, binning, sort keys, histograms, results before delivery) in a static
+ * device array of about 128 MiB that each device holding the library keeps until the process
+ * ends.  It calls no hipMalloc / hipFree and allocates no host memory.  Limits: at most 1048576
+ * lines and 4 scalar columns (PMX_ERR_CAPACITY beyond), lines and outputs in device memory,
+ * outputs PMX_F32 or PMX_F64; PMX_LINE_INV_H2 may be OR'd into precision as for
+ * pmx_interpolate_lines.  Every call on one device shares the array: calls that overlap on a
+ * device (other threads or contexts) race.  Synchronous on return. */
+int pmx_interpolate_lines_noalloc(struct pmx_ctx *ctx, int32_t strategy, int32_t precision,
+                                  const pmx_catalog *cat, const pmx_lines *lines, const pmx_output *out);
+
 /* --- drivers: build + interpolate in one call ------------------------------------- */
 /* ≙ Partia.PointSamples_interpolation(backend, grid, input, catalog, strategy)
  * (ext/CUDAExt/.../PointSamples_interpolation.jl:12-29): LBVH build (with write-back of
