@@ -196,14 +196,14 @@ struct pmx_batch {
     std::vector<BatchUnit> units;
     DevBuf units_dev, counter, split_buf;
     pts::PointSplit split{};                      // heavy point tiles (gather / symmetric)
-    int split_budget = 8192, split_item = 4096;
+    int split_budget = 4096, split_item = 4096;
     int64_t total_tiles = 0;
     size_t lds_bytes = 0;
     size_t lds_items = 0;                         // dynamic LDS of the split-item kernel
     bool deriv = false;                           // some unit holds derivative terms
     int ncu = 256;
     // launch shape (autotuned)
-    int waves_per_cu = 16, bucket_points = 16, bucket_lines = 512, heavy_first = 1;
+    int waves_per_cu = 16, bucket_points = 64, bucket_lines = 512, heavy_first = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4] = {};
@@ -641,7 +641,7 @@ int pmx_batch_tune(pmx_batch* b) {
                         {&b->bucket_lines, {128, 256, 512, 1024}},
                         {&b->bucket_points, {8, 16, 32, 64}},
                         {&b->heavy_first, {0, 1}},
-                        {&b->split_budget, {2048, 8192, 32768}}};
+                        {&b->split_budget, {4096, 8192, 16384}}};
         double best_ms = 0;
         PMX_TRY(make_units(b));
         PMX_TRY(time_interpolation(b, &best_ms));

@@ -49,7 +49,7 @@ struct __align__(16) Node32 {
 // leaves.
 struct Tuning {
     int lines_tiled = 1;
-    int bucket_points = 32;
+    int bucket_points = 64;
     int bucket_lines = 64;
     int tiled_bucket = 1024;
     int tiled_near = 16;
@@ -93,7 +93,7 @@ struct pmx_ctx {
     pmx::Tuning tune;          // launch-shape knobs (environment at creation)
     void* noalloc_scratch = nullptr;   // this device's address of the no-allocation scratch array
     bool line_inv_h2 = false;  // the current line call weights by Inorm / h^2 (PMX_LINE_INV_H2)
-    int split_budget = 8192;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
+    int split_budget = 4096;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
     int split_item = 4096;
     pmx::DevBuf split;         // split-tile counters, item queue and slots (FP32c points)
     pmx::DevBuf gridpts;       // samples of a Cartesian structured grid (x | y | z), then its axes
@@ -167,7 +167,7 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20) {
 // build (pmx_build.hip)
 // build_lbvh flag (internal): capture-safe build, see pmx_build.hip
 // capacities of the split-tile storage (slots of FP64 sums, queued subtree items)
-constexpr int kSplitSlots = 8192, kSplitItems = 1 << 21;
+constexpr int kSplitSlots = 65536, kSplitItems = 1 << 21;
 constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,

@@ -191,9 +191,69 @@ __device__ inline float d2_pos32(const PointLane& q, const BPos& P, int j, float
     return dx * dx + dy * dy + dz * dz;
 }
 
+// Squared distance from a point to an FP32 node box (FP32, no guard: a heuristic only).
+__device__ inline float d2_point_box_plain(const float c[3], const Node32& b) {
+    const float mn[3] = {b.mnx, b.mny, b.mnz}, mx[3] = {b.mxx, b.mxy, b.mxz};
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float t = c[k] < mn[k] ? mn[k] - c[k] : (c[k] > mx[k] ? c[k] - mx[k] : 0.0f);
+        s += t * t;
+    }
+    return s;
+}
+
+// The leaf run of at most T.bucket particles reached by descending from the root towards the
+// point c, at each node into the child whose box is closer (a leaf child is its position).
+// The right child of a node is the escape link of its left child.  Wave-uniform.
+__device__ inline void seed_run(const T32& T, const float c[3], int& lo, int& cnt) {
+    const int n = (int)T.n;
+    int node = T.root;
+    Node32 nd = T.nodes[node];
+    for (int depth = 0; depth < 64; ++depth) {
+        if (uniform(nd.cnt) <= T.bucket) break;
+        const int l = uniform(nd.left);
+        float dl, dr;
+        int r;
+        Node32 nl{}, nr{};
+        if (l >= n) {
+            const float4 L = T.leaf[l - n];
+            const float p[3] = {L.x - c[0], L.y - c[1], L.z - c[2]};
+            dl = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+            r = uniform(T.leaf_esc[l - n]);
+        } else {
+            nl = T.nodes[l];
+            dl = d2_point_box_plain(c, nl);
+            r = uniform(nl.escape);
+        }
+        if (r >= n) {
+            const float4 R = T.leaf[r - n];
+            const float p[3] = {R.x - c[0], R.y - c[1], R.z - c[2]};
+            dr = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        } else {
+            nr = T.nodes[r];
+            dr = d2_point_box_plain(c, nr);
+        }
+        const bool right = uniform(dr < dl ? 1 : 0) != 0;
+        const int next = right ? r : l;
+        if (next >= n) {                       // a leaf: the run of that one particle
+            lo = next - n;
+            cnt = 1;
+            return;
+        }
+        nd = right ? nr : nl;
+    }
+    lo = uniform(nd.lo);
+    cnt = uniform(nd.cnt);
+}
+
 // LBVH_find_nearest_h (LinearBVHQueries.jl:118-141) for a tile: exact FP64 argmin of d2,
-// ties to the smaller sorted index (leaves come in increasing index order and only a
-// strictly smaller FP64 distance replaces the best).
+// ties to the smaller sorted index.  The reference walks from the root with no bound and takes
+// a strictly smaller distance; here the search first tests the leaf run nearest to the tile's
+// centre (a greedy descent), so the bound prunes the walk from its first node.  A candidate
+// replaces the best when its FP64 distance is smaller, or equal with a smaller index, so the
+// result is the lexicographic minimum of (d2, index) whatever the visiting order; nodes at the
+// best distance are never pruned (guard band), so ties are all seen.
 __device__ inline int64_t nearest32(const T32& T, const PointLane& q, BPos& P) {
     int64_t best = -1;
     double best64 = __builtin_inf();
@@ -211,13 +271,37 @@ __device__ inline int64_t nearest32(const T32& T, const PointLane& q, BPos& P) {
             if (ballot(maybe)) {
                 if (maybe) {
                     const double e = d2_exact64(q.p64, T, lo + j);
-                    if (e < best64) { best64 = e; best = lo + j; best32 = f32_up(e); }
+                    if (e < best64 || (e == best64 && lo + j < best)) { best64 = e; best = lo + j; best32 = f32_up(e); }
                 }
             }
         }
         __syncthreads();
     };
-    walk(T, ballot(q.valid) != 0, need, bucket);
+    if (!ballot(q.valid)) return -1;
+    if (T.root != 0) {
+        // centre of the tile's valid samples
+        float lo3[3], hi3[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo3[k] = q.valid ? q.ph[k] : __builtin_inff();
+            hi3[k] = q.valid ? q.ph[k] : -__builtin_inff();
+        }
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lo3[k] = fminf(lo3[k], __shfl_xor(lo3[k], o));
+                hi3[k] = fmaxf(hi3[k], __shfl_xor(hi3[k], o));
+            }
+        }
+        float c[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = 0.5f * (lo3[k] + hi3[k]);      // equal on every lane
+        int slo = 0, scnt = 0;
+        seed_run(T, c, slo, scnt);
+        bucket(slo, scnt);
+    }
+    walk(T, true, need, bucket);
     return best;
 }
 
@@ -243,6 +327,57 @@ __device__ inline bool node_need_point(const PointLane& q, const Node32& nd, flo
     const float rb = Kvf * nd.hmax, r2b = rb * rb;
     const float r2 = S == kGather ? radius2_32 : (S == kScatter ? r2b : fmaxf(radius2_32, r2b));
     return q.valid && !(d2_box32(q, nd) > r2 * (1.0f + kEps));
+}
+
+// The box of a tile's valid samples (FP64 coordinates rounded outwards to FP32) and the largest
+// squared search radius of its samples (gather / symmetric; 0 for scatter).  Wave-uniform.
+struct TileBox {
+    float mn[3], mx[3];
+    float r2;
+};
+
+template <int S>
+__device__ inline TileBox tile_box(const PointLane& q, float radius2_32) {
+    double lo[3], hi[3];
+    float r2 = q.valid && S != kScatter ? radius2_32 : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = q.valid ? q.p64[k] : __builtin_inf();
+        hi[k] = q.valid ? q.p64[k] : -__builtin_inf();
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], o));
+        }
+        r2 = fmaxf(r2, __shfl_xor(r2, o));      // a lane without particle (NaN) loses to any number
+    }
+    TileBox B;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { B.mn[k] = f32_down(lo[k]); B.mx[k] = f32_up(hi[k]); }
+    B.r2 = r2;
+    return B;
+}
+
+// Can staged particle j be accepted by any sample of the tile?  Its distance from the tile's box
+// against the largest radius a pair with it can have, conservative as d2_box32 (a rejected
+// particle is further than its radius rule from every sample).
+template <int S>
+__device__ inline bool reaches_tile(const TileBox& B, const BPos& P, int j, float Kvf) {
+    float s = 0.0f;
+    const float xh[3] = {P.x[j], P.y[j], P.z[j]}, xl[3] = {P.xl[j], P.yl[j], P.zl[j]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float lo_gap = (B.mn[k] - xh[k]) - xl[k];
+        const float hi_gap = (xh[k] - B.mx[k]) + xl[k];
+        const float t = lo_gap > 0.0f ? lo_gap : (hi_gap > 0.0f ? hi_gap : 0.0f);
+        s += t * t;
+    }
+    const float rb = Kvf * P.h[j], r2b = rb * rb;
+    const float r2 = S == kGather ? B.r2 : (S == kScatter ? r2b : fmaxf(B.r2, r2b));
+    return !(s * (1.0f - 2.0f * kEps) > r2 * (1.0f + kEps));
 }
 
 // Scalars are staged from the FP32 payload; the fields under a derivative are staged in FP64
@@ -363,6 +498,7 @@ struct PointWalk<K, S, false> {
     float Kvf, sig, radius2_32;
     int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
     int spent;                  // leaves tested so far (wave-uniform)
+    TileBox box;                // the tile's samples and their largest radius (wave-uniform)
 
     __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<false>& Y_, const PointLane& q_)
         : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
@@ -378,6 +514,7 @@ struct PointWalk<K, S, false> {
             radius2_32 = (float)radius2_64;
             invha64 = 1.0 / ha64;
         }
+        box = tile_box<S>(q, radius2_32);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 3; ++k) { Y.ph[k][lane] = q.ph[k]; Y.pl[k][lane] = q.pl[k]; }
@@ -387,6 +524,7 @@ struct PointWalk<K, S, false> {
     }
 
     __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
+
 
     __device__ double sum(int a) const { return Y.acc[a][lane]; }
 
@@ -431,8 +569,8 @@ struct PointWalk<K, S, false> {
         __syncthreads();
     }
 
-    // a run of leaves lo .. lo+cnt-1: stage, test every particle against every lane's sample,
-    // queue the accepted pairs and process them 64 at a time
+    // a run of leaves lo .. lo+cnt-1: stage; test the particles that can reach the tile's box
+    // against every lane's sample; queue the accepted pairs and process them 64 at a time
     __device__ __attribute__((always_inline)) void run(int lo, int cnt) {
         const int64_t n = T.n;
         stage_pos(T, P, lo, cnt);
@@ -443,7 +581,9 @@ struct PointWalk<K, S, false> {
             for (int f = 0; f < cat.ns; ++f) Y.s[f][lane] = T.pay[(int64_t)cat.scol[f] * n + li];
         }
         __syncthreads();
-        for (int j = 0; j < cnt; ++j) {
+        const uint64_t cand = ballot(lane < cnt && reaches_tile<S>(box, P, lane, Kvf));
+        for (uint64_t cm = cand; cm; cm &= cm - 1) {
+            const int j = __builtin_ctzll(cm);
             float dx, dy, dz;
             const float d2 = d2_pos32(q, P, j, dx, dy, dz);
             const bool acc = accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64);
@@ -483,6 +623,7 @@ struct PointWalk<K, S, true> {
     float Kvf, radius2_32;
     int qhead, qtail;           // wave-uniform queue positions (pairs appended / processed)
     int spent;                  // leaves tested so far (wave-uniform)
+    TileBox box;                // the tile's samples and their largest radius (wave-uniform)
 
     __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<true>& Y_, const PointLane& q_)
         : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
@@ -497,6 +638,7 @@ struct PointWalk<K, S, true> {
             radius2_32 = (float)radius2_64;
             invha64 = 1.0 / ha64;
         }
+        box = tile_box<S>(q, radius2_32);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 3; ++k) { Y.ph[k * kWave + lane] = q.ph[k]; Y.pl[k * kWave + lane] = q.pl[k]; }
@@ -505,6 +647,7 @@ struct PointWalk<K, S, true> {
     }
 
     __device__ bool need(const Node32& nd) const { return node_need_point<S>(q, nd, Kvf, radius2_32); }
+
 
     __device__ double sum(int a) const { return cat.amap[a] < 0 ? 0.0 : Y.acc[cat.amap[a] * kWave + lane]; }
 
@@ -601,7 +744,9 @@ struct PointWalk<K, S, true> {
                 Y.d[f * kMaxBucket + lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
         }
         __syncthreads();
-        for (int j = 0; j < cnt; ++j) {
+        const uint64_t cand = ballot(lane < cnt && reaches_tile<S>(box, P, lane, Kvf));
+        for (uint64_t cm = cand; cm; cm &= cm - 1) {
+            const int j = __builtin_ctzll(cm);
             float dx, dy, dz;
             const float d2 = d2_pos32(q, P, j, dx, dy, dz);
             const bool acc = accept_point<S>(T, q, d2, P.h[j], lo + j, Kvf, Kv, radius2_32, radius2_64);
