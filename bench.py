@@ -228,13 +228,7 @@ def main():
     achieved = pair_bytes / t_kernel / 1e9
     flops_pair = 103 if w.kind == "points" else 30
     flops = flops_pair * n_acc
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get(f"{w.kind}_{main_key}", {}).get("bytes")
-        except Exception:
-            traffic = None
+    traffic, traffic_src = measured_traffic(f"{w.kind}_{main_key}", n=w.n, P=w.P)
 
     errors = {}
     if "f32c" in results and "f64" in results:
@@ -263,7 +257,7 @@ def main():
         "data": f"synthetic (deterministic SplitMix64 {'warped' if w.kind == 'lines' else 'flat'} disc, seed 0)",
         "config": {"workload": w.desc, "particles": n, "samples": P, "path": main_key},
         "build_particles_per_s": n / (r["build_ms"] * 1e-3),
-        "build_roofline": build_roofline(n, k, r["build_ms"]),
+        "build_roofline": build_roofline(n, k, r["build_ms"], w.kind),
         "interp_points_per_s": P / (r["interp_ms"] * 1e-3),
         "build_ms": r["build_ms"],
         "interp_ms": r["interp_ms"],
@@ -286,6 +280,7 @@ def main():
                      "algorithmic_bytes_per_launch": pair_bytes, "bytes_per_unit": rec, "units": n_acc,
                      "unit_name": "accepted (sample, particle) pairs",
                      "measured_GBs": (traffic / t_kernel / 1e9) if traffic else None,
+                     "traffic_source": traffic_src,
                      "traversal_model_bytes": trav_bytes,
                      "valu_TFLOPs": flops / t_kernel / 1e12,
                      "valu_frac": flops / t_kernel / 1e12 / FP32_PEAK_TFLOPS},
@@ -630,14 +625,57 @@ def per_call_compare(jobs, torch, dev):
     return res
 
 
-def build_roofline(n, k, build_ms, passes=8):
+def measured_traffic(entry, n=None, P=None):
+    """HBM bytes per launch of a kernel group from profiles/pmc_traffic.json (calibrated rocprofv3
+    FETCH_SIZE + WRITE_SIZE of an earlier profiled run), with where they come from.  The entry is
+    used only when its workload has this run's particle and sample counts; otherwise (None,
+    reason)."""
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        e = json.load(open(tfile)).get(entry)
+    except Exception:
+        e = None
+    if not e:
+        return None, {"file": "profiles/pmc_traffic.json", "entry": entry, "used": False, "reason": "no entry"}
+    wl = e.get("workload", {})
+    src = {"file": "profiles/pmc_traffic.json", "entry": entry, "commit": e.get("commit"), "workload": wl,
+           "kernel": e.get("kernel")}
+    if (n is not None and wl.get("particles") != n) or (P is not None and "samples" in wl and wl["samples"] != P):
+        src.update(used=False, reason="profiled workload differs from this run")
+        return None, src
+    src["used"] = True
+    return e.get("bytes"), src
+
+
+def build_roofline(n, k, build_ms, kind=None, passes=8):
     """Algorithmic HBM bytes of the LBVH build per particle (FP32 storage, k quant columns,
     8-bit radix passes): bbox 12 + encode 24 + sort (8 + 24 P) + column gather (4 + 8 (6 + k))
-    + BRT (8 + 32) + refit 56 = 192 + 8 k + 24 P, against the 8 TB/s peak."""
+    + BRT (8 + 32) + refit 56 = 192 + 8 k + 24 P, against the 8 TB/s peak.  Beside it, the
+    measured bytes of one build (calibrated FETCH + WRITE of the build kernels, profiled on the
+    line-map workload) when the particle count matches."""
     b = 192 + 8 * k + 24 * passes
     gbs = b * n / (build_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "bytes_per_particle": b, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS}
+    out = {"bound": "hbm", "bytes_per_particle": b, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": gbs / HBM_PEAK_GBS, "traffic": None, "measured_GBs": None}
+    if kind == "lines":
+        tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+        try:
+            e = json.load(open(tfile)).get("build_lines")
+        except Exception:
+            e = None
+        if e and e.get("workload", {}).get("particles") == n:
+            # one build: 8 radix passes, one FP32 refit of the short nodes, 10 refit rounds
+            per_build = {"k_radix_hist": passes, "k_radix_scan": passes, "k_radix_scatter": passes,
+                         "k_refit_round": 10, "k_fill_u32": 4}
+            tot = 0
+            for name, v in e["kernels"].items():
+                cnt = per_build.get(name, 1)
+                tot += cnt * (v["fetch_bytes_per_dispatch"] + v["write_bytes_per_dispatch"])
+            out["traffic"] = tot
+            out["measured_GBs"] = tot / (build_ms * 1e-3) / 1e9
+            out["traffic_source"] = {"file": "profiles/pmc_traffic.json", "entry": "build_lines",
+                                     "commit": e.get("commit")}
+    return out
 
 
 def cpu_baseline(w, nsample):

@@ -4,6 +4,8 @@ The checks use definitions, not the oracle's construction steps:
   * every internal node covers a contiguous leaf range, its two children partition that
     range, and the split sits where the first bit differing between the range's first and
     last code flips (the defining property of a binary radix tree, Karras 2012);
+  * internal node i's range has leaf i as one end (Karras' node numbering, which fixes the ids:
+    a consistent relabelling of the internal nodes fails it);
   * escape[id] is the DFS-preorder successor outside id's subtree, computed by explicit
     recursion (as the reference's _build_escape_stackless!, BinaryRadixTree.jl:283-312);
   * node boxes / hmax equal brute-force min/max over the covered leaves.
@@ -48,6 +50,10 @@ def check_tree(t):
         lo, mid = leaf_range(t, l)
         mid2, hi = leaf_range(t, r)
         assert mid2 == mid + 1
+        # Karras id assignment: internal node i's leaf range has leaf i as one end
+        assert lo == i or hi == i, (i, lo, hi)
+        # and the right child is the escape (DFS successor) of the left child
+        assert int(t.escape[l - 1]) == r
         a, b = codes[lo - 1], codes[hi - 1]
         bit = 63 - (a ^ b).bit_length() + 1          # leading zeros of a ^ b
         pos = 63 - bit                               # index of the first differing bit

@@ -1,0 +1,105 @@
+"""AddressSanitizer + UndefinedBehaviorSanitizer runs of host code, no device needed:
+  * the oracle (oracle/partia_oracle.c) built with gcc and driven by tests/sanitize/oracle_driver.c
+    through builds, queries and interpolations on small, duplicate and single-particle sets;
+  * the host side of libpartia_mi355x (every source, host code instrumented, device code as
+    usual) driven by tests/sanitize/host_driver.cpp: the line-kernel tables and the Phantom
+    parser on valid, truncated and corrupted dumps.
+A sanitizer report aborts the driver with a non-zero status."""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from distorteddustydiscanalysistoolkits_amd import phantomdump as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "distorteddustydiscanalysistoolkits_amd", "csrc")
+SAN_ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+
+
+def _run(cmd, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    assert r.returncode == 0, f"{' '.join(cmd[:4])} ...\n{r.stdout[-3000:]}\n{r.stderr[-6000:]}"
+    return r
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_oracle_under_asan_ubsan(tmp_path):
+    exe = str(tmp_path / "oracle_driver")
+    _run(["gcc", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+          "-fno-sanitize-recover=undefined", "-fopenmp", "-ffp-contract=off", "-std=c11",
+          os.path.join(ROOT, "oracle", "partia_oracle.c"), os.path.join(ROOT, "tests", "sanitize", "oracle_driver.c"),
+          "-lm", "-o", exe])
+    assert "__asan_init" in _run(["nm", exe]).stdout, "the driver is not instrumented"
+    r = _run([exe], env=dict(SAN_ENV, OMP_NUM_THREADS="2"))
+    assert "oracle driver ok" in r.stdout
+
+
+def _dumps(tmp_path):
+    rng = np.random.default_rng(5)
+    n = 200
+    g = {k: rng.standard_normal(n).astype(np.float32) for k in ("x", "y", "z")}
+    g["h"] = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    arrays = [(W.DEF_REAL, k, g[k]) for k in ("x", "y", "z", "h")]
+    arrays.append((W.I8, "itype", np.ones(n, np.int8)))
+    header = {W.DEF_INT: [("nparttot", n), ("npartoftype", n), ("npartoftype", 0)],
+              W.DEF_REAL: [("time", 1.0), ("hfact", 1.2), ("massoftype", 1e-6)]}
+    paths = []
+    for rb in (4, 8):
+        p = str(tmp_path / f"good{rb}")
+        W.write_dump(p, header=header, blocks=[(n, arrays), (0, [])], real_bytes=rb)
+        paths.append(p)
+    good = open(paths[0], "rb").read()
+    for t in range(60):
+        b = bytearray(good)
+        if t % 3 == 0:
+            b = b[:int(rng.integers(0, len(b)))]
+        else:
+            for _ in range(int(rng.integers(1, 6))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        p = str(tmp_path / f"bad{t}")
+        open(p, "wb").write(bytes(b))
+        paths.append(p)
+    return paths
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_host_library_under_asan_ubsan(tmp_path):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    common = ["-O1", "-g", "-fno-omit-frame-pointer", "-fPIC", "-std=c++17", "-ffp-contract=off",
+              "-I" + os.path.join(ROOT, "include")]
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=undefined"]
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+
+    def compile_one(f):
+        obj = str(tmp_path / (f + ".o"))
+        cmd = [hipcc] + common + san
+        if f.endswith(".hip"):
+            cmd += ["-x", "hip", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+        _run(cmd + ["-c", os.path.join(CSRC, f), "-o", obj])
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(compile_one, srcs))
+    # the driver is plain C++: compiled by the same clang in host-only mode, then linked with the
+    # library objects (the link brings the sanitizer runtimes for the host side)
+    clangxx = os.path.join(os.path.dirname(os.path.realpath(hipcc)), "..", "lib", "llvm", "bin", "clang++")
+    if not os.path.exists(clangxx):
+        clangxx = "/opt/rocm/lib/llvm/bin/clang++"
+    drv = str(tmp_path / "host_driver.o")
+    _run([clangxx, "-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-fsanitize=address,undefined",
+          "-fno-sanitize-recover=undefined", "-I" + os.path.join(ROOT, "include"), "-c",
+          os.path.join(ROOT, "tests", "sanitize", "host_driver.cpp"), "-o", drv])
+    exe = str(tmp_path / "host_driver")
+    _run([hipcc, "--offload-arch=gfx950", "-fPIC"] + san + [drv] + objs + ["-o", exe])
+    assert "__asan_init" in _run(["nm", exe]).stdout, "the driver is not instrumented"
+    r = _run([exe] + _dumps(tmp_path), env=SAN_ENV)
+    assert "host driver ok" in r.stdout
+    parsed = int(r.stdout.split("host driver ok: ")[1].split()[0])
+    assert parsed >= 2
