@@ -31,8 +31,10 @@ def hipcc() -> str:
 
 
 def common_flags():
+    # -Werror=shadow: a kernel-local name that hides an outer one captured by a lambda (a tile
+    # bound, a counter) silently changes which value the lambda reads
     return ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
-            "-Wno-unused-variable", "-I" + os.path.join(HERE, "..", "include")]
+            "-Wno-unused-variable", "-Werror=shadow", "-I" + os.path.join(HERE, "..", "include")]
 
 
 def compile_one(src: str) -> str:

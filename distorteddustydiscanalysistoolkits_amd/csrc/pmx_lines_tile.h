@@ -230,10 +230,10 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 double2 Lnext = WG1 ? make_double2(0.0, 0.0) : st_sorted[k];
                 for (int it = 0; it < len; ++it) {
                     const int kk = k - C0;
-                    const double2 L = WG1 ? ST[kk] : Lnext;
+                    const double2 Ln = WG1 ? ST[kk] : Lnext;
                     k = (k + 1 == b) ? a : k + 1;
                     if constexpr (!WG1) Lnext = st_sorted[k];
-                    const double ds = sb - L.x, dt = tb - L.y;
+                    const double ds = sb - Ln.x, dt = tb - Ln.y;
                     const double d2 = ds * ds + dt * dt;
                     if (d2 > r2hi) continue;
                     bool ok = d2 < r2lo;
@@ -261,18 +261,18 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         if (x < xmax) {
                             int i = (int)x;
                             i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
-                            const double t = x - (double)i;
+                            const double ft = x - (double)i;
                             float2 Lv;                                 // lut[i], lut[i + 1]
                             __builtin_memcpy(&Lv, lut + i, sizeof(Lv));
-                            const double L0 = (double)Lv.x;
-                            I = fma(t, (double)Lv.y - L0, L0);
+                            const double I0 = (double)Lv.x;
+                            I = fma(ft, (double)Lv.y - I0, I0);
                         }
                         const double w = wih * I;
                         atomicAdd(&acc[kk], w);
                         if (cat.n > 0) atomicAdd(&acc[kCap + kk], (double)A0 * w);
                         if (cat.n > 1) atomicAdd(&acc[2 * kCap + kk], (double)A1 * w);
-                        for (int cc = 2; cc < cat.n; ++cc)
-                            atomicAdd(&acc[(1 + cc) * kCap + kk], (double)w_A[cc * 64 + j] * w);
+                        for (int c = 2; c < cat.n; ++c)
+                            atomicAdd(&acc[(1 + c) * kCap + kk], (double)w_A[c * 64 + j] * w);
                     } else {
                         const unsigned long long ip = (unsigned long long)pj;
                         atomicAdd(&dg[kk], 1ull);
