@@ -664,13 +664,8 @@ def build_roofline(n, k, build_ms, kind=None, passes=8):
         except Exception:
             e = None
         if e and e.get("workload", {}).get("particles") == n:
-            # one build: 8 radix passes, one FP32 refit of the short nodes, 10 refit rounds
-            per_build = {"k_radix_hist": passes, "k_radix_scan": passes, "k_radix_scatter": passes,
-                         "k_refit_round": 10, "k_fill_u32": 4}
-            tot = 0
-            for name, v in e["kernels"].items():
-                cnt = per_build.get(name, 1)
-                tot += cnt * (v["fetch_bytes_per_dispatch"] + v["write_bytes_per_dispatch"])
+            # per-kernel entries are totals over the kernel's dispatches in one build
+            tot = sum(v["fetch_bytes"] + v["write_bytes"] for v in e["kernels"].values())
             out["traffic"] = tot
             out["measured_GBs"] = tot / (build_ms * 1e-3) / 1e9
             out["traffic_source"] = {"file": "profiles/pmc_traffic.json", "entry": "build_lines",

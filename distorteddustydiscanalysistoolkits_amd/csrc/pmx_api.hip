@@ -565,6 +565,7 @@ int pmx_export_particles(pmx_ctx* c, double* const* cols, int32_t ncols) {
         PMX_HIP(c, hipSetDevice(c->device));
         if (!cols || ncols < 0 || ncols > 6 + c->tree.ncol) return set_error(c, PMX_ERR_DIMENSION, "export_particles: ncols");
         const int64_t n = c->tree.n;
+        PMX_TRY(ensure_cols64(c));
         PMX_HIP(c, hipStreamSynchronize(c->stream));
         for (int j = 0; j < ncols; ++j)
             if (cols[j]) PMX_HIP(c, hipMemcpy(cols[j], c->cols64.as<double>() + (size_t)j * n, 8 * (size_t)n, hipMemcpyDeviceToHost));

@@ -434,7 +434,9 @@ __global__ void __launch_bounds__(kBlock) k_pack_records(ColPtrs src, int ncols,
 
 // Also writes the FP32 leaf data of the compensated path: {x_hi, y_hi, z_hi, h} and
 // {x_lo, y_lo, z_lo, 1/h} (hi + lo = the FP64 coordinate to 2^-48), and the payload columns
-// m/rho | quant... | rho in FP32 followed by m/rho in FP64.
+// m/rho | quant... | rho in FP32 followed by m/rho in FP64.  An FP32 input's FP64 columns are
+// not written: the leaf (residual zero) and payload arrays hold its values exactly, and
+// k_gather_cols64 writes them when a caller needs the FP64 layout.
 struct Leaf32Out {
     float4* leaf;
     float4* leaflo;
@@ -452,15 +454,16 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
         int32_t o = order[i];
         o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;   // in-bounds guard (order is a permutation)
         const T* r = rec + (int64_t)o * ncols;
+        constexpr bool kCols = sizeof(T) == 8;
         double v[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             v[c] = (double)r[c];
-            cols64[(int64_t)c * n + i] = v[c];
+            if (kCols) cols64[(int64_t)c * n + i] = v[c];
         }
         for (int c = 6; c < ncols; ++c) {
             const double q = (double)r[c];
-            cols64[(int64_t)c * n + i] = q;
+            if (kCols) cols64[(int64_t)c * n + i] = q;
             o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
         }
         const float xh = (float)v[0], yh = (float)v[1], zh = (float)v[2];
@@ -472,6 +475,18 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
         o32.pay[(int64_t)(1 + ncol) * n + i] = (float)v[5];
         o32.w64[i] = w;
         o32.ih64[i] = 1.0 / v[4];
+    }
+}
+
+// FP64 columns of an FP32 input in Morton order, from the packed records.
+__global__ void __launch_bounds__(kBlock) k_gather_cols64(const float* __restrict__ rec, int ncols,
+                                                          const int32_t* __restrict__ order, int64_t n,
+                                                          double* __restrict__ cols64) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int32_t o = order[i];
+        o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;
+        const float* r = rec + (int64_t)o * ncols;
+        for (int c = 0; c < ncols; ++c) cols64[(int64_t)c * n + i] = (double)r[c];
     }
 }
 
@@ -575,8 +590,9 @@ __global__ void __launch_bounds__(kBlock) k_escape_internal(const int32_t* __res
 }
 
 // Leaf escapes (BinaryRadixTree.jl:262-281): left child -> right sibling, right child ->
-// escape of the parent.  Independent per leaf once the internal escapes exist.
-__global__ void __launch_bounds__(kBlock) k_escape_leaf(const int32_t* __restrict__ lr,
+// escape of the parent.  Independent per leaf once the internal escapes exist.  Also writes
+// the leaves' (zero) child links, which k_brt does not touch.
+__global__ void __launch_bounds__(kBlock) k_escape_leaf(int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ parent, int64_t n,
                                                         int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc) {
     for (int64_t li = (int64_t)blockIdx.x * kBlock + threadIdx.x; li < n; li += (int64_t)gridDim.x * kBlock) {
@@ -587,6 +603,20 @@ __global__ void __launch_bounds__(kBlock) k_escape_leaf(const int32_t* __restric
         else e = (lr[2 * p] == (int32_t)id) ? lr[2 * p + 1] : escape[p];
         escape[id] = e;
         leaf_esc[li] = e;
+        lr[2 * id] = 0;
+        lr[2 * id + 1] = 0;
+    }
+}
+
+// The tree entries no construction kernel writes: the child links of id 0 and the parent of the
+// root (id 1; also the only leaf when n = 1), and escape[0].
+__global__ void k_tree_heads(int32_t* __restrict__ lr, int32_t* __restrict__ parent, int32_t* __restrict__ escape) {
+    if (threadIdx.x == 0) {
+        lr[0] = 0;
+        lr[1] = 0;
+        parent[0] = 0;
+        parent[1] = 0;
+        escape[0] = 0;
     }
 }
 
@@ -619,16 +649,36 @@ __device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __r
     n32[i] = b;
 }
 
-__device__ inline void combine_node(const double* __restrict__ cols64, const int32_t* __restrict__ lr,
+// Box and h of leaf li: from cols64 (FP64 input) or exactly from the FP32 leaf (FP32 input).
+__device__ inline Leaf64 read_leaf(const double* __restrict__ cols64, const float4* __restrict__ leaf32, int64_t n,
+                                   int64_t li) {
+    Leaf64 L;
+    if (leaf32) {
+        const float4 H = leaf32[li];
+        L.x = (double)H.x;
+        L.y = (double)H.y;
+        L.z = (double)H.z;
+        L.h = (double)H.w;
+    } else {
+        L.x = cols64[li];
+        L.y = cols64[n + li];
+        L.z = cols64[2 * n + li];
+        L.h = cols64[4 * n + li];
+    }
+    return L;
+}
+
+__device__ inline void combine_node(const double* __restrict__ cols64, const float4* __restrict__ leaf32,
+                                    const int32_t* __restrict__ lr,
                                     const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p,
                                     const int32_t* __restrict__ rlo, const int32_t* __restrict__ rhi,
                                     Node32* __restrict__ n32) {
     const int32_t l = lr[2 * p], r = lr[2 * p + 1];
     double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
     if (l >= n) {
-        const int64_t li = l - n;
-        lmn[0] = lmx[0] = cols64[li]; lmn[1] = lmx[1] = cols64[n + li]; lmn[2] = lmx[2] = cols64[2 * n + li];
-        hl = cols64[4 * n + li];
+        const Leaf64 L = read_leaf(cols64, leaf32, n, l - n);
+        lmn[0] = lmx[0] = L.x; lmn[1] = lmx[1] = L.y; lmn[2] = lmx[2] = L.z;
+        hl = L.h;
     } else if (nodes) {
         const Node64 N = nodes[l];
         for (int k = 0; k < 3; ++k) { lmn[k] = N.mn[k]; lmx[k] = N.mx[k]; }
@@ -642,9 +692,9 @@ __device__ inline void combine_node(const double* __restrict__ cols64, const int
         hl = N.hmax;
     }
     if (r >= n) {
-        const int64_t ri = r - n;
-        rmn[0] = rmx[0] = cols64[ri]; rmn[1] = rmx[1] = cols64[n + ri]; rmn[2] = rmx[2] = cols64[2 * n + ri];
-        hr = cols64[4 * n + ri];
+        const Leaf64 L = read_leaf(cols64, leaf32, n, r - n);
+        rmn[0] = rmx[0] = L.x; rmn[1] = rmx[1] = L.y; rmn[2] = rmx[2] = L.z;
+        hr = L.h;
     } else if (nodes) {
         const Node64 N = nodes[r];
         for (int k = 0; k < 3; ++k) { rmn[k] = N.mn[k]; rmx[k] = N.mx[k]; }
@@ -706,7 +756,8 @@ __device__ inline void block_append(bool take, int32_t p, int32_t* __restrict__ 
 
 constexpr int kRefitPad = 64;   // largest leaf range of a node refit directly (staged window margin)
 
-__global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64, const int32_t* __restrict__ lr,
+__global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64,
+                                                        const float4* __restrict__ leaf32, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
                                                         const int32_t* __restrict__ range_lo,
                                                         const int32_t* __restrict__ range_hi, int64_t n,
@@ -725,14 +776,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
         const int64_t w0 = i0 - 1 - kPad > 0 ? i0 - 1 - kPad : 0;
         const int64_t w1 = i0 + kBlock + kPad < n ? i0 + kBlock + kPad : n;
         __syncthreads();
-        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) {
-            Leaf64 L;
-            L.x = cols64[k];
-            L.y = cols64[n + k];
-            L.z = cols64[2 * n + k];
-            L.h = cols64[4 * n + k];
-            win[k - w0] = L;
-        }
+        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) win[k - w0] = read_leaf(cols64, leaf32, n, k);
         __syncthreads();
         const int64_t i = i0 + threadIdx.x;
         bool take = false;
@@ -774,7 +818,8 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
 // One bottom-up round: complete every node of the list from its two complete children, then
 // count it at its parent; a parent counted twice is complete-able in the next round.
 struct RefitTree {
-    const double* cols64;       // x | y | z | m | h ... in Morton order
+    const double* cols64;       // x | y | z | m | h ... in Morton order (FP64 input)
+    const float4* leaf32;       // x y z h (FP32 input: exact; null for an FP64 input)
     const int32_t* lr;
     const int32_t* parent;
     const int32_t* escape;
@@ -793,7 +838,7 @@ __device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int
     bool take = false;
     int32_t p = 0;
     if (valid && x > 0 && x < n) {
-        combine_node(R.cols64, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
+        combine_node(R.cols64, R.leaf32, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
         const int32_t* parent = R.parent;
         uint32_t* visited = R.visited;
         p = parent[x];
@@ -902,7 +947,8 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     uint32_t* counts = reinterpret_cast<uint32_t*>(c->scratch_c.p);
     hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(kBlock), 0, s, counts, kMaxRounds, 0u);
     RefitTree R;
-    R.cols64 = c->cols64.as<double>();
+    R.cols64 = c->tree.f32in ? nullptr : c->cols64.as<double>();
+    R.leaf32 = c->tree.f32in ? c->leaf32.as<float4>() : nullptr;
     R.lr = c->lr.as<int32_t>();
     R.parent = c->parent.as<int32_t>();
     R.escape = c->escape.as<int32_t>();
@@ -913,7 +959,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     R.nodes = fp64 ? c->nodes64.as<Node64>() : nullptr;
     R.n32 = c->nodes32.as<Node32>();
     const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
-    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.lr, R.escape, R.rlo, R.rhi, n,
+    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape, R.rlo, R.rhi, n,
                        R.visited, R.nodes, R.n32, std::min(c->tune.refit_small, kRefitPad), listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
@@ -1000,8 +1046,9 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_dedup_apply, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
     if (!async) PMX_HIP(c, hipEventRecord(c->ev[1], s));
     // ---- columns into Morton order ----
-    PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
-    double* cols64 = c->cols64.as<double>();
+    const bool f32in = p->dtype == PMX_F32;
+    if (!f32in) PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
+    double* cols64 = f32in ? nullptr : c->cols64.as<double>();
     ColPtrs cp;
     for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
     const unsigned g = grid_for(n, kBlock, 16384);
@@ -1026,6 +1073,8 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
                            order, n, cols64, o32);
     }
+    c->tree.f32in = f32in;
+    c->tree.cols64_ready = !f32in;
     if (!async) PMX_HIP(c, hipEventRecord(c->ev[2], s));
     // ---- radix tree, escapes, refit ----
     PMX_TRY(ensure(c, c->lr, 8 * (size_t)(2 * n + 2)));
@@ -1040,11 +1089,10 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     int32_t* lr = c->lr.as<int32_t>();
     int32_t* parent = c->parent.as<int32_t>();
     int32_t* escape = c->escape.as<int32_t>();
-    const unsigned g2 = grid_for(2 * n + 2, kBlock, 16384);
-    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)lr, 4 * n + 4, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)parent, 2 * n + 2, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(g2), dim3(kBlock), 0, s, (uint32_t*)escape, 2 * n + 2, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, c->visited.as<uint32_t>(), n + 2, 0u);
+    // No fills: k_brt writes the links, parents and ranges of every internal node, the escape
+    // kernels every escape and the leaves' (zero) links, k_refit_small the arrival counter of
+    // every node the bottom-up rounds count at
+    hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
         hipLaunchKernelGGL(k_brt, dim3(g), dim3(kBlock), 0, s, codes, n, lr, parent, c->range_hi.as<int32_t>(),
                            c->range_lo.as<int32_t>());
@@ -1083,6 +1131,9 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         for (int64_t i = 0; i < n; ++i) order_out[i] = tmp[(size_t)i];
     }
     if ((flags & PMX_BUILD_WRITEBACK) && p->mem == PMX_MEM_HOST) {
+        PMX_TRY(ensure_cols64(c));
+        cols64 = c->cols64.as<double>();
+        PMX_HIP(c, hipStreamSynchronize(s));
         void* outp[24] = {p->x, p->y, p->z, p->m, p->h, p->rho};
         for (int j = 0; j < p->ncol; ++j) outp[6 + j] = p->quant[j];
         if (p->dtype == PMX_F64) {
@@ -1121,12 +1172,24 @@ namespace pmx {
 // Device address of the NaN-coordinate counter of the context's last build.
 // The plain FP64 path's structures, built on first use after a build: Leaf64 records and the
 // FP64 node boxes (a second bottom-up refit; the build refits the FP32 tree only).
+int ensure_cols64(pmx_ctx* c) {
+    if (!c->tree.f32in || c->tree.cols64_ready) return PMX_OK;
+    const int64_t n = c->tree.n;
+    const int ncols = 6 + c->tree.ncol;
+    PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
+    hipLaunchKernelGGL(k_gather_cols64, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, c->stream,
+                       c->records.as<float>(), ncols, c->order.as<int32_t>(), n, c->cols64.as<double>());
+    PMX_HIP(c, hipGetLastError());
+    c->tree.cols64_ready = true;
+    return PMX_OK;
+}
+
 int ensure_leaf64(pmx_ctx* c) {
     if (c->tree.leaf64_ready) return PMX_OK;
+    PMX_TRY(ensure_cols64(c));
     const int64_t n = c->tree.n;
     if (n >= 2) {
         const unsigned g = grid_for(n, kBlock, 16384);
-        hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, c->stream, c->visited.as<uint32_t>(), n + 2, 0u);
         PMX_TRY(refit_rounds(c, n, g, true));
     }
     PMX_TRY(ensure(c, c->leaf64, sizeof(Leaf64) * (size_t)std::max<int64_t>(n, 1)));

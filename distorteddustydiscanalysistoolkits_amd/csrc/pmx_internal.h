@@ -64,6 +64,10 @@ struct Tree {
     bool built = false;
     bool f32_ready = false;
     bool leaf64_ready = false;    // plain FP64 path: Leaf64 records and FP64 node boxes, built on first use
+    // FP32 input: the FP32 leaf and payload arrays hold every input value exactly, so the build
+    // writes no FP64 column copy; cols64 is gathered from the packed records on first use
+    bool f32in = false;
+    bool cols64_ready = false;
 };
 
 }  // namespace pmx
@@ -237,6 +241,8 @@ int sort_queries(pmx_ctx* c, const double* x, const double* y, const double* z, 
                  const LineWork* ws = nullptr);
 // Leaf64 records of the current tree from cols64 (the plain FP64 path packs them on first use).
 int ensure_leaf64(pmx_ctx* c);
+// cols64 of the current tree (a gather of the packed input records for an FP32 input).
+int ensure_cols64(pmx_ctx* c);
 int interp_points_f32c(pmx_ctx* c, int strategy, const CatChunk& cat, const int32_t* perm,
                        const double* qx, const double* qy, const double* qz, int64_t npts, float* out, int64_t ldo);
 int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, const int32_t* perm,

@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kWave) k_points32_nb(T32 T, double Kv, const d
             if (ha_in) ha = valid ? ha_in[qi] : 0.0;
             else {
                 const int64_t best = nearest32(T, q, P);
-                ha = best < 0 ? __builtin_nan("") : T.cols64[4 * n + best];
+                ha = best < 0 ? __builtin_nan("") : h64(T, best);
             }
             const double r = Kv * ha;
             radius2_64 = r * r;
@@ -228,11 +228,12 @@ __device__ inline LineLane line_lane(const double* o, const double* dvec, int64_
 
 // Exact reference line-leaf d2 in FP64 (_squared_distance_line_coords).
 __device__ inline double d2_line_exact64(const double o[3], const double dv[3], const T32& T, int64_t li) {
-    const int64_t n = T.n;
+    double px, py, pz;
+    pos64(T, li, px, py, pz);
     double D2 = 0.0, Dm = 0.0, d;
-    d = T.cols64[li] - o[0]; D2 += d * d; Dm += d * dv[0];
-    d = T.cols64[n + li] - o[1]; D2 += d * d; Dm += d * dv[1];
-    d = T.cols64[2 * n + li] - o[2]; D2 += d * d; Dm += d * dv[2];
+    d = px - o[0]; D2 += d * d; Dm += d * dv[0];
+    d = py - o[1]; D2 += d * d; Dm += d * dv[1];
+    d = pz - o[2]; D2 += d * d; Dm += d * dv[2];
     return D2 - Dm * Dm;
 }
 
@@ -290,7 +291,7 @@ __device__ inline bool accept_line(const T32& T, const LineLane& q, float d2, fl
     if (ballot(band)) {
         if (band) {
             const double e = d2_line_exact64(q.o64, q.d64, T, li);
-            const double rr = Kv * T.cols64[4 * T.n + li];
+            const double rr = Kv * h64(T, li);
             acc = e <= rr * rr;
         }
     }
@@ -338,7 +339,7 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
             if (lane < cnt) {
                 const int64_t li = lo + lane;
                 Y.w64[lane] = T.w64[li];
-                Y.invh64[lane] = 1.0 / T.cols64[4 * n + li];
+                Y.invh64[lane] = 1.0 / h64(T, li);
                 for (int f = 0; f < cat.n; ++f) Y.f[f][lane] = T.pay[(int64_t)cat.fcol[f] * n + li];
             }
             __syncthreads();
@@ -495,7 +496,9 @@ T32 tree32(pmx_ctx* c, int bucket) {
     T.leaf = c->leaf32.as<float4>();
     T.leaflo = c->leaf32.as<float4>() + c->tree.n;
     T.leaf_esc = c->leaf_esc.as<int32_t>();
-    T.cols64 = c->cols64.as<double>();
+    T.cols64 = c->tree.f32in ? nullptr : c->cols64.as<double>();
+    T.f32in = c->tree.f32in ? 1 : 0;
+    T.ncol = c->tree.ncol;
     T.pay = c->pay32.as<float>();
     T.w64 = reinterpret_cast<const double*>(c->pay32.as<float>() + payf + (payf & 1));
     T.ih64 = T.w64 + c->tree.n;

@@ -161,7 +161,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 const double Ru = Kv * (double)H.w * (1.0 + 1e-6);
                 const double Rpu = Ru + f.sqrt_band + f.pad;
                 if (!(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1)) {
-                    const double hb = T.cols64[4 * n + p];
+                    const double hb = T.f32in ? (double)H.w : T.cols64[4 * n + p];
                     const double rr = Kv * hb;
                     const double r2 = rr * rr;
                     const double Rp = rr + f.sqrt_band + f.pad;
@@ -240,10 +240,12 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                     if (!ok) {
                         // reference formula on the FP64 columns (_squared_distance_line_coords)
                         const int64_t li = perm[C0 + kk];
+                        double px, py, pz;
+                        pos64(T, pj, px, py, pz);
                         double D2 = 0.0, Dm = 0.0, e;
-                        e = T.cols64[pj] - o[li]; D2 += e * e; Dm += e * f.d[0];
-                        e = T.cols64[n + pj] - o[nl + li]; D2 += e * e; Dm += e * f.d[1];
-                        e = T.cols64[2 * n + pj] - o[2 * nl + li]; D2 += e * e; Dm += e * f.d[2];
+                        e = px - o[li]; D2 += e * e; Dm += e * f.d[0];
+                        e = py - o[nl + li]; D2 += e * e; Dm += e * f.d[1];
+                        e = pz - o[2 * nl + li]; D2 += e * e; Dm += e * f.d[2];
                         ok = D2 - Dm * Dm <= r2;
                     }
                     if (!ok) continue;

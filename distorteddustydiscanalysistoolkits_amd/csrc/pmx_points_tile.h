@@ -160,11 +160,12 @@ __device__ inline float d2_box32(const PointLane& q, const Node32& b) {
 
 // Exact reference point-leaf d2 in FP64 (_squared_distance_point_coords).
 __device__ inline double d2_exact64(const double p[3], const T32& T, int64_t li) {
-    const int64_t n = T.n;
+    double px, py, pz;
+    pos64(T, li, px, py, pz);
     double s = 0.0, d;
-    d = p[0] - T.cols64[li]; s += d * d;
-    d = p[1] - T.cols64[n + li]; s += d * d;
-    d = p[2] - T.cols64[2 * n + li]; s += d * d;
+    d = p[0] - px; s += d * d;
+    d = p[1] - py; s += d * d;
+    d = p[2] - pz; s += d * d;
     return s;
 }
 
@@ -316,7 +317,7 @@ __device__ inline bool accept_point(const T32& T, const PointLane& q, float d2, 
     if (ballot(band)) {
         if (band) {
             const double e = d2_exact64(q.p64, T, li);
-            acc = e <= radius2_rule<S>(Kv, radius2_64, T.cols64[4 * T.n + li]);
+            acc = e <= radius2_rule<S>(Kv, radius2_64, h64(T, li));
         }
     }
     return acc;
@@ -508,7 +509,7 @@ struct PointWalk<K, S, false> {
     __device__ void setup(int64_t nearest) {
         double invha64 = 0.0;
         if (S != kScatter) {
-            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * T.n + nearest];
+            const double ha64 = nearest < 0 ? __builtin_nan("") : h64(T, nearest);
             const double radius = Kv * ha64;
             radius2_64 = radius * radius;
             radius2_32 = (float)radius2_64;
@@ -632,7 +633,7 @@ struct PointWalk<K, S, true> {
     __device__ void setup(int64_t nearest) {
         double invha64 = 0.0;
         if (S != kScatter) {
-            const double ha64 = nearest < 0 ? __builtin_nan("") : T.cols64[4 * T.n + nearest];
+            const double ha64 = nearest < 0 ? __builtin_nan("") : h64(T, nearest);
             const double radius = Kv * ha64;
             radius2_64 = radius * radius;
             radius2_32 = (float)radius2_64;
@@ -741,7 +742,7 @@ struct PointWalk<K, S, true> {
             Y.ih64[lane] = T.ih64[li];
             for (int f = 0; f < cat.ns; ++f) Y.s[f * kMaxBucket + lane] = T.pay[(int64_t)cat.scol[f] * n + li];
             for (int f = 0; f < cat.nder; ++f)
-                Y.d[f * kMaxBucket + lane] = T.cols64[(int64_t)cat.dcol[f] * n + li];
+                Y.d[f * kMaxBucket + lane] = col64(T, cat.dcol[f], li);
         }
         __syncthreads();
         const uint64_t cand = ballot(lane < cnt && reaches_tile<S>(box, P, lane, Kvf));

@@ -13,14 +13,41 @@ struct T32 {
     const float4* __restrict__ leaf;
     const float4* __restrict__ leaflo;
     const int32_t* __restrict__ leaf_esc;
-    const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant
+    const double* __restrict__ cols64;     // FP64 x | y | z | m | h | rho | quant (FP64 input only)
     const float* __restrict__ pay;         // m/rho | quant... | rho
     const double* __restrict__ w64;        // m/rho in FP64
     const double* __restrict__ ih64;       // 1/h in FP64
     int64_t n;
     int32_t root;
     int32_t bucket;
+    int32_t f32in;                         // FP32 input: FP64 values come from leaf / pay exactly
+    int32_t ncol;                          // quant columns
 };
+
+// FP64 particle values in Morton order.  An FP32 input's values are exact in the FP32 leaf
+// (x y z h, residual zero) and payload (quant..., rho) arrays; an FP64 input reads cols64.
+// The branch is on a kernel argument (wavefront-uniform).
+__device__ inline void pos64(const T32& T, int64_t p, double& x, double& y, double& z) {
+    if (T.f32in) {
+        const float4 H = T.leaf[p];
+        x = (double)H.x;
+        y = (double)H.y;
+        z = (double)H.z;
+    } else {
+        x = T.cols64[p];
+        y = T.cols64[T.n + p];
+        z = T.cols64[2 * T.n + p];
+    }
+}
+__device__ inline double h64(const T32& T, int64_t p) {
+    return T.f32in ? (double)T.leaf[p].w : T.cols64[4 * T.n + p];
+}
+// Column c = 5 (rho) or 6 + j (quant j).
+__device__ inline double col64(const T32& T, int c, int64_t p) {
+    if (!T.f32in) return T.cols64[(int64_t)c * T.n + p];
+    const int64_t row = c == 5 ? 1 + T.ncol : c - 5;      // payload: m/rho | quant... | rho
+    return (double)T.pay[row * T.n + p];
+}
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 // Barrier of one wavefront over its LDS traffic: a wavefront's LDS operations complete in
