@@ -52,8 +52,8 @@ void throw_test_fault() {
 // Knobs of a new context (see Tuning); out-of-range values are clamped.
 void read_tuning(Tuning& t) {
     if (const char* v = std::getenv("PMX_LINES_TILED")) t.lines_tiled = std::atoi(v) != 0;
-    if (const char* v = std::getenv("PMX_BUCKET_POINTS")) t.bucket_points = std::max(1, std::min(64, std::atoi(v)));
-    if (const char* v = std::getenv("PMX_BUCKET_LINES")) t.bucket_lines = std::max(1, std::min(64, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_BUCKET_POINTS")) t.bucket_points = std::max(kMinRun, std::min(64, std::atoi(v)));
+    if (const char* v = std::getenv("PMX_BUCKET_LINES")) t.bucket_lines = std::max(kMinRun, std::min(64, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_BUCKET")) t.tiled_bucket = std::max(1, std::min(1 << 16, std::atoi(v)));
     if (const char* v = std::getenv("PMX_TILED_NEAR")) t.tiled_near = std::max(0, std::min(1 << 20, std::atoi(v)));
     if (const char* v = std::getenv("PMX_REFIT_SMALL")) t.refit_small = std::max(1, std::min(1024, std::atoi(v)));

@@ -639,7 +639,7 @@ int pmx_batch_tune(pmx_batch* b) {
         };
         Knob knobs[] = {{&b->waves_per_cu, {4, 8, 16, 32}},
                         {&b->bucket_lines, {128, 256, 512, 1024}},
-                        {&b->bucket_points, {8, 16, 32, 64}},
+                        {&b->bucket_points, {16, 32, 64}},
                         {&b->heavy_first, {0, 1}},
                         {&b->split_budget, {4096, 8192, 16384}}};
         double best_ms = 0;

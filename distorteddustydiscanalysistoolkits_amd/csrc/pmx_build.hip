@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "pmx_internal.h"
 
@@ -478,6 +479,24 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
     }
 }
 
+// The reference's in-place permute! of the input columns (InterpolationInput.jl:211-219) for
+// columns in device memory: column c of the caller becomes rec[order[i]][c].  The records are a
+// copy of the columns, so the columns can be overwritten in place.
+struct ColOut {
+    void* p[24];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_writeback(const T* __restrict__ rec, int ncols,
+                                                      const int32_t* __restrict__ order, int64_t n, ColOut dst) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int32_t o = order[i];
+        o = ((uint64_t)(uint32_t)o < (uint64_t)n) ? o : 0;
+        const T* r = rec + (int64_t)o * ncols;
+        for (int c = 0; c < ncols; ++c) static_cast<T*>(dst.p[c])[i] = r[c];
+    }
+}
+
 // FP64 columns of an FP32 input in Morton order, from the packed records.
 __global__ void __launch_bounds__(kBlock) k_gather_cols64(const float* __restrict__ rec, int ncols,
                                                           const int32_t* __restrict__ order, int64_t n,
@@ -506,52 +525,74 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(const double* __restrict__
 // Karras radix tree (BinaryRadixTree.jl:81-281).  Unified ids: internal 1..n-1, leaf k
 // (1-based) -> n-1+k.  All index arithmetic below is 1-based as in the reference.
 // ---------------------------------------------------------------------------------
-__device__ inline int delta(const uint64_t* __restrict__ c, int64_t i, int64_t j) {
-    // _longest_common_prefix_length(codes, i, j), MortonCode.jl:380-387 (1-based)
-    uint64_t a = c[i - 1], b = c[j - 1];
-    if (a == b) return __clzll((uint64_t)i ^ (uint64_t)j);
-    return __clzll(a ^ b);
-}
-
 // _build_child! (BinaryRadixTree.jl:209-243) with _find_range (:148-179) and
-// _split_position (:181-202).  lr holds left at 2*id and right at 2*id+1.
+// _split_position (:181-202), and the internal escape of Algorithm 2 of Prokopenko &
+// Lebrun-Grandie (_build_adjacent! + escapes, BinaryRadixTree.jl:204-260): with
+// adj[k] = delta(k, k + 1) (typemax(Int32) at k = n), node i's escape follows from the two
+// adjacency values around the end of its range, computed here from the codes directly.
+// lr holds left at 2*id and right at 2*id+1.
+// A workgroup's nodes read codes near their own ids (ranges of small nodes, the first steps of
+// the searches): a window of codes around the workgroup's ids is staged in LDS, and reads
+// outside it go to global memory.
+constexpr int kBrtPad = 128;
+
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
-                                                int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n - 1;
-         i += (int64_t)gridDim.x * kBlock) {
-        int dl = (i > 1) ? delta(c, i, i - 1) : -1;
-        int dr = (i < n) ? delta(c, i, i + 1) : -1;
+                                                int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo,
+                                                int32_t* __restrict__ escape) {
+    __shared__ uint64_t win[kBlock + 2 * kBrtPad];
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t m = n - 1;                                  // internal nodes 1 .. n-1
+    const int64_t iters = (m + stride - 1) / stride;
+    for (int64_t itr = 0; itr < iters; ++itr) {
+        const int64_t i0 = 1 + itr * stride + (int64_t)blockIdx.x * kBlock;   // first id of the block
+        // window: codes of 1-based indices w0 .. w1-1
+        const int64_t w0 = i0 - kBrtPad > 1 ? i0 - kBrtPad : 1;
+        const int64_t w1 = i0 + kBlock + kBrtPad <= n + 1 ? i0 + kBlock + kBrtPad : n + 1;
+        __syncthreads();
+        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) win[k - w0] = c[k - 1];
+        __syncthreads();
+        auto code = [&](int64_t j) -> uint64_t { return (j >= w0 && j < w1) ? win[j - w0] : c[j - 1]; };
+        // _longest_common_prefix_length(codes, i, j), MortonCode.jl:380-387 (1-based)
+        auto delta = [&](int64_t a, int64_t b) -> int {
+            const uint64_t ca = code(a), cb = code(b);
+            if (ca == cb) return __clzll((uint64_t)a ^ (uint64_t)b);
+            return __clzll(ca ^ cb);
+        };
+        const int64_t i = i0 + threadIdx.x;
+        if (i > n - 1) continue;
+        int dl = (i > 1) ? delta(i, i - 1) : -1;
+        int dr = (i < n) ? delta(i, i + 1) : -1;
         int64_t d = (dr - dl > 0) ? 1 : ((dr - dl < 0) ? -1 : 1);
-        int dmin = (1 <= i - d && i - d <= n) ? delta(c, i, i - d) : -1;
+        int dmin = (1 <= i - d && i - d <= n) ? delta(i, i - d) : -1;
         int64_t lmax = 2;
         for (;;) {
             int64_t j = i + lmax * d;
             if (j < 1 || j > n) break;
-            if (delta(c, i, j) <= dmin) break;
+            if (delta(i, j) <= dmin) break;
             lmax *= 2;
         }
         int64_t l = 0;
         for (int64_t t = lmax >> 1; t > 0; t >>= 1) {
             int64_t j = i + (l + t) * d;
-            if (j >= 1 && j <= n && delta(c, i, j) > dmin) l += t;
+            if (j >= 1 && j <= n && delta(i, j) > dmin) l += t;
         }
         int64_t j = i + l * d;
         int64_t lo = i < j ? i : j, hi = i < j ? j : i;
         range_hi[i] = (int32_t)hi;
         range_lo[i] = (int32_t)lo;
         int64_t split;
-        if (c[lo - 1] == c[hi - 1]) {
+        if (code(lo) == code(hi)) {
             split = (lo + hi) >> 1;
             split = split < lo ? lo : (split > hi - 1 ? hi - 1 : split);
         } else {
-            int prefix = delta(c, lo, hi);
+            int prefix = delta(lo, hi);
             split = lo;
             int64_t step = hi - lo;
             while (step > 1) {
                 step = (step + 1) >> 1;
                 int64_t ns = split + step;
-                if (ns < hi && delta(c, lo, ns) > prefix) split = ns;
+                if (ns < hi && delta(lo, ns) > prefix) split = ns;
             }
         }
         const int64_t off = n - 1;
@@ -561,29 +602,12 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         lr[2 * i + 1] = (int32_t)right;
         parent[left] = (int32_t)i;
         parent[right] = (int32_t)i;
-    }
-}
-
-// _build_adjacent! (BinaryRadixTree.jl:204-207); adj[n] = typemax(Int32).
-__global__ void __launch_bounds__(kBlock) k_adj(const uint64_t* __restrict__ c, int64_t n, int32_t* __restrict__ adj) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n; i += (int64_t)gridDim.x * kBlock)
-        adj[i] = (i == n) ? 2147483647 : delta(c, i, i + 1);
-}
-
-// Internal escapes, Algorithm 2 of Prokopenko & Lebrun-Grandie (BinaryRadixTree.jl:245-260).
-__global__ void __launch_bounds__(kBlock) k_escape_internal(const int32_t* __restrict__ range_hi,
-                                                            const int32_t* __restrict__ adj, int64_t n,
-                                                            int32_t* __restrict__ escape) {
-    const int64_t off = n - 1;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x + 1; i <= n - 1;
-         i += (int64_t)gridDim.x * kBlock) {
-        int64_t hi = range_hi[i];
         int32_t e;
         if (hi == n) e = 0;
         else if (hi == n - 1) e = (int32_t)(off + n);
         else {
-            int64_t next = hi + 1;
-            e = (adj[next] < adj[next - 1]) ? (int32_t)(off + next) : (int32_t)next;
+            const int64_t next = hi + 1;     // adj[next] = delta(next, next + 1), next + 1 <= n
+            e = (delta(next, next + 1) < delta(next - 1, next)) ? (int32_t)(off + next) : (int32_t)next;
         }
         escape[i] = e;
     }
@@ -733,77 +757,159 @@ __device__ inline void wave_append(bool take, int32_t p, int32_t* __restrict__ l
 // at the number of children that are already complete; nodes with both children complete
 // form the first list of the bottom-up rounds.
 
-// Block-wide version of wave_append: one atomic per workgroup of kBlock threads (the list
-// counter is a single address that every workgroup increments).  Call from every thread.
-__device__ inline void block_append(bool take, int32_t p, int32_t* __restrict__ list, uint32_t* __restrict__ count) {
-    __shared__ uint32_t wave_cnt[kBlock / 64];
-    __shared__ uint32_t base_all;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t m = __ballot(take);
-    if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(m);
+// Block-wide appends buffered in LDS: the ids a workgroup appends over its grid-stride loop are
+// compacted into an LDS buffer and flushed to the global list with one atomic per flush.  One
+// device-scope atomic on a single address costs about 11 ns serialised (measured: 62500
+// workgroups with one atomicAdd each on one counter take 0.71 ms, spread over 64 cache lines
+// 0.02 ms), so an atomic per workgroup and iteration would dominate the refit.
+constexpr int kAppendCap = 2048;
+struct BlockAppender {
+    int32_t* buf;        // LDS, kAppendCap ids
+    uint32_t* n;         // LDS, ids held
+    uint32_t* wave_cnt;  // LDS, kBlock / 64 counts
+    uint32_t* base;      // LDS, flush destination
+};
+
+__device__ inline void appender_flush(BlockAppender& A, int32_t* __restrict__ list, uint32_t* __restrict__ count) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int w = 0; w < kBlock / 64; ++w) tot += wave_cnt[w];
-        base_all = tot ? atomicAdd(count, tot) : 0u;
-    }
+    const uint32_t held = *A.n;
+    if (held == 0) return;
+    if (threadIdx.x == 0) *A.base = atomicAdd(count, held);
     __syncthreads();
-    uint32_t off = base_all;
-    for (int w = 0; w < wave; ++w) off += wave_cnt[w];
-    if (take) list[off + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = p;
+    const uint32_t b = *A.base;
+    for (uint32_t k = threadIdx.x; k < held; k += kBlock) list[b + k] = A.buf[k];
+    __syncthreads();
+    if (threadIdx.x == 0) *A.n = 0;
     __syncthreads();
 }
 
+// Call from every thread of the workgroup.
+__device__ inline void appender_push(BlockAppender& A, bool take, int32_t p, int32_t* __restrict__ list,
+                                     uint32_t* __restrict__ count) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(take);
+    if (lane == 0) A.wave_cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = *A.n, tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t c = A.wave_cnt[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    if (take) A.buf[off + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) *A.n += tot;
+    __syncthreads();
+    if (*A.n > (uint32_t)(kAppendCap - kBlock)) appender_flush(A, list, count);
+}
+
+#define PMX_BLOCK_APPENDER(A)                                 \
+    __shared__ int32_t A##_buf[kAppendCap];                   \
+    __shared__ uint32_t A##_n, A##_wc[kBlock / 64], A##_base; \
+    if (threadIdx.x == 0) A##_n = 0;                          \
+    __syncthreads();                                          \
+    BlockAppender A{A##_buf, &A##_n, A##_wc, &A##_base}
+
 constexpr int kRefitPad = 64;   // largest leaf range of a node refit directly (staged window margin)
 
+// min_run > 0 (FP32 tree only): a node whose parent covers at most min_run leaves gets no
+// record.  Every FP32 walk hands over a node of at most min_run leaves as one leaf run, so it
+// never reads the children of such a node; the rounds above read only children of nodes larger
+// than small_range >= min_run.
+// F32: FP32 input and FP32 tree only; the leaf window is the exact FP32 leaves (16 bytes each) and
+// the boxes are FP32 min / max, which equal the outward rounding of the FP64 min / max.
+template <bool F32>
 __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64,
                                                         const float4* __restrict__ leaf32, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
+                                                        const int32_t* __restrict__ parent,
                                                         const int32_t* __restrict__ range_lo,
                                                         const int32_t* __restrict__ range_hi, int64_t n,
                                                         uint32_t* __restrict__ visited, Node64* nodes,
-                                                        Node32* __restrict__ n32, int small_range,
+                                                        Node32* __restrict__ n32, int small_range, int min_run,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
     // Internal node i covers leaves with one end at i - 1 or i, so the block's nodes read a
     // window of leaves around its own ids: staged once in LDS, then read by every node.
     constexpr int kPad = kRefitPad;
-    __shared__ Leaf64 win[kBlock + 2 * kPad + 1];    // leaves i0 - 1 - kPad .. i0 + kBlock + kPad - 1
+    constexpr int kWin = kBlock + 2 * kPad + 1;      // leaves i0 - 1 - kPad .. i0 + kBlock + kPad - 1
+    using WinT = typename std::conditional<F32, float4, Leaf64>::type;
+    __shared__ WinT win[kWin];
+    PMX_BLOCK_APPENDER(app);
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                            // internal nodes 1 .. n-1
     const int64_t iters = (m + stride - 1) / stride;    // uniform trip count keeps ballots convergent
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t i0 = 1 + it * stride + (int64_t)blockIdx.x * kBlock;
+        const int64_t i = i0 + threadIdx.x;
+        // the node's own entries first (independent loads, in flight during the staging)
+        int32_t lo = 0, hi = -1, pa = 0, lch = 0, rch = 0, esc = 0;
+        if (i < n) {
+            lo = range_lo[i];
+            hi = range_hi[i];
+            lch = lr[2 * i];
+            rch = lr[2 * i + 1];
+            esc = escape[i];
+            if (min_run > 0) pa = parent[i];
+        }
         const int64_t w0 = i0 - 1 - kPad > 0 ? i0 - 1 - kPad : 0;
         const int64_t w1 = i0 + kBlock + kPad < n ? i0 + kBlock + kPad : n;
         __syncthreads();
-        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) win[k - w0] = read_leaf(cols64, leaf32, n, k);
+        for (int64_t k = w0 + threadIdx.x; k < w1; k += kBlock) {
+            if constexpr (F32) win[k - w0] = leaf32[k];
+            else win[k - w0] = read_leaf(cols64, leaf32, n, k);
+        }
         __syncthreads();
-        const int64_t i = i0 + threadIdx.x;
         bool take = false;
         if (i < n) {
-            const int32_t lo = range_lo[i], hi = range_hi[i];
-            if (lo >= 1 && hi <= n && hi - lo + 1 <= small_range) {
-                double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-                double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-                double hm = -__builtin_inf();
+            bool unread = false;
+            if (min_run > 0 && hi - lo + 1 <= min_run && pa > 0 && pa < n)
+                unread = range_hi[pa] - range_lo[pa] + 1 <= min_run;
+            if (unread) {
+                // no record: no FP32 walk reaches this node
+            } else if (lo >= 1 && hi <= n && hi - lo + 1 <= small_range) {
                 // small_range <= kPad: the leaf range lies inside the staged window
-                for (int32_t k = lo - 1; k < hi; ++k) {
-                    const Leaf64 L = win[k - w0];
-                    mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
-                    mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
-                    mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];
-                    hm = L.h > hm ? L.h : hm;
+                if constexpr (F32) {
+                    float4 a = win[lo - 1 - w0];
+                    float mn0 = a.x, mn1 = a.y, mn2 = a.z, mx0 = a.x, mx1 = a.y, mx2 = a.z, hm = a.w;
+                    for (int32_t k = lo; k < hi; ++k) {
+                        const float4 L = win[k - w0];
+                        mn0 = fminf(mn0, L.x); mx0 = fmaxf(mx0, L.x);
+                        mn1 = fminf(mn1, L.y); mx1 = fmaxf(mx1, L.y);
+                        mn2 = fminf(mn2, L.z); mx2 = fmaxf(mx2, L.z);
+                        hm = fmaxf(hm, L.w);
+                    }
+                    Node32 b;
+                    b.mnx = mn0; b.mny = mn1; b.mnz = mn2; b.hmax = hm;
+                    b.mxx = mx0; b.mxy = mx1; b.mxz = mx2;
+                    b.left = lch;
+                    b.escape = esc;
+                    b.lo = lo - 1;
+                    b.cnt = hi - lo + 1;
+                    b.pad = 0;
+                    n32[i] = b;
+                } else {
+                    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+                    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+                    double hm = -__builtin_inf();
+                    for (int32_t k = lo - 1; k < hi; ++k) {
+                        const Leaf64 L = win[k - w0];
+                        mn[0] = L.x < mn[0] ? L.x : mn[0]; mx[0] = L.x > mx[0] ? L.x : mx[0];
+                        mn[1] = L.y < mn[1] ? L.y : mn[1]; mx[1] = L.y > mx[1] ? L.y : mx[1];
+                        mn[2] = L.z < mn[2] ? L.z : mn[2]; mx[2] = L.z > mx[2] ? L.z : mx[2];
+                        hm = L.h > hm ? L.h : hm;
+                    }
+                    Node64 out;
+                    for (int k = 0; k < 3; ++k) { out.mn[k] = mn[k]; out.mx[k] = mx[k]; }
+                    out.hmax = hm;
+                    out.left = lch;
+                    out.escape = esc;
+                    store_node(out, i, range_lo, range_hi, nodes, n32);
                 }
-                Node64 out;
-                for (int k = 0; k < 3; ++k) { out.mn[k] = mn[k]; out.mx[k] = mx[k]; }
-                out.hmax = hm;
-                out.left = lr[2 * i];
-                out.escape = escape[i];
-                store_node(out, i, range_lo, range_hi, nodes, n32);
             } else {
                 uint32_t done = 0;
+                const int32_t chs[2] = {lch, rch};
                 for (int side = 0; side < 2; ++side) {
-                    const int32_t ch = lr[2 * i + side];
+                    const int32_t ch = chs[side];
                     if (ch >= n) done += 1;
                     else if (ch > 0 && range_hi[ch] - range_lo[ch] + 1 <= small_range) done += 1;
                 }
@@ -811,8 +917,9 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                 take = done == 2;
             }
         }
-        block_append(take, (int32_t)i, next, next_count);
+        appender_push(app, take, (int32_t)i, next, next_count);
     }
+    appender_flush(app, next, next_count);
 }
 
 // One bottom-up round: complete every node of the list from its two complete children, then
@@ -833,7 +940,7 @@ struct RefitTree {
 
 template <bool BLOCK>
 __device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int32_t* __restrict__ next,
-                                  uint32_t* __restrict__ next_count) {
+                                  uint32_t* __restrict__ next_count, BlockAppender* app = nullptr) {
     const int64_t n = R.n;
     bool take = false;
     int32_t p = 0;
@@ -844,20 +951,22 @@ __device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int
         p = parent[x];
         if (p > 0 && p < n) take = atomicAdd(&visited[p], 1u) == 1u;
     }
-    if (BLOCK) block_append(take, p, next, next_count);
+    if (BLOCK) appender_push(*app, take, p, next, next_count);
     else wave_append(take, p, next, next_count);
 }
 
 __global__ void __launch_bounds__(kBlock) k_refit_round(RefitTree R, const int32_t* __restrict__ cur,
                                                         const uint32_t* __restrict__ cur_count,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
+    PMX_BLOCK_APPENDER(app);
     const int64_t count = *cur_count;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t k = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        refit_step<true>(R, k < count ? cur[k] : 0, k < count, next, next_count);
+        refit_step<true>(R, k < count ? cur[k] : 0, k < count, next, next_count, &app);
     }
+    appender_flush(app, next, next_count);
 }
 
 // The remaining rounds (short lists near the root) in one workgroup: rounds are separated by
@@ -958,9 +1067,15 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     R.visited = c->visited.as<uint32_t>();
     R.nodes = fp64 ? c->nodes64.as<Node64>() : nullptr;
     R.n32 = c->nodes32.as<Node32>();
-    const unsigned gs = grid_for(n, kBlock, 1 << 20);      // one internal node per thread
-    hipLaunchKernelGGL(k_refit_small, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape, R.rlo, R.rhi, n,
-                       R.visited, R.nodes, R.n32, std::min(c->tune.refit_small, kRefitPad), listA, &counts[0]);
+    const unsigned gs = grid_for(n, kBlock, 8 * (int64_t)c->ncu);   // grid-stride, appends buffered per workgroup
+    const int small = std::max(kMinRun, std::min(c->tune.refit_small, kRefitPad));
+    if (!fp64 && R.leaf32)
+        hipLaunchKernelGGL(k_refit_small<true>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
+                           R.parent, R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, kMinRun, listA, &counts[0]);
+    else
+        hipLaunchKernelGGL(k_refit_small<false>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
+                           R.parent, R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, fp64 ? 0 : kMinRun, listA,
+                           &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
         int32_t* cur = (round & 1) ? listB : listA;
@@ -1082,7 +1197,6 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->escape, 4 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->range_hi, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->range_lo, 4 * (size_t)(n + 2)));
-    PMX_TRY(ensure(c, c->adj, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->visited, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->leaf_esc, 4 * (size_t)n));
     PMX_TRY(ensure(c, c->nodes64, sizeof(Node64) * (size_t)(n + 1)));
@@ -1094,11 +1208,8 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     // every node the bottom-up rounds count at
     hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
-        hipLaunchKernelGGL(k_brt, dim3(g), dim3(kBlock), 0, s, codes, n, lr, parent, c->range_hi.as<int32_t>(),
-                           c->range_lo.as<int32_t>());
-        hipLaunchKernelGGL(k_adj, dim3(g), dim3(kBlock), 0, s, codes, n, c->adj.as<int32_t>());
-        hipLaunchKernelGGL(k_escape_internal, dim3(g), dim3(kBlock), 0, s, c->range_hi.as<int32_t>(),
-                           c->adj.as<int32_t>(), n, escape);
+        hipLaunchKernelGGL(k_brt, dim3(grid_for(n, kBlock, 1 << 20)), dim3(kBlock), 0, s, codes, n, lr, parent,
+                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape);
     }
     hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
     if (n >= 2) {
@@ -1129,6 +1240,18 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         std::vector<int32_t> tmp((size_t)n);
         PMX_HIP(c, hipMemcpy(tmp.data(), order, 4 * (size_t)n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) order_out[i] = tmp[(size_t)i];
+    }
+    if ((flags & PMX_BUILD_WRITEBACK) && p->mem == PMX_MEM_DEVICE) {
+        ColOut dst;
+        for (int j = 0; j < ncols; ++j) dst.p[j] = const_cast<void*>(in[j]);
+        if (p->dtype == PMX_F64)
+            hipLaunchKernelGGL(k_writeback<double>, dim3(g), dim3(kBlock), 0, s, c->records.as<double>(), ncols,
+                               order, n, dst);
+        else
+            hipLaunchKernelGGL(k_writeback<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
+                               order, n, dst);
+        PMX_HIP(c, hipGetLastError());
+        PMX_HIP(c, hipStreamSynchronize(s));
     }
     if ((flags & PMX_BUILD_WRITEBACK) && p->mem == PMX_MEM_HOST) {
         PMX_TRY(ensure_cols64(c));

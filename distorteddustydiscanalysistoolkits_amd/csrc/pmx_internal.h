@@ -42,11 +42,16 @@ struct __align__(16) Node32 {
 
 // Launch-shape knobs of a context, read from the environment when it is created:
 // PMX_LINES_TILED=0 sends every line set through the union walk; PMX_BUCKET_POINTS /
-// PMX_BUCKET_LINES set the leaf-run size the sample-tile walks hand over at once (1..64);
+// PMX_BUCKET_LINES set the leaf-run size the sample-tile walks hand over at once (kMinRun..64);
 // PMX_TILED_BUCKET the leaf-run size of the line-map tile walk (1..65536); PMX_TILED_NEAR how
 // close (in cells) a node's box must be to the tile for the walk to read it whole;
 // PMX_REFIT_SMALL the leaf-run length up to which the refit computes a node directly from its
 // leaves.
+// Every FP32 walk hands over a node of at most kMinRun leaves as one leaf run (leaf-run knobs
+// are clamped to at least kMinRun), so the build writes no FP32 record for a node whose parent
+// covers at most kMinRun leaves: no walk reads it.
+constexpr int kMinRun = 16;
+
 struct Tuning {
     int lines_tiled = 1;
     int bucket_points = 64;
@@ -176,10 +181,12 @@ constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
                int64_t* order_out);
-// hist: the histogram buffer, 4 * 256 * (ceil(n / 4096) + 1) bytes (null: the context's).
+// Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), 8 bits per pass; the
+// result is in keys_alt / vals_alt when *result_in_alt.  hist: the histogram buffer of
+// radix_hist_bytes(n) bytes (null: the context's).
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
                      int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist = nullptr);
-inline size_t radix_hist_bytes(int64_t n) { return sizeof(uint32_t) * 256 * (size_t)((n + 4095) / 4096 + 1); }
+constexpr size_t radix_hist_bytes(int64_t n) { return sizeof(uint32_t) * 256 * (size_t)((n + 4095) / 4096 + 1); }
 
 // Device work memory of one line-map call.  The per-call entry points take it from the
 // context's grow-only buffers (ws == nullptr); the no-allocation entry point passes regions of

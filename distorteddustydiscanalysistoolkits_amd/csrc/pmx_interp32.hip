@@ -504,7 +504,7 @@ T32 tree32(pmx_ctx* c, int bucket) {
     T.ih64 = T.w64 + c->tree.n;
     T.n = c->tree.n;
     T.root = c->tree.root;
-    T.bucket = std::max(1, std::min(bucket, kMaxBucket));
+    T.bucket = std::max(kMinRun, std::min(bucket, kMaxBucket));
     return T;
 }
 

@@ -39,7 +39,7 @@ constexpr size_t kOffVals0 = kOffKeys0 + up256(8 * (size_t)kMaxLines);
 constexpr size_t kOffKeys1 = kOffVals0 + up256(4 * (size_t)kMaxLines);
 constexpr size_t kOffVals1 = kOffKeys1 + up256(8 * (size_t)kMaxLines);
 constexpr size_t kOffHist = kOffVals1 + up256(4 * (size_t)kMaxLines);
-constexpr size_t kHistBytes = sizeof(uint32_t) * 256 * (size_t)((kMaxLines + 4095) / 4096 + 1);
+constexpr size_t kHistBytes = radix_hist_bytes(kMaxLines);
 constexpr size_t kOffCells = kOffHist + up256(kHistBytes);
 constexpr size_t kCellsBytes = ((4 * (size_t)(kMaxCells + 1) + 15) & ~size_t(15)) + 16 * (size_t)kMaxLines;
 constexpr size_t kScratchBytes = kOffCells + up256(kCellsBytes);                // about 128 MiB

@@ -114,7 +114,7 @@ __device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& 
 }
 
 // FP32 tree view of the context's current LBVH (prepare_f32 must have run); `bucket` is the
-// largest leaf run a walk hands over in one piece (clamped to 1..64).
+// largest leaf run a walk hands over in one piece (clamped to kMinRun..64).
 T32 tree32(pmx_ctx* c, int bucket);
 
 }  // namespace pmx

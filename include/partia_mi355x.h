@@ -64,7 +64,7 @@ enum { PMX_PREC_F64 = 0, PMX_PREC_F32C = 1 };
  * batches (pmx_batch_*) always use 1 / h. */
 enum { PMX_LINE_INV_H2 = 0x100 };
 /* pmx_build_lbvh flags */
-enum { PMX_BUILD_WRITEBACK = 1 };   /* permute the caller's host columns into Morton order */
+enum { PMX_BUILD_WRITEBACK = 1 };   /* permute the caller's columns (host or device) into Morton order */
 
 /* ≙ InterpolationInput{3,T,V,K,NCOLUMN} (InterpolationInput.jl:43-51). */
 typedef struct pmx_particles {
@@ -140,8 +140,8 @@ const char *pmx_version(void);
  * the device and kept in the context for the interpolation calls.
  *   codes_out (n, nullable): de-duplicated sorted codes (enc.codes)
  *   order_out (n, nullable): 0-based original index of each sorted slot (enc.order - 1)
- *   flags & PMX_BUILD_WRITEBACK with host columns: the caller's x, y, z, m, h, rho and
- *   quant arrays are overwritten in Morton order, as the reference's permute! does. */
+ *   flags & PMX_BUILD_WRITEBACK: the caller's x, y, z, m, h, rho and quant arrays (host or
+ *   device memory) are overwritten in Morton order, as the reference's permute! does. */
 int pmx_build_lbvh(struct pmx_ctx *ctx, const pmx_particles *p, int32_t flags,
                    uint64_t *codes_out, int64_t *order_out);
 
@@ -185,7 +185,7 @@ int pmx_interpolate_lines_noalloc(struct pmx_ctx *ctx, int32_t strategy, int32_t
 /* --- drivers: build + interpolate in one call ------------------------------------- */
 /* ≙ Partia.PointSamples_interpolation(backend, grid, input, catalog, strategy)
  * (ext/CUDAExt/.../PointSamples_interpolation.jl:12-29): LBVH build (with write-back of
- * the Morton-ordered columns when they are host arrays), interpolation, synchronise. */
+ * the Morton-ordered columns into the caller's arrays), interpolation, synchronise. */
 int pmx_point_samples_interpolation(struct pmx_ctx *ctx, const pmx_particles *p,
                                     int32_t strategy, int32_t precision, const pmx_catalog *cat,
                                     const pmx_points *pts, const pmx_output *out);

@@ -3,8 +3,8 @@ empty and oversized inputs (BinaryRadixTree.jl:85-90), a NaN coordinate (UInt64(
 _quantize_coords, MortonCode.jl:473-498), a non-scatter line strategy
 (LineSamples_interpolation.jl:48-52), catalog slots outside the input's columns
 (InterpolationCatalog.jl:141-188), an exception inside an entry point (PMX_TEST_THROW) -- each
-leaving the context usable -- and the in-place Morton-order write-back of the caller's host
-columns (LinearBVH!, InterpolationInput.jl:211-219)."""
+leaving the context usable -- and the in-place Morton-order write-back of the caller's host and
+device columns (LinearBVH!, InterpolationInput.jl:211-219)."""
 import ctypes as C
 
 import numpy as np
@@ -107,3 +107,24 @@ def test_writeback_permutes_host_columns(ctx, dtype):
     # the permuted columns describe the same particles: a second build on them is the identity
     _, order2 = ctx.build(*[c.copy() for c in mine], quant=[q.copy() for q in mq], want_order=True)
     assert np.array_equal(order2, np.arange(order2.size))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_writeback_permutes_device_columns(ctx, dtype):
+    import torch
+    cols, quant = disc_columns(7000, seed=11, warped=True, dtype=dtype)
+    tree, ocols, oquant = oracle_tree(cols, quant)
+    dev = [torch.from_numpy(c.copy()).to("cuda:0") for c in cols]
+    dq = [torch.from_numpy(q.copy()).to("cuda:0") for q in quant]
+    ctx.build(*dev, quant=dq, writeback=True)
+    order = tree.order - 1
+    for got, orig, ref in zip(dev + dq, cols + quant, ocols + oquant):
+        g = got.cpu().numpy()
+        assert g.dtype == orig.dtype
+        assert np.array_equal(g, orig[order])
+        assert np.array_equal(g.astype(np.float64), ref)
+    # without the flag the caller's device columns are left as they are
+    dev2 = [torch.from_numpy(c.copy()).to("cuda:0") for c in cols]
+    ctx.build(*dev2, quant=[torch.from_numpy(q.copy()).to("cuda:0") for q in quant])
+    for got, orig in zip(dev2, cols):
+        assert np.array_equal(got.cpu().numpy(), orig)
