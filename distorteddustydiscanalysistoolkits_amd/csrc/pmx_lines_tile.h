@@ -149,7 +149,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         // sweep are dealt round-robin to the lanes (particles with large h do not hold the
         // other lanes idle), and a row's lines are one contiguous run of sorted lines.
         auto sweep = [&](int64_t p, bool valid) {
-            int nrow = 0;
+            int nrow = 0, ncolc = 0;
             if (valid) {
                 const float4 H = T.leaf[p];
                 const float4 Lo = T.leaflo[p];
@@ -174,6 +174,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                     const int cy0 = max(gy0 - cy_base, 0), cy1 = min(gy1 - cy_base, kTS - 1);
                     if (cx0 <= cx1 && cy0 <= cy1) {
                         nrow = cy1 - cy0 + 1;
+                        ncolc = cx1 - cx0 + 1;
                         w_sb[lane] = sb;
                         w_tb[lane] = tb;
                         w_r2[lane] = r2;
@@ -189,15 +190,35 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                     }
                 }
             }
-            // exclusive prefix of the item counts across the wave
-            int incl = nrow;
+            // Item order: grouped by the number of covered columns (the pair loop's length),
+            // longest first, so the 64 items dealt to the lanes at once have similar lengths.
+            // Exclusive prefix of the item counts per group across the wave: three words of
+            // 10-bit fields (groups 8 7 6 | 5 4 3 | 2 1; at most 512 items per group).
+            const int gw = ncolc >= 6 ? 0 : (ncolc >= 3 ? 1 : 2);
+            const int gs = gw == 0 ? 8 - ncolc : (gw == 1 ? 5 - ncolc : 2 - ncolc);
+            uint32_t iw[3] = {0u, 0u, 0u};
+            if (nrow > 0) iw[gw] = (uint32_t)nrow << (10 * gs);
+#pragma unroll
             for (int o2 = 1; o2 < 64; o2 <<= 1) {
-                const int u = __shfl_up(incl, o2);
-                if (lane >= o2) incl += u;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const uint32_t u = __shfl_up(iw[q], o2);
+                    if (lane >= o2) iw[q] += u;
+                }
             }
-            const int total = __shfl(incl, 63);
+            uint32_t tw[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) tw[q] = (uint32_t)uniform((int)__shfl(iw[q], 63));
+            int total = 0, excl = 0;
+#pragma unroll
+            for (int c = 8; c >= 1; --c) {           // groups in decreasing column count
+                const int cw = c >= 6 ? 0 : (c >= 3 ? 1 : 2);
+                const int cs_ = cw == 0 ? 8 - c : (cw == 1 ? 5 - c : 2 - c);
+                const int tot_c = (int)((tw[cw] >> (10 * cs_)) & 1023u);
+                if (c == ncolc) excl = total + (int)((iw[cw] >> (10 * cs_)) & 1023u) - nrow;
+                total += tot_c;
+            }
             if (total == 0) return;
-            const int excl = incl - nrow;
             w_start[lane] = (uint16_t)excl;
             for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
             sync();
