@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--precision", default="best", choices=["best", "f32c", "f64", "both"])
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--map", type=int, default=1024, help="lines workload: map side")
+    ap.add_argument("--geometry", default="faceon", choices=["faceon", "perspective"],
+                    help="lines workload: parallel face-on lines, or the rays of a camera 500 above the "
+                         "disc through the same image points (slopes up to 0.2)")
     ap.add_argument("--grid", default="256,256,64", help="points workload: ns,nphi,nz")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-samples", type=int, default=16384, help="samples in the CPU baseline leg")
@@ -81,6 +84,7 @@ class Workload:
     def __init__(self, a, torch, dev):
         from distorteddustydiscanalysistoolkits_amd import _lib, disc
         self.kind = a.workload
+        self.geometry = a.geometry if a.workload == "lines" else "faceon"
         t0 = time.time()
         if self.kind == "lines":
             self.n = a.particles or 16_000_000
@@ -88,7 +92,7 @@ class Workload:
             self.qnames = ("rho", "vz")
             self.kernel = KERNEL_C4
             self.kname = "C4"
-            o, dv = disc.faceon_lines(a.map)
+            o, dv = disc.faceon_lines(a.map) if a.geometry == "faceon" else disc.perspective_lines(a.map)
             self.samples = (o, dv)
             self.P = a.map * a.map
             self.dsamples = ([torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in o],
@@ -97,7 +101,8 @@ class Workload:
             self.L = 2
             self.strategy = _lib.PMX_SCATTER
             self.fields = ("rho_column", "vz_column")
-            self.desc = (f"warped disc {self.n} particles (FP32 storage), Wendland C4, face-on {a.map}x{a.map} "
+            view = "face-on" if a.geometry == "faceon" else "perspective (camera at z = 500)"
+            self.desc = (f"warped disc {self.n} particles (FP32 storage), Wendland C4, {view} {a.map}x{a.map} "
                          "column map: column density + column-averaged vz, itpScatter")
         else:
             self.n = a.particles or 4_000_000
@@ -228,7 +233,8 @@ def main():
     achieved = pair_bytes / t_kernel / 1e9
     flops_pair = 103 if w.kind == "points" else 30
     flops = flops_pair * n_acc
-    traffic, traffic_src = measured_traffic(f"{w.kind}_{main_key}", n=w.n, P=w.P)
+    geo = "" if getattr(w, "geometry", "faceon") == "faceon" else f"_{w.geometry}"
+    traffic, traffic_src = measured_traffic(f"{w.kind}_{main_key}{geo}", n=w.n, P=w.P)
 
     errors = {}
     if "f32c" in results and "f64" in results:

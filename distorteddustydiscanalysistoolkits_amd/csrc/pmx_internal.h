@@ -108,6 +108,7 @@ struct pmx_ctx {
     pmx::DevBuf gridpts;       // samples of a Cartesian structured grid (x | y | z), then its axes
     pmx::DevBuf records;       // particle records (AoS) for the Morton-order permutation
     pmx::DevBuf tl_st, tl_part, tl_cells;  // line-map binning (pmx_lines_tiled.hip)
+    pmx::DevBuf tl_sl, tl_tsl;             // nearly parallel sets: per-line slopes (unsorted, sorted), per-tile bounds
     hipEvent_t ev[12] = {};
     double timings[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };

@@ -27,6 +27,19 @@ struct Frame {
     double sqrt_band;                  // sqrt(band): R + sqrt(band) >= sqrt(R^2 + band)
     double inv_cs, inv_ct;             // 1 / cell size (binning and footprints use the same products)
     double near;                       // cells: nodes whose box is this close are read whole
+    // nearly parallel sets: d is the lines' mean direction, lines are binned where they cross the
+    // reference plane at depth w0 along d, and the particles' depths lie in w0 + [dwlo, dwhi]
+    double w0, dwlo, dwhi;
+};
+
+// Nearly parallel line sets (the TILT kernels): per sorted line its slopes against the mean
+// direction d, (d_i.u / d_i.d, d_i.v / d_i.d), and its direction's components on u and v; per
+// tile the bounds of its lines' slopes, 1 / the smallest d_i.d, and 1 + the largest |slope|.
+constexpr int kTileSl = 6;
+struct TiltView {
+    const double4* sl = nullptr;       // (slope s, slope t, d_i.u, d_i.v) per sorted line
+    const double* tb = nullptr;        // kTileSl doubles per tile
+    const double* d = nullptr;         // line directions [3][nl] (exact re-checks)
 };
 
 __device__ inline int clamp_cell(double c, int g) { return c < 0.0 ? 0 : (c > (double)(g - 1) ? g - 1 : (int)c); }
@@ -58,7 +71,7 @@ __device__ inline void project_box(const Node32& b, const double a[3], double& l
 // on the vector-memory counter, so prefetching the next line does not wait for the LDS
 // accumulator atomics issued after it.
 struct alignas(16) LineTileLDS {
-    double w_sb[64], w_tb[64], w_r2[64], w_wih[64], w_xs[64];
+    double w_sb[64], w_tb[64], w_r2[64], w_wih[64], w_xs[64], w_dw[64];
     int32_t cs[kCells + 1];
     int32_t w_p[64], w_cc[64];          // w_cc: cx0 | cx1 << 8 | cy0 << 16 (tile-local)
     int32_t pend[128];                  // particles that reach the tile, not yet swept
@@ -83,14 +96,19 @@ __host__ __device__ inline size_t lines_tile_lds(int ncol, int mode, bool wg1 = 
 // ahead (a global read waits on the vector-memory counter, not behind the LDS atomics).
 // Always inlined: it is called from several kernels of one file, and an out-of-line call
 // makes each caller's register allocation conservative.
-template <int K, int MODE, bool WG1 = false, bool H2 = false>
+// TILT: a nearly parallel set (tv): a particle at depth w0 + dw meets line i where the line's
+// reference-plane point shifted by dw times its slopes lies, so the particle's image on the
+// reference plane spans its position minus dw times the tile's slope range, and its footprint
+// there widens by 1 / d_i.d; each pair's squared distance is |e|^2 - (e.d_i)^2 with e the offset
+// from the line's point at the particle's depth.
+template <int K, int MODE, bool WG1 = false, bool H2 = false, bool TILT = false>
 __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, const Frame& f, const TileCat& cat, const int32_t* __restrict__ cstart,
                                   const int32_t* __restrict__ perm, const double2* __restrict__ st_sorted,
                                   const double* __restrict__ o, int64_t nl, const float* __restrict__ lut,
                                   double inv_dq, float* __restrict__ out, int64_t ldo, uint64_t* __restrict__ digest,
-                                  int64_t tile, LineTileLDS& L, double* dyn) {
+                                  int64_t tile, LineTileLDS& L, double* dyn, const TiltView& tv = TiltView{}) {
     int32_t* cs = L.cs;
-    double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wih = L.w_wih, *w_xs = L.w_xs;
+    double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wih = L.w_wih, *w_xs = L.w_xs, *w_dw = L.w_dw;
     int32_t *w_p = L.w_p, *w_cc = L.w_cc;
     uint16_t* w_start = L.w_start;
     unsigned char* w_own = L.w_own;
@@ -122,6 +140,19 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
     const double rt0 = cy_base == 0 ? -__builtin_inf() : f.to + (double)cy_base * f.ct;
     const double rt1 = cy_base + kTS >= f.gy ? __builtin_inf() : f.to + (double)(cy_base + kTS) * f.ct;
     const double near_s = f.near * f.cs, near_t = f.near * f.ct;
+    // TILT: the tile's slope bounds, footprint widening and residual factor (uniform)
+    double ss_lo = 0.0, ss_hi = 0.0, st_lo = 0.0, st_hi = 0.0, icm = 1.0, padm = 1.0;
+    if constexpr (TILT) {
+        const double* q = tv.tb + tile * kTileSl;
+        ss_lo = q[0]; ss_hi = q[1]; st_lo = q[2]; st_hi = q[3]; icm = q[4]; padm = q[5];
+    }
+    // image interval of a position at depth offset dw: [v - hi, v - lo] with [lo, hi] the range of
+    // slope x dw over the tile's slopes (dw a point or an interval [d0, d1])
+    auto shift_range = [&](double lo_sl, double hi_sl, double d0, double d1, double& mn, double& mx) {
+        const double a = lo_sl * d0, b = lo_sl * d1, c = hi_sl * d0, e = hi_sl * d1;
+        mn = fmin(fmin(a, b), fmin(c, e));
+        mx = fmax(fmax(a, b), fmax(c, e));
+    };
 
     for (int C0 = L0; C0 < L1; C0 += kCap) {
         const int m = min(kCap, L1 - C0);
@@ -134,13 +165,27 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         }
         sync();
 
-        auto need = [&](const Node32& nd) {
-            double s0, s1, t0, t1;
+        // node box projected on the plane (TILT: its image over the box's depths)
+        auto image_box = [&](const Node32& nd, double& s0, double& s1, double& t0, double& t1) {
             project_box(nd, f.u, s0, s1);
             project_box(nd, f.v, t0, t1);
+            if constexpr (TILT) {
+                double w0n, w1n, mn, mx;
+                project_box(nd, f.d, w0n, w1n);
+                shift_range(ss_lo, ss_hi, w0n - f.w0, w1n - f.w0, mn, mx);
+                s0 -= mx;
+                s1 -= mn;
+                shift_range(st_lo, st_hi, w0n - f.w0, w1n - f.w0, mn, mx);
+                t0 -= mx;
+                t1 -= mn;
+            }
+        };
+        auto need = [&](const Node32& nd) {
+            double s0, s1, t0, t1;
+            image_box(nd, s0, s1, t0, t1);
             const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
             const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
-            const double R = Kv * (double)nd.hmax;
+            const double R = Kv * (double)nd.hmax * icm;
             const double Rp = R + f.sqrt_band + f.pad;
             return gs * gs + gt * gt <= Rp * Rp;
         };
@@ -157,19 +202,31 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                              z = (double)H.z + (double)Lo.z;
                 const double sb = (x * f.u[0] + y * f.u[1]) + z * f.u[2];
                 const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
+                // image of the position on the reference plane (TILT), else the position itself
+                double dw = 0.0, s_lo = sb, s_hi = sb, t_lo = tb, t_hi = tb;
+                if constexpr (TILT) {
+                    dw = ((x * f.d[0] + y * f.d[1]) + z * f.d[2]) - f.w0;
+                    double mn, mx;
+                    shift_range(ss_lo, ss_hi, dw, dw, mn, mx);
+                    s_lo = sb - mx;
+                    s_hi = sb - mn;
+                    shift_range(st_lo, st_hi, dw, dw, mn, mx);
+                    t_lo = tb - mx;
+                    t_hi = tb - mn;
+                }
                 // reject on the FP32 h (round-to-nearest, widened by 1e-6) before more loads
-                const double Ru = Kv * (double)H.w * (1.0 + 1e-6);
+                const double Ru = Kv * (double)H.w * (1.0 + 1e-6) * icm;
                 const double Rpu = Ru + f.sqrt_band + f.pad;
-                if (!(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1)) {
+                if (!(s_hi + Rpu < rs0 || s_lo - Rpu > rs1 || t_hi + Rpu < rt0 || t_lo - Rpu > rt1)) {
                     const double hb = T.f32in ? (double)H.w : T.cols64[4 * n + p];
                     const double rr = Kv * hb;
                     const double r2 = rr * rr;
-                    const double Rp = rr + f.sqrt_band + f.pad;
+                    const double Rp = rr * icm + f.sqrt_band + f.pad;
                     // global cell range of the footprint, clamped like the binning, then tile-local
-                    const int gx0 = clamp_cell(floor((sb - Rp - f.so) * f.inv_cs), f.gx);
-                    const int gx1 = clamp_cell(floor((sb + Rp - f.so) * f.inv_cs), f.gx);
-                    const int gy0 = clamp_cell(floor((tb - Rp - f.to) * f.inv_ct), f.gy);
-                    const int gy1 = clamp_cell(floor((tb + Rp - f.to) * f.inv_ct), f.gy);
+                    const int gx0 = clamp_cell(floor((s_lo - Rp - f.so) * f.inv_cs), f.gx);
+                    const int gx1 = clamp_cell(floor((s_hi + Rp - f.so) * f.inv_cs), f.gx);
+                    const int gy0 = clamp_cell(floor((t_lo - Rp - f.to) * f.inv_ct), f.gy);
+                    const int gy1 = clamp_cell(floor((t_hi + Rp - f.to) * f.inv_ct), f.gy);
                     const int cx0 = max(gx0 - cx_base, 0), cx1 = min(gx1 - cx_base, kTS - 1);
                     const int cy0 = max(gy0 - cy_base, 0), cy1 = min(gy1 - cy_base, kTS - 1);
                     if (cx0 <= cx1 && cy0 <= cy1) {
@@ -178,6 +235,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         w_sb[lane] = sb;
                         w_tb[lane] = tb;
                         w_r2[lane] = r2;
+                        if constexpr (TILT) w_dw[lane] = dw;
                         w_p[lane] = (int32_t)p;
                         w_cc[lane] = cx0 | cx1 << 8 | cy0 << 16;
                         if (MODE == 0) {
@@ -230,6 +288,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 const int b = min(cs[row * kTS + ((cc >> 8) & 0xff) + 1], C0 + m);
                 if (a >= b) continue;
                 const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
+                const double dwj = TILT ? w_dw[j] : 0.0;
                 const double r2hi = r2 + f.band, r2lo = r2 - f.band;
                 const int64_t pj = w_p[j];
                 // per-item factors in registers: (m/rho) / h, 1 / (h dq), the item's fields
@@ -249,13 +308,26 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 int k = a + (lane % len);
                 // the next line's (s, t) is read one pair ahead (WG1: from LDS, in the pair)
                 double2 Lnext = WG1 ? make_double2(0.0, 0.0) : st_sorted[k];
+                double4 Snext = make_double4(0.0, 0.0, 0.0, 0.0);
+                if constexpr (TILT) Snext = tv.sl[k];
                 for (int it = 0; it < len; ++it) {
                     const int kk = k - C0;
                     const double2 Ln = WG1 ? ST[kk] : Lnext;
+                    double4 Sl = Snext;
                     k = (k + 1 == b) ? a : k + 1;
                     if constexpr (!WG1) Lnext = st_sorted[k];
-                    const double ds = sb - Ln.x, dt = tb - Ln.y;
-                    const double d2 = ds * ds + dt * dt;
+                    if constexpr (TILT) Snext = tv.sl[k];
+                    double d2;
+                    if constexpr (TILT) {
+                        // offset from the line's point at the particle's depth, minus its part along
+                        // the line (d_i.d > 0: e has no component along d)
+                        const double es = fma(-Sl.x, dwj, sb - Ln.x), et = fma(-Sl.y, dwj, tb - Ln.y);
+                        const double ed = Sl.z * es + Sl.w * et;
+                        d2 = fmax((es * es + et * et) - ed * ed, 0.0);
+                    } else {
+                        const double ds = sb - Ln.x, dt = tb - Ln.y;
+                        d2 = ds * ds + dt * dt;
+                    }
                     if (d2 > r2hi) continue;
                     bool ok = d2 < r2lo;
                     if (!ok) {
@@ -263,10 +335,16 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         const int64_t li = perm[C0 + kk];
                         double px, py, pz;
                         pos64(T, pj, px, py, pz);
+                        double dl0 = f.d[0], dl1 = f.d[1], dl2 = f.d[2];
+                        if constexpr (TILT) {
+                            dl0 = tv.d[li];
+                            dl1 = tv.d[nl + li];
+                            dl2 = tv.d[2 * nl + li];
+                        }
                         double D2 = 0.0, Dm = 0.0, e;
-                        e = px - o[li]; D2 += e * e; Dm += e * f.d[0];
-                        e = py - o[nl + li]; D2 += e * e; Dm += e * f.d[1];
-                        e = pz - o[2 * nl + li]; D2 += e * e; Dm += e * f.d[2];
+                        e = px - o[li]; D2 += e * e; Dm += e * dl0;
+                        e = py - o[nl + li]; D2 += e * e; Dm += e * dl1;
+                        e = pz - o[2 * nl + li]; D2 += e * e; Dm += e * dl2;
                         ok = D2 - Dm * Dm <= r2;
                     }
                     if (!ok) continue;
@@ -311,13 +389,25 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         // sweeps stay full where most of a run misses the tile.
         int npend = 0;                 // wave-uniform
         // (on the FP32 position alone: its residual moves the projection by less than pad32)
+        // (TILT: the image's residual is at most (1 + max |slope|) times a projection's)
         auto reaches = [&](int64_t p) {
             const float4 H = T.leaf[p];
             const double x = (double)H.x, y = (double)H.y, z = (double)H.z;
             const double sb = (x * f.u[0] + y * f.u[1]) + z * f.u[2];
             const double tb = (x * f.v[0] + y * f.v[1]) + z * f.v[2];
-            const double Rpu = Kv * (double)H.w * (1.0 + 1e-6) + f.sqrt_band + f.pad32;
-            return !(sb + Rpu < rs0 || sb - Rpu > rs1 || tb + Rpu < rt0 || tb - Rpu > rt1);
+            double s_lo = sb, s_hi = sb, t_lo = tb, t_hi = tb;
+            if constexpr (TILT) {
+                const double dw = ((x * f.d[0] + y * f.d[1]) + z * f.d[2]) - f.w0;
+                double mn, mx;
+                shift_range(ss_lo, ss_hi, dw, dw, mn, mx);
+                s_lo = sb - mx;
+                s_hi = sb - mn;
+                shift_range(st_lo, st_hi, dw, dw, mn, mx);
+                t_lo = tb - mx;
+                t_hi = tb - mn;
+            }
+            const double Rpu = Kv * (double)H.w * (1.0 + 1e-6) * icm + f.sqrt_band + f.pad32 * padm;
+            return !(s_hi + Rpu < rs0 || s_lo - Rpu > rs1 || t_hi + Rpu < rt0 || t_lo - Rpu > rt1);
         };
         auto flush = [&](int cnt) {
             sync();
@@ -348,8 +438,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
             if (cnt <= 64) return true;
             if (cnt > T.bucket) return false;
             double s0, s1, t0, t1;
-            project_box(nd, f.u, s0, s1);
-            project_box(nd, f.v, t0, t1);
+            image_box(nd, s0, s1, t0, t1);
             const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
             const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
             return gs <= near_s && gt <= near_t;

@@ -146,6 +146,20 @@ def flatten_cartesian(xa, ya, za):
     return X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")
 
 
+def perspective_lines(n: int, lo: float = -100.0, hi: float = 100.0, height: float = 500.0):
+    """n x n column-map lines of a pinhole camera at (0, 0, height) looking down on the disc: the
+    line of image point (x_i, y_j, 0) passes through it and the camera; origins (x_i, y_j, 0),
+    unit directions towards the camera.  Slopes reach (hi - lo) / (2 height) at the image edges
+    (0.2 with the defaults)."""
+    ax = linrange(lo, hi, n)
+    X, Y = np.meshgrid(ax, ax, indexing="ij")
+    ox, oy = X.ravel(order="F"), Y.ravel(order="F")
+    oz = np.zeros_like(ox)
+    dx, dy, dz = -ox, -oy, np.full_like(ox, height)
+    nrm = np.sqrt(dx * dx + dy * dy + dz * dz)
+    return (ox, oy, oz), (dx / nrm, dy / nrm, dz / nrm)
+
+
 def faceon_lines(n: int, lo: float = -100.0, hi: float = 100.0):
     """n x n face-on column-map lines: origins (x_i, y_j, 0), direction (0, 0, 1)."""
     ax = linrange(lo, hi, n)

@@ -294,16 +294,33 @@ def _digest_from_csr(off, idx):
     return cnt, s1, s2
 
 
-@pytest.mark.parametrize("geometry", ["faceon", "tilted", "edgeon"])
+def _random_tilts(m, amp, seed=5):
+    o, _ = _tilted_lines(m, (0.0, 0.0, 1.0))
+    rng = np.random.default_rng(seed)
+    tilt = rng.uniform(-amp, amp, (2, o[0].size))
+    nrm = np.sqrt(1.0 + tilt[0] ** 2 + tilt[1] ** 2)
+    return o, [tilt[0] / nrm, tilt[1] / nrm, 1.0 / nrm]
+
+
+@pytest.mark.parametrize("geometry", ["faceon", "tilted", "edgeon", "perspective", "perspective_steep",
+                                      "random_tilt"])
 def test_tiled_line_digest_matches_oracle(built, geometry):
+    """Parallel sets, and nearly parallel ones (a camera's rays; lines tilted at random by up to
+    0.1, whose per-tile slope spread is still small here) on the tiled path."""
     if geometry == "faceon":
         o, d = disc.faceon_lines(40)
     elif geometry == "tilted":
         o, d = _tilted_lines(32, (0.3, -0.2, 0.9))
-    else:
+    elif geometry == "edgeon":
         o, d = _tilted_lines(32, (0.0, 1.0, 0.0))
+    elif geometry == "perspective":
+        o, d = disc.perspective_lines(40)
+    elif geometry == "perspective_steep":           # slopes up to 0.33 at the image edges
+        o, d = disc.perspective_lines(40, height=300.0)
+    else:
+        o, d = _random_tilts(16, 0.1)
     dg = built["ctx"].line_digest(o, d)
-    assert dg is not None, "parallel lines must take the tiled path"
+    assert dg is not None, "parallel and nearly parallel lines must take the tiled path"
     roff, ridx = built["tree"].line_neighbors("M4", "scatter", o, d)
     ref = _digest_from_csr(roff, ridx)
     for g, r in zip(dg, ref):
@@ -312,22 +329,36 @@ def test_tiled_line_digest_matches_oracle(built, geometry):
 
 
 def test_tiled_line_digest_declines_nonparallel(built):
+    # a line that is not unit-length
     o, d = _tilted_lines(8, (0.0, 0.0, 1.0))
     d[0] = d[0].copy()
     d[0][3] = 1e-3
     assert built["ctx"].line_digest(o, d) is None
+    # slopes spread too far inside a tile for the tiled path (many lines, random tilts of 0.4)
+    o, d = _random_tilts(48, 0.4)
+    assert built["ctx"].line_digest(o, d) is None
+    # a line more than acos(0.7) from the mean direction
+    o, d = disc.perspective_lines(16)
+    d = [c.copy() for c in d]
+    d[0][5], d[1][5], d[2][5] = 0.8, 0.0, 0.6
+    assert built["ctx"].line_digest(o, d) is None
 
 
-@pytest.mark.parametrize("geometry", ["tilted", "nonparallel"])
+@pytest.mark.parametrize("geometry", ["tilted", "nonparallel", "perspective", "wide_random"])
 def test_f32c_lines_other_geometries(built, geometry):
+    """FP32c fields of the tiled path (parallel lines, a camera's rays) and of the union walk
+    (random tilts of 0.2 and 0.4 over this disc's depth, declined by the tiled path) against the
+    oracle."""
     if geometry == "tilted":
         o, d = _tilted_lines(24, (0.3, -0.2, 0.9))
+    elif geometry == "nonparallel":
+        o, d = _random_tilts(24, 0.2)
+    elif geometry == "perspective":
+        o, d = disc.perspective_lines(24, height=400.0)
+        assert built["ctx"].line_digest(o, d) is not None
     else:
-        o, d = _tilted_lines(24, (0.0, 0.0, 1.0))
-        rng = np.random.default_rng(5)
-        tilt = rng.uniform(-0.2, 0.2, (2, o[0].size))
-        nrm = np.sqrt(1.0 + tilt[0] ** 2 + tilt[1] ** 2)
-        d = [tilt[0] / nrm, tilt[1] / nrm, 1.0 / nrm]
+        o, d = _random_tilts(48, 0.4)
+        assert built["ctx"].line_digest(o, d) is None
     cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
     outs = [np.empty(o[0].size) for _ in range(2)]
     built["ctx"].interpolate_lines(_lib.PMX_SCATTER, _lib.PMX_PREC_F32C, cat, o, d, outs)
