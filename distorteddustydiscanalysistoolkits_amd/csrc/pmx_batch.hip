@@ -50,7 +50,31 @@ struct BatchUnit {
     int64_t ldo;
     int64_t tile0;
     int64_t ntiles;
+    // line units: the tolerances of f are re-derived from the rebuilt particle box before every
+    // interpolation (k_frame_scale): root box (n >= 2) or the single leaf, and the lines' scale
+    const Node64* root;
+    double amax;
+    int tilt;                  // nearly parallel line set (tv)
+    lmap::TiltView tv;
 };
+
+// Line units: the frame's tolerances from the current particle box (the particles may move
+// between runs; the lines stay as planned).
+__global__ void k_frame_scale(BatchUnit* __restrict__ units, int nunits) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    BatchUnit& U = units[u];
+    if (U.kind != PMX_JOB_LINES) return;
+    double E = U.amax;
+    if (U.T.n >= 2) {
+        for (int k = 0; k < 3; ++k) E = fmax(E, fmax(fabs(U.root->mn[k]), fabs(U.root->mx[k])));
+    } else {
+        // the single particle: its FP32 position is within 2^-24 of the FP64 one
+        const float4 H = U.T.leaf[0];
+        E = fmax(E, (1.0 + 1e-6) * fmax(fabs((double)H.x), fmax(fabs((double)H.y), fabs((double)H.z))));
+    }
+    lmap::frame_scale(U.f, 2.0 * E);
+}
 
 // Sample tiles of derivative sub-catalogs: the tile layout is sized by the unit's catalog.
 template <int K, int S>
@@ -61,7 +85,8 @@ __device__ void batch_deriv_tile(const BatchUnit& U, int64_t t, pts::BPos& P, vo
 
 // DERIV: the batch holds derivative units (an instantiation of its own, launched only then, so
 // that scalar batches keep the smaller register budget).
-template <int K, bool DERIV>
+// TILT: the batch holds nearly parallel line sets (likewise an instantiation of its own).
+template <int K, bool DERIV, bool TILT>
 __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ units, int nunits, int64_t total,
                                               unsigned long long* __restrict__ counter, pts::PointSplit split) {
     // dynamic LDS shared by the tile kinds: points {BPos, BTile}, lines {LineTileLDS, dyn}
@@ -106,6 +131,13 @@ __global__ void __launch_bounds__(64) k_batch(const BatchUnit* __restrict__ unit
                 pts::points_tile<K, kSymmetric, false>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t,
                                                        P, Y, sp);
         } else {
+            if constexpr (TILT) {
+                if (U.tilt) {
+                    lmap::lines_tile<K, 0, true, false, true>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut,
+                                                              U.inv_dq, U.out, U.ldo, nullptr, t, L, ldyn, U.tv);
+                    continue;
+                }
+            }
             lmap::lines_tile<K, 0, true>(U.T, U.f, U.lcat, U.cstart, U.lperm, U.sts, U.o, U.nl, U.lut, U.inv_dq, U.out, U.ldo,
                                    nullptr, t, L, ldyn);
         }
@@ -201,6 +233,7 @@ struct pmx_batch {
     size_t lds_bytes = 0;
     size_t lds_items = 0;                         // dynamic LDS of the split-item kernel
     bool deriv = false;                           // some unit holds derivative terms
+    bool tilt = false;                            // some line unit is a nearly parallel set
     int ncu = 256;
     // launch shape (autotuned)
     int waves_per_cu = 16, bucket_points = 64, bucket_lines = 512, heavy_first = 1;
@@ -297,6 +330,7 @@ int make_units(pmx_batch* b) {
     size_t lds = std::max<size_t>(sizeof(pts::BPos) + sizeof(pts::BTile<false>), sizeof(lmap::LineTileLDS) + 64);
     size_t lds_items = sizeof(pts::BTile<false>);
     b->deriv = false;
+    b->tilt = false;
     std::vector<std::pair<double, BatchUnit>> list;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const pmx_batch_job& J = b->jobs[j];
@@ -354,6 +388,11 @@ int make_units(pmx_batch* b) {
             U.cstart = b->plan[j].cstart;
             U.lperm = b->plan[j].perm;
             U.sts = b->plan[j].sts;
+            U.root = c->nodes64.as<Node64>() + 1;
+            U.amax = b->plan[j].amax;
+            U.tilt = b->plan[j].tilt ? 1 : 0;
+            U.tv = b->plan[j].tv;
+            if (U.tilt) b->tilt = true;
             U.o = b->lines_o[j];
             U.nl = nP;
             U.lut = c->lutf.as<float>() + (size_t)b->kernel * kLutN;
@@ -387,7 +426,8 @@ void launch_batch(pmx_batch* b, int64_t grid) {
     hipStream_t s = b->stream;
     const BatchUnit* U = b->units_dev.as<BatchUnit>();
     const pts::PointSplit& sp = b->split;
-    auto kb = b->deriv ? k_batch<K, true> : k_batch<K, false>;
+    auto kb = b->tilt ? (b->deriv ? k_batch<K, true, true> : k_batch<K, false, true>)
+                      : (b->deriv ? k_batch<K, true, false> : k_batch<K, false, false>);
     auto ki = b->deriv ? k_batch_items<K, true> : k_batch_items<K, false>;
     const size_t lds_items = b->deriv ? b->lds_items : 0;
     hipLaunchKernelGGL(kb, dim3(grid), dim3(64), b->lds_bytes, s, U, (int)b->units.size(), b->total_tiles,
@@ -401,6 +441,8 @@ void launch_batch(pmx_batch* b, int64_t grid) {
 // callers' columns (stream-ordered, capturable).
 int enqueue_interpolation(pmx_batch* b) {
     hipStream_t s = b->stream;
+    const int nu = (int)b->units.size();
+    hipLaunchKernelGGL(k_frame_scale, dim3((nu + 63) / 64), dim3(64), 0, s, b->units_dev.as<BatchUnit>(), nu);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, b->counter.as<unsigned long long>());
     if (b->split.budget > 0) hipLaunchKernelGGL(k_zero_split, dim3(1), dim3(64), 0, s, b->split.counters);
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(b->total_tiles, (int64_t)b->ncu * b->waves_per_cu));
@@ -557,7 +599,8 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
                 r = lines_plan(c, q, q + 3 * nP, nP, &b->plan[j], &ok);
                 if (r != PMX_OK) return ctx_fail(b, c, r, j);
                 if (!ok) return bfail(b, PMX_ERR_ARGUMENT, "batch job " + std::to_string(j) +
-                                                               ": lines must be one parallel unit-direction set");
+                                                               ": lines must be one parallel or nearly parallel "
+                                                               "unit-direction set");
                 b->lines_o[j] = q;
             }
             PMXB_HIP(b, hipStreamSynchronize(c->stream));

@@ -42,6 +42,19 @@ struct TiltView {
     const double* d = nullptr;         // line directions [3][nl] (exact re-checks)
 };
 
+// Tolerances of a frame from the scale E of every coordinate (twice the largest magnitude of
+// the particle box and the line points): |FP64 reference d^2 - projected d^2| <= ~1e-14 E^2
+// (rounding of both forms, direction normalisation within 1e-12), so a band of 1e-11 E^2 around
+// (K h)^2 sends every doubtful pair to the reference formula; |x - hi| <= 2^-24 |x| <= 2^-25 E per
+// coordinate, so a projection from the FP32 position alone is within sqrt(3) 2^-25 E = 5.2e-8 E
+// of the exact one (pad32).
+__host__ __device__ inline void frame_scale(Frame& f, double E) {
+    f.band = 1e-11 * E * E;
+    f.pad = 1e-12 * E;
+    f.pad32 = f.pad + 2e-7 * E;
+    f.sqrt_band = sqrt(f.band);
+}
+
 __device__ inline int clamp_cell(double c, int g) { return c < 0.0 ? 0 : (c > (double)(g - 1) ? g - 1 : (int)c); }
 __device__ inline int cell_of(double v, double org, double inv_size, int g) { return clamp_cell(floor((v - org) * inv_size), g); }
 
@@ -472,6 +485,9 @@ struct LinePlan {
     const int32_t* cstart;
     const int32_t* perm;
     const double2* sts;
+    double amax;          // largest magnitude of the lines' points (the frame's scale with the particles)
+    bool tilt;            // nearly parallel set (tv)
+    TiltView tv;
 };
 
 }  // namespace lmap

@@ -282,6 +282,7 @@ struct Binned {
     unsigned* next_tile;       // tile counter of the line-map kernel
     bool tilt = false;         // nearly parallel: tv holds the slopes
     TiltView tv;
+    double amax = 0.0;         // largest magnitude of the lines' points
 };
 
 template <int K, int MODE, bool H2, bool TILT>
@@ -462,22 +463,12 @@ int bin_lines(pmx_ctx* c, const LineWork* ws, const double* o, const double* d, 
         tmax = std::max(tmax, -part[5 * b + 3]);
         amax = std::max(amax, -part[5 * b + 4]);
     }
-    // scale of all coordinates: particle box and line origins (and the lines' reference-plane points)
+    // scale of all coordinates: particle box (the single particle when n = 1) and line origins
+    // (and the lines' reference-plane points)
     double E = amax;
-    if (c->tree.n >= 2)
-        for (int k = 0; k < 3; ++k) E = std::max(E, std::max(std::fabs(root.mn[k]), std::fabs(root.mx[k])));
-    else
-        E = std::max(E, 1.0);
+    for (int k = 0; k < 3; ++k) E = std::max(E, std::max(std::fabs(root.mn[k]), std::fabs(root.mx[k])));
     if (!(E < 1e150)) return PMX_OK;
-    E *= 2.0;
-    // |FP64 reference d^2 - projected d^2| <= ~1e-14 E^2 (rounding of both forms, direction
-    // normalisation within 1e-12); band 1e-11 E^2 leaves a wide margin
-    f.band = 1e-11 * E * E;
-    f.pad = 1e-12 * E;
-    // |x - hi| <= 2^-24 |x| <= 2^-25 E per coordinate, so the hi-only projection on a unit axis
-    // is within sqrt(3) 2^-25 E = 5.2e-8 E of the exact one
-    f.pad32 = f.pad + 2e-7 * E;
-    f.sqrt_band = std::sqrt(f.band);
+    frame_scale(f, 2.0 * E);
     // cells: about one line per cell, lines at cell centres for a regular lattice
     const double es = smax - smin, et = tmax - tmin;
     int64_t gx, gy;
@@ -565,6 +556,7 @@ int bin_lines(pmx_ctx* c, const LineWork* ws, const double* o, const double* d, 
         out->tv.tb = c->tl_tsl.as<double>();
         out->tv.d = d;
     }
+    out->amax = amax;
     out->f = f;
     out->cells = cells;
     out->perm = perm;
@@ -601,12 +593,15 @@ int interp_lines_tiled(pmx_ctx* c, const int* slots, const int* shep, int ncols,
 
 int lines_plan(pmx_ctx* c, const double* o, const double* d, int64_t nl, lmap::LinePlan* plan, bool* ok) {
     Binned B;
-    PMX_TRY(bin_lines(c, nullptr, o, d, nl, ok, &B, false));
+    PMX_TRY(bin_lines(c, nullptr, o, d, nl, ok, &B));
     if (*ok) {
         plan->f = B.f;
         plan->cstart = B.cells;
         plan->perm = B.perm;
         plan->sts = B.sts;
+        plan->amax = B.amax;
+        plan->tilt = B.tilt;
+        plan->tv = B.tv;
     }
     return PMX_OK;
 }

@@ -226,8 +226,8 @@ int pmx_point_traversal_stats(struct pmx_ctx *ctx, int32_t strategy, const pmx_p
                               int64_t *stats);
 int pmx_line_traversal_stats(struct pmx_ctx *ctx, const pmx_lines *lines, int64_t *stats);
 
-/* Neighbour-set digest of the tiled line-map path (parallel unit-direction lines, the path
- * pmx_interpolate_lines takes for PMX_PREC_F32C on such lines): for each line i (host
+/* Neighbour-set digest of the tiled line-map path (parallel or nearly parallel unit-direction
+ * lines, the path pmx_interpolate_lines takes for PMX_PREC_F32C on such lines): for each line i (host
  * arrays of 3 n uint64) digest[i] = accepted count, digest[n + i] = sum of accepted sorted
  * particle indices, digest[2n + i] = sum of their squares (mod 2^64).  *used = 0 (digest
  * untouched) when the lines do not qualify.  No reference counterpart: a check that the
@@ -263,7 +263,10 @@ int pmx_last_timings(const struct pmx_ctx *ctx, double *ms, int32_t n);
  * pmx_interpolate_points; line jobs take scalars only (as pmx_interpolate_lines).
  * Restrictions: the FP32-compensated path, one SPH kernel for all jobs,
  * particle columns / samples / outputs in device memory (FP32 outputs), line jobs must be one
- * parallel unit-direction line set each.  Job buffers are borrowed for the batch's lifetime. */
+ * parallel or nearly parallel unit-direction line set each (as the tiled path of
+ * pmx_interpolate_lines takes them).  Job buffers are borrowed for the batch's lifetime; particle
+ * values may change between runs (the line-map tolerances follow the rebuilt particle box), the
+ * lines must not. */
 typedef struct pmx_batch pmx_batch;
 enum { PMX_JOB_POINTS = 0, PMX_JOB_LINES = 1 };
 typedef struct pmx_batch_job {

@@ -27,6 +27,9 @@ def _job_specs():
         dict(n=3000, seed=4, warped=False, kind="lines", samples=disc.faceon_lines(16), slots=[0], shep=[0]),
         dict(n=12000, seed=5, warped=False, kind="points", samples=cyl, strategy=_lib.PMX_SCATTER, slots=[0, 1, 2],
              shep=[1, 1, 1]),
+        # a camera's rays (nearly parallel set, slopes up to 0.25)
+        dict(n=7000, seed=6, warped=True, kind="lines", samples=disc.perspective_lines(20, height=400.0), slots=[0, 3],
+             shep=[0, 1]),
     ]
 
 
@@ -126,6 +129,15 @@ def test_batch_against_oracle(batch_setup):
         g = g.double().cpu().numpy()
         assert np.array_equal(np.isnan(g), np.isnan(r))
         assert max(rel_norms(g, r)) <= F32C_TOL
+    k = 5                                              # the camera's rays
+    cols, quant = host[k]
+    tree, _, _ = oracle_tree(cols, quant)
+    o, d = specs[k]["samples"]
+    ref = tree.line_samples("M4", "scatter", specs[k]["slots"], specs[k]["shep"], o, d)
+    for g, r in zip(jobs[k]["out"], ref):
+        g = g.double().cpu().numpy()
+        assert np.array_equal(np.isnan(g), np.isnan(r))
+        assert max(rel_norms(g, r)) <= F32C_TOL
     k = 0                                              # symmetric points on a cylindrical grid
     cols, quant = host[k]
     tree, _, _ = oracle_tree(cols, quant)
@@ -133,6 +145,28 @@ def test_batch_against_oracle(batch_setup):
     g = jobs[k]["out"][0].double().cpu().numpy()
     assert np.array_equal(np.isnan(g), np.isnan(ref[0]))
     assert max(rel_norms(g, ref[0])) <= F32C_TOL
+
+
+def test_batch_particles_grow_between_runs(batch_setup):
+    """The particles of the line jobs spread to 4x their extent after the batch was planned: the
+    frames' tolerances follow the rebuilt particle boxes (eager and replayed).  Against the
+    per-call path to the FP32c tolerance: with the particles 4x deeper, the per-call path may
+    send the camera's rays to the union walk, whose weights round differently."""
+    b, jobs, specs, _ = batch_setup
+    line_jobs = [k for k, sp in enumerate(specs) if sp["kind"] == "lines"]
+    try:
+        for k in line_jobs:
+            for c in jobs[k]["cols"][:3] + [jobs[k]["cols"][4]]:
+                c.mul_(4.0)                            # x, y, z, h (exact in FP32)
+        for graph in (False, True):
+            b.run(graph=graph)
+            _compare_all([jobs[k] for k in line_jobs], [specs[k] for k in line_jobs], tol=F32C_TOL)
+    finally:
+        for k in line_jobs:
+            for c in jobs[k]["cols"][:3] + [jobs[k]["cols"][4]]:
+                c.mul_(0.25)
+    b.run(graph=True)
+    _compare_all(jobs, specs)
 
 
 def test_batch_split_tiles(batch_setup, monkeypatch):
