@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* __restrict__ hi
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __restrict__ keys_in,
+__global__ void __launch_bounds__(kBlock, 4) k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                           const int32_t* __restrict__ vals_in, int64_t n,
                                                           int shift, int ntiles,
                                                           const uint32_t* __restrict__ hist,
@@ -222,8 +222,9 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
                                                           uint64_t* __restrict__ keys_out,
                                                           int32_t* __restrict__ vals_out) {
     constexpr int kPerWave = kTile / kWaves;   // each wavefront ranks one contiguous quarter
-    __shared__ uint64_t s_keys[kTile];
-    __shared__ int32_t s_vals[kTile];
+    // keys staged in digit order, then (the same bytes) the values: 38 KB of LDS, 4 workgroups
+    // per CU
+    __shared__ uint64_t s_stage[kTile];
     __shared__ uint32_t s_gofs[kBins];     // global destination of this tile's digit run
     __shared__ uint32_t s_lstart[kBins];   // start of the digit run inside the tile
     __shared__ uint32_t s_wc[kWaves][kBins];   // per-wave digit counts, then per-wave run bases
@@ -242,41 +243,32 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
     __syncthreads();
 
     uint64_t k[kItems];
-    int32_t v[kItems];
-    int dg[kItems];
-    uint32_t rk[kItems];
+    uint32_t dr[kItems];     // digit << 16 | rank inside the wave's digit run; 0xffffffff: no item
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const int64_t i = wbase + r * 64 + lane;
-        if (i < n) {
-            k[r] = keys_in[i];
-            v[r] = vals_in[i];
-            dg[r] = (int)((k[r] >> shift) & (kBins - 1));
-        } else {
-            k[r] = 0;
-            v[r] = 0;
-            dg[r] = -1;
-        }
+        k[r] = i < n ? keys_in[i] : 0ull;
     }
     // rank inside the wavefront's quarter: lanes of equal digit found by ballots, a running
     // count per digit kept by the wavefront (its own LDS row, no block barriers)
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        const int d = dg[r];
-        uint64_t peers = __ballot(d >= 0);
+        const bool valid = wbase + r * 64 + lane < n;
+        const int d = (int)((k[r] >> shift) & (kBins - 1));
+        uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
-            const uint64_t bal = __ballot(d >= 0 && ((d >> b) & 1));
+            const uint64_t bal = __ballot(valid && ((d >> b) & 1));
             peers &= ((d >> b) & 1) ? bal : ~bal;
         }
         const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
         uint32_t run = 0;
-        if (d >= 0) run = s_wc[wave][d];
+        if (valid) run = s_wc[wave][d];
         __builtin_amdgcn_wave_barrier();
-        if (d >= 0 && before == 0) s_wc[wave][d] = run + (uint32_t)__popcll(peers);
+        if (valid && before == 0) s_wc[wave][d] = run + (uint32_t)__popcll(peers);
         __builtin_amdgcn_wave_barrier();
-        rk[r] = run + before;
+        dr[r] = valid ? ((uint32_t)d << 16 | (run + before)) : 0xffffffffu;
     }
     __syncthreads();
     // digit t: bases of the four quarters inside the digit's run, run starts inside the tile
@@ -289,27 +281,42 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) s_wc[w][t] = lst + wb[w];
     __syncthreads();
+    // slots of this thread's items; keys into the staging buffer
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        const int d = dg[r];
-        if (d >= 0) {
-            const uint32_t slot = s_wc[wave][d] + rk[r];
-            if (slot < (uint32_t)kTile) {
-                s_keys[slot] = k[r];
-                s_vals[slot] = v[r];
-            }
+        if (dr[r] != 0xffffffffu) {
+            const uint32_t slot = s_wc[wave][dr[r] >> 16] + (dr[r] & 0xffffu);
+            dr[r] = slot < (uint32_t)kTile ? slot : 0xffffffffu;
+            if (dr[r] != 0xffffffffu) s_stage[slot] = k[r];
         }
     }
     __syncthreads();
-    const int64_t count = (n - base) < kTile ? (n - base) : kTile;
-    for (int j = t; j < count; j += kBlock) {
-        const uint64_t key = s_keys[j];
-        const int d = (int)((key >> shift) & (kBins - 1));
-        const uint32_t dst = s_gofs[d] + (uint32_t)(j - s_lstart[d]);
-        if (dst < (uint64_t)n) {   // always true for a consistent histogram; keeps stores in bounds
-            keys_out[dst] = key;
-            vals_out[dst] = s_vals[j];
+    // keys out (consecutive lanes store consecutive addresses of a digit run), keeping each
+    // staged slot's destination for the values
+    const int count = (int)((n - base) < kTile ? (n - base) : kTile);
+    uint32_t dst[kItems];
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+        const int j = t + q * kBlock;
+        dst[q] = 0xffffffffu;
+        if (j < count) {
+            const uint64_t key = s_stage[j];
+            const int d = (int)((key >> shift) & (kBins - 1));
+            dst[q] = s_gofs[d] + (uint32_t)(j - s_lstart[d]);
+            if (dst[q] < (uint64_t)n) keys_out[dst[q]] = key;   // always true for a consistent histogram
         }
+    }
+    __syncthreads();
+    // the values (read only now: fewer registers live through the ranking)
+    int32_t* s_v = reinterpret_cast<int32_t*>(s_stage);
+#pragma unroll
+    for (int r = 0; r < kItems; ++r)
+        if (dr[r] != 0xffffffffu) s_v[dr[r]] = vals_in[wbase + r * 64 + lane];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+        const int j = t + q * kBlock;
+        if (j < count && dst[q] < (uint64_t)n) vals_out[dst[q]] = s_v[j];
     }
 }
 
