@@ -141,6 +141,29 @@ def flatten_cylindrical(sa, pa, za):
     return S * np.cos(P), S * np.sin(P), Z.copy()
 
 
+def spherical_axes(r=(1.0, 100.0, 64), phi=(0.0, 2 * np.pi, 64), theta=(0.0, np.pi, 32)):
+    """Axes of StructuredGrid(Spherical, r, phi, theta) (StructuredGrid.jl:492-518): LinRange on r,
+    half-open [phimin, phimax) with step (phimax - phimin) / nphi, and theta = acos of a LinRange
+    from cos(thetamin) to cos(thetamax) (equal-area spacing).  A number for r gives the shell of
+    StructuredGrid(Spherical, rconst, phi, theta) (StructuredGrid.jl:427-454).  cos / acos are the
+    platform's (Julia's own may differ in the last bit)."""
+    ra = np.array([float(r)], np.float64) if np.isscalar(r) else linrange(*r)
+    dphi = (phi[1] - phi[0]) / phi[2]
+    pa = phi[0] + dphi * np.arange(phi[2], dtype=np.float64)
+    ta = np.arccos(linrange(np.cos(theta[0]), np.cos(theta[1]), theta[2]))
+    return ra, pa, ta
+
+
+def flatten_spherical(ra, pa, ta):
+    """Cartesian sample points of a spherical grid in column-major (r fastest) order (coordinate_grid
+    for Spherical, StructuredGrid.jl:184-204, with _sph2cart: x = r sin(theta) cos(phi),
+    y = r sin(theta) sin(phi), z = r cos(theta), coordinate_transformations.jl:197-203)."""
+    R, P, T = np.meshgrid(ra, pa, ta, indexing="ij")
+    R, P, T = (a.ravel(order="F") for a in (R, P, T))
+    rs = R * np.sin(T)
+    return rs * np.cos(P), rs * np.sin(P), R * np.cos(T)
+
+
 def flatten_cartesian(xa, ya, za):
     X, Y, Z = np.meshgrid(xa, ya, za, indexing="ij")
     return X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")

@@ -257,6 +257,10 @@ class Cylindrical:
     pass
 
 
+class Spherical:
+    pass
+
+
 class StructuredGrid:
     """Tensor-product grid (StructuredGrid.jl:42); values column-major over the axes."""
 
@@ -267,23 +271,44 @@ class StructuredGrid:
 
     @classmethod
     def build(cls, COORD, *params):
-        from .disc import cylindrical_axes, linrange
+        from .disc import cylindrical_axes, linrange, spherical_axes
         if COORD is Cartesian:
             axes = [linrange(a, b, n) for a, b, n in params]
         elif COORD is Cylindrical:
             axes = list(cylindrical_axes(*params))
+        elif COORD is Spherical:
+            # StructuredGrid(Spherical, rparams | rconst, phiparams, thetaparams), StructuredGrid.jl:427-518
+            if len(params) != 3:
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "Spherical grids take (r | rconst, phi, theta)")
+            r, phi, theta = params
+            shell = np.isscalar(r)
+            nr = 1 if shell else r[2]
+            if not (nr >= 1 and phi[2] >= 1 and theta[2] >= 1):
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "nr, nϕ, nθ must be ≥ 1")
+            if shell and not r >= 0.0:
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "radial range must satisfy r ≥ 0 ")
+            if not shell and not (r[0] >= 0.0 and r[1] > r[0]):
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "radial range must satisfy 0 ≤ rmin < rmax")
+            if not (phi[0] >= 0.0 and (phi[1] <= 2 * np.pi or np.isclose(phi[1], 2 * np.pi)) and phi[1] > phi[0]):
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "angular range must satisfy 0 ≤ ϕmin < ϕmax ≤ 2π")
+            if not (theta[0] >= 0.0 and theta[1] <= np.pi and theta[1] > theta[0]):
+                raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "polar range must satisfy 0 ≤ θmin < θmax ≤ π")
+            axes = list(spherical_axes(r, phi, theta))
         else:
             raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"unsupported coordinate system {COORD}")
         return cls(np.zeros([len(a) for a in axes], order="F"), axes)
 
 
 def flatten(COORD, grid: StructuredGrid) -> PointSamples:
-    """transform.jl:53-72: Cartesian sample coordinates in column-major order."""
-    from .disc import flatten_cartesian, flatten_cylindrical
+    """transform.jl:53-72: Cartesian sample coordinates in column-major order (Cartesian,
+    Cylindrical, Spherical)."""
+    from .disc import flatten_cartesian, flatten_cylindrical, flatten_spherical
     if COORD is Cartesian:
         coor = flatten_cartesian(*grid.axes)
     elif COORD is Cylindrical:
         coor = flatten_cylindrical(*grid.axes)
+    elif COORD is Spherical:
+        coor = flatten_spherical(*grid.axes)
     else:
         raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"unsupported coordinate system {COORD}")
     return PointSamples(np.ravel(grid.grid, order="F").copy(), coor)

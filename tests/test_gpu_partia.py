@@ -1,5 +1,5 @@
 """The host mirror of Partia's interface driving the library on the GPU, end to end:
-StructuredGrid_interpolation (cylindrical: host flatten, PointSamples_interpolation, reshape;
+StructuredGrid_interpolation (cylindrical and spherical: host flatten, PointSamples_interpolation, reshape;
 Cartesian: samples flattened on the device from the axes), PointSamples_interpolation and
 LineSamples_interpolation, each against the oracle on copies of the same input (the calls permute
 the caller's columns into Morton order, as LinearBVH! does).  The device-flattened Cartesian grid
@@ -57,6 +57,24 @@ def test_structured_grid_cylindrical(columns, precision):
     ref = tree.point_samples("M4", "symmetric", CAT_RHO_V, *disc.flatten_cylindrical(*grid.axes))
     _check([np.ravel(g.grid, order="F") for g in bundle.grids], ref, F32C_TOL if precision == "f32c" else F64_TOL,
            "cylindrical")
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+@pytest.mark.parametrize("shape", ["ball", "shell"])
+def test_structured_grid_spherical(columns, precision, shape):
+    """StructuredGrid(Spherical, ...) through the host flatten (transform.jl:68-71, _sph2cart),
+    PointSamples_interpolation and the reshape (restore_struct), against the oracle."""
+    cols, tree = columns
+    r = (2.0, 90.0, 10) if shape == "ball" else 40.0
+    grid = P.StructuredGrid.build(P.Spherical, r, (0.0, 2 * np.pi, 12), (0.2, np.pi - 0.2, 9))
+    backend = P.MI355XComputeBackend(precision=precision)
+    cat = _catalog()
+    bundle = P.StructuredGrid_interpolation(backend, P.Spherical, grid, _input(cols), cat, P.itpSymmetric)
+    assert bundle.names == cat.ordered_names
+    assert all(g.grid.shape == grid.size for g in bundle.grids)
+    ref = tree.point_samples("M4", "symmetric", CAT_RHO_V, *disc.flatten_spherical(*grid.axes))
+    _check([np.ravel(g.grid, order="F") for g in bundle.grids], ref, F32C_TOL if precision == "f32c" else F64_TOL,
+           "spherical")
 
 
 @pytest.mark.parametrize("precision", ["f32c", "f64"])

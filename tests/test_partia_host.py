@@ -67,3 +67,38 @@ def test_flatten_restore_roundtrip():
     c = P.StructuredGrid.build(P.Cartesian, (0.2, 0.7, 3), (0.25, 0.55, 2), (0.3, 0.6, 2))
     fc = P.flatten(P.Cartesian, c)
     assert np.allclose(fc.coor[0][:3], [0.2, 0.45, 0.7], rtol=0, atol=1e-16) and fc.coor[1][3] == 0.55
+
+
+def test_spherical_grid_axes_flatten_and_errors():
+    """StructuredGrid(Spherical, ...) (StructuredGrid.jl:427-518): r on a LinRange (or one shell
+    radius), phi half-open, theta = acos of a LinRange in cos(theta); flatten in column-major order
+    with _sph2cart (coordinate_transformations.jl:197-203); the reference's ArgumentError texts."""
+    g = P.StructuredGrid.build(P.Spherical, (1.0, 10.0, 4), (0.0, 2 * np.pi, 6), (0.0, np.pi, 5))
+    ra, pa, ta = g.axes
+    assert g.size == (4, 6, 5)
+    assert np.array_equal(ra, [1.0, 4.0, 7.0, 10.0])
+    assert np.allclose(pa, np.arange(6) * (2 * np.pi / 6), rtol=0, atol=1e-15)
+    assert np.allclose(np.cos(ta), np.linspace(1.0, -1.0, 5), rtol=0, atol=1e-15)   # equal-area in cos(theta)
+    flat = P.flatten(P.Spherical, g)
+    x, y, z = flat.coor
+    i = 0
+    for k in range(5):                    # theta slowest, r fastest
+        for j in range(6):
+            for m in range(4):
+                r, ph, th = ra[m], pa[j], ta[k]
+                assert (x[i], y[i], z[i]) == (r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th))
+                i += 1
+    assert np.allclose(np.sqrt(x * x + y * y + z * z), np.tile(ra, 30), rtol=1e-14)
+    back = P.restore_struct(P.Spherical, flat, g.axes)
+    assert back.size == g.size
+    shell = P.StructuredGrid.build(P.Spherical, 5.0, (0.0, np.pi, 4), (0.1, 3.0, 3))
+    assert shell.size == (1, 4, 3) and np.array_equal(shell.axes[0], [5.0])
+    for bad, text in ((((1.0, 0.5, 4), (0.0, 1.0, 2), (0.0, 1.0, 2)), "radial"),
+                      (((0.0, 1.0, 4), (0.0, 7.0, 2), (0.0, 1.0, 2)), "angular"),
+                      (((0.0, 1.0, 4), (0.0, 1.0, 2), (0.0, 4.0, 2)), "polar"),
+                      (((0.0, 1.0, 0), (0.0, 1.0, 2), (0.0, 1.0, 2)), "must be")):
+        with pytest.raises(P.ArgumentError) as e:
+            P.StructuredGrid.build(P.Spherical, *bad)
+        assert text in str(e.value)
+    with pytest.raises(P.ArgumentError):
+        P.StructuredGrid.build(P.Spherical, -1.0, (0.0, 1.0, 2), (0.0, 1.0, 2))
