@@ -51,6 +51,40 @@ def compile_one(src: str) -> str:
     return obj
 
 
+# The GridDataset writer / reader (pmx_griddataset.cpp): a host library of its own on the HDF5 C
+# library, so that the interpolation library does not depend on HDF5.  libstdc++ is linked
+# statically: the HDF5 directory's own C++ runtime must not be picked up through the run path.
+GD_OUT = os.path.join(HERE, "libpartia_griddataset.so")
+
+
+def hdf5_dir():
+    for cand in (os.environ.get("PMX_HDF5_DIR"), "/opt/conda", "/usr"):
+        if cand and os.path.exists(os.path.join(cand, "include", "hdf5.h")):
+            return cand
+    return None
+
+
+def build_griddataset(verbose: bool = False):
+    h5 = hdf5_dir()
+    if h5 is None:
+        if verbose:
+            print("HDF5 headers not found: libpartia_griddataset.so not built")
+        return None
+    libdir = os.path.join(h5, "lib")
+    tmp = GD_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Werror=shadow",
+           "-I" + os.path.join(h5, "include"), "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(CSRC, "pmx_griddataset.cpp"), "-o", tmp, "-L" + libdir, "-lhdf5",
+           "-Wl,-rpath," + libdir, "-static-libstdc++", "-static-libgcc", "-Wl,--no-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, GD_OUT)
+    if verbose:
+        print("built", GD_OUT)
+    return GD_OUT
+
+
 def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
@@ -62,6 +96,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_griddataset(verbose)
     return OUT
 
 

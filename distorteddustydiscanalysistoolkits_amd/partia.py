@@ -12,6 +12,9 @@ written against Partia's API read the same here:
                               (GPU method: ext/CUDAExt/.../PointSamples_interpolation.jl:12)
   LineSamples_interpolation   grid_interpolation/LineSamples_interpolation.jl:47
   StructuredGrid_interpolation  grid_interpolation/StructuredGrid_interpolation.jl:46
+  GridDataset, write_GridDataset, write_GridBundle, read_GridDataset
+                              Grids/griddataset/GridDataset.jl, IO/gridsIO/write_grids.jl,
+                              IO/gridsIO/read_grids.jl (HDF5, libpartia_griddataset.so)
 
 The compute runs only in the HIP library; a missing library raises.  As in the reference,
 interpolation permutes the caller's particle columns into Morton order in place
@@ -334,6 +337,113 @@ class GridBundle:
 
     def __getitem__(self, name):
         return self.grids[self.names.index(name)]
+
+
+# --------------------------------------------------------------- GridDataset (HDF5) ----
+
+PARTIA_VERSION = "0.1.0"      # the version of Partia.jl whose interface this mirrors (Project.toml)
+
+
+class GridDataset:
+    """A GridBundle with its metadata (GridDataset.jl:3-6): params maps names to str, int, bool or
+    float values."""
+
+    def __init__(self, data: GridBundle, params: dict):
+        self.data = data
+        self.params = dict(params)
+
+
+def file_identifier(operation_name: str = "", now=None) -> str:
+    """logging.jl:152-156: "FT:Partia:<version> (<operation | CommonOutput>): dd/mm/yyyy HH:MM:SS.s"
+    (Julia's "s" is the millisecond count without padding)."""
+    import datetime
+    t = now or datetime.datetime.now()
+    stamp = t.strftime("%d/%m/%Y %H:%M:%S") + f".{t.microsecond // 1000}"
+    return f"FT:Partia:{PARTIA_VERSION} ({operation_name or 'CommonOutput'}): {stamp}"
+
+
+def _float_name(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "Float32"
+    if dt == np.float64:
+        return "Float64"
+    raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"unsupported grid element type {dt}")
+
+
+def write_GridDataset(gd: GridDataset, filename: str = "PRGridDataset.h5") -> None:
+    """write_grids.jl:29-74 (PointSamples) and 98-143 (StructuredGrid): every grid must share the
+    first grid's coordinate / axis vectors (object identity, as the reference's ===)."""
+    from . import griddataset as G
+    grids = gd.data.grids
+    g1 = grids[0]
+    if isinstance(g1, StructuredGrid):
+        kind, vecs, what = G.STRUCTURED, g1.axes, "StructuredGrid axes"
+        for i, gi in enumerate(grids[1:], start=2):
+            for d in range(len(vecs)):
+                if gi.axes[d] is not vecs[d]:
+                    raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"{what} not shared: grid={i} dim={d + 1}")
+    elif isinstance(g1, PointSamples):
+        kind, vecs, what = G.POINTSAMPLES, g1.coor, "PointSamples coor"
+        for i, gi in enumerate(grids[1:], start=2):
+            for d in range(len(vecs)):
+                if gi.coor[d] is not vecs[d]:
+                    raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"{what} not shared: grid={i} dim={d + 1}")
+    else:
+        raise ArgumentError(_lib.PMX_ERR_ARGUMENT, "GridDataset grids must be StructuredGrid or PointSamples")
+    try:
+        G.write(filename, kind, list(vecs), gd.data.names, [np.asarray(g.grid) for g in grids],
+                list(gd.params.items()))
+    except G.GridDatasetError as e:
+        if e.code == G.GD_ERR_ARGUMENT:
+            raise ArgumentError(_lib.PMX_ERR_ARGUMENT, str(e)) from None
+        raise
+
+
+def write_GridBundle(gb: GridBundle, filename: str = "PRGridDataset.h5", *, code_units: dict,
+                     operation_name: str = "", params: dict | None = None) -> None:
+    """write_grids.jl:181-215: schema, system, dtype, grid type, code units and file identifier
+    added to the metadata, then write_GridDataset."""
+    import platform
+    p = {} if params is None else dict(params)
+    required = ("umass", "udist", "utime", "umagfd")
+    notfound = tuple(k for k in required if k not in code_units)
+    if notfound:
+        shown = "(" + ", ".join(":" + k for k in notfound) + ("," if len(notfound) == 1 else "") + ")"
+        raise ArgumentError(_lib.PMX_ERR_ARGUMENT, f"code_units missing required keys: {shown}")
+    p["schema_name"] = "GridDataset"
+    p["schema_id"] = "GridDataset.v1"
+    p["system_kernel"] = platform.system()              # Julia's string(Sys.KERNEL), e.g. "Linux"
+    p["dtype"] = _float_name(np.asarray(gb.grids[0].grid).dtype)
+    p["grid_type"] = type(gb.grids[0]).__name__
+    for k, v in code_units.items():
+        p[k] = float(v)
+    p["file_identifier"] = file_identifier(operation_name)
+    write_GridDataset(GridDataset(gb, p), filename)
+
+
+def read_GridDataset(filename: str = "PRGridDataset.h5") -> GridDataset:
+    """read_grids.jl:29-86: the grids share one set of vectors (read once); StructuredGrid values
+    are reshaped column-major to the axes' lengths."""
+    from . import griddataset as G
+    try:
+        kind, vecs, names, flat, params = G.read(filename)
+    except G.GridDatasetError as e:
+        if e.code == G.GD_ERR_ARGUMENT:
+            raise ArgumentError(_lib.PMX_ERR_ARGUMENT, str(e)) from None
+        raise
+    if kind == G.STRUCTURED:
+        axes = tuple(vecs)
+        size = tuple(v.size for v in axes)
+        grids = []
+        for g in flat:
+            sg = StructuredGrid(np.reshape(g, size, order="F"), axes)
+            sg.axes = axes                     # one shared set of axis vectors
+            grids.append(sg)
+    else:
+        coor = tuple(vecs)
+        grids = [PointSamples(g, coor) for g in flat]
+    return GridDataset(GridBundle(grids, names), params)
 
 
 # --------------------------------------------------------------------------- drivers ----

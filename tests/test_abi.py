@@ -1,4 +1,4 @@
-"""C-ABI checks that need no GPU: the library loads, exports every symbol the public
+"""C-ABI checks that need no GPU: the libraries load and export every symbol the public
 headers (include/*.h) declare, rejects calls without a device cleanly, and its host-side line-kernel
 table agrees with the oracle's and with the reference's own table."""
 import ctypes as C
@@ -13,13 +13,14 @@ from distorteddustydiscanalysistoolkits_amd import _lib
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
+GD_HEADER = os.path.join(ROOT, "include", "partia_griddataset.h")       # libpartia_griddataset.so
+HEADERS = sorted(h for h in glob.glob(os.path.join(ROOT, "include", "*.h")) if h != GD_HEADER)
 GOLDEN = os.path.join(ROOT, "tests", "golden", "line_lut_sample.npz")
 
 
-def declared_symbols():
+def declared_symbols(headers=None):
     syms = set()
-    for h in HEADERS:
+    for h in headers or HEADERS:
         txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
         syms |= set(re.findall(r"\b(pmx_[a-z0-9_]+)\s*\(", txt))
     return sorted(syms)
@@ -42,6 +43,19 @@ def test_binding_declares_every_signature():
     no_args = {"pmx_version", "pmx_phantom_open_error"}
     missing = [s for s in _lib.EXPORTS if s not in no_args and getattr(lib, s).argtypes is None]
     assert not missing, missing
+
+
+def test_griddataset_library_exports_and_binds_its_header():
+    from distorteddustydiscanalysistoolkits_amd import griddataset as G
+    if not os.path.exists(G.LIB):
+        pytest.skip("libpartia_griddataset.so not built (no HDF5)")
+    lib = G.lib()
+    syms = declared_symbols([GD_HEADER])
+    assert len(syms) == 9
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    unbound = [s for s in syms if s != "pmx_gd_last_error" and getattr(lib, s).argtypes is None]
+    assert not unbound, unbound
 
 
 def test_version_and_error_without_context():

@@ -108,3 +108,25 @@ def test_line_samples(columns):
                                          P.itpScatter)
     ref = tree.line_samples("M4", "scatter", [0, 3], [0, 1], o, d)
     _check([g.grid for g in bundle.grids], ref, F32C_TOL, "lines")
+
+
+def test_grid_bundle_to_hdf5_and_back(columns, tmp_path):
+    """StructuredGrid_interpolation on the device, then write_GridBundle and read_GridDataset
+    (write_grids.jl:181-215, read_grids.jl:29-86): the file holds the bundle bit for bit."""
+    import os
+    from distorteddustydiscanalysistoolkits_amd import griddataset as G
+    if not os.path.exists(G.LIB):
+        pytest.skip("libpartia_griddataset.so not built (no HDF5)")
+    cols, _ = columns
+    grid = P.StructuredGrid.build(P.Cylindrical, (1.0, 100.0, 16), (0.0, 2 * np.pi, 16), (-5.0, 5.0, 8))
+    bundle = P.StructuredGrid_interpolation(P.MI355XComputeBackend(precision="f32c"), P.Cylindrical, grid,
+                                            _input(cols), _catalog(), P.itpSymmetric)
+    fn = str(tmp_path / "c2_small.h5")
+    P.write_GridBundle(bundle, fn, code_units=dict(umass=1.0, udist=1.0, utime=1.0, umagfd=1.0),
+                       operation_name="cylindrical")
+    gd = P.read_GridDataset(fn)
+    assert gd.data.names == bundle.names
+    assert gd.params["grid_type"] == "StructuredGrid" and gd.params["dtype"] == "Float32"
+    for g, r in zip(gd.data.grids, bundle.grids):
+        assert g.size == grid.size
+        assert np.array_equal(g.grid, r.grid, equal_nan=True)

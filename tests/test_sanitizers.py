@@ -75,7 +75,8 @@ def test_host_library_under_asan_ubsan(tmp_path):
               "-I" + os.path.join(ROOT, "include")]
     san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
            "-Xarch_host", "-fno-sanitize-recover=undefined"]
-    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    from distorteddustydiscanalysistoolkits_amd import build as hipbuild
+    srcs = list(hipbuild.SOURCES)          # the sources of libpartia_mi355x.so
 
     def compile_one(f):
         obj = str(tmp_path / (f + ".o"))
