@@ -351,6 +351,9 @@ void plane_basis(const double d0[3], Frame& f) {
         f.v[1] = d0[2] * f.u[0] - d0[0] * f.u[2];
         f.v[2] = d0[0] * f.u[1] - d0[1] * f.u[0];
     }
+    f.au = unit_axis(f.u);
+    f.av = unit_axis(f.v);
+    f.ad = unit_axis(f.d);
 }
 
 // Nearly parallel sets: a line at most acos(0.7) = 45.6 degrees from the mean direction (its
