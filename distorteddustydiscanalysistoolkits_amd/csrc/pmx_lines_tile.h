@@ -226,11 +226,15 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 t1 -= mn;
             }
         };
+        // need(nd) also notes whether the node's box lies near the tile, for take(nd) on the same
+        // node right after it (the walk always tests need first)
+        bool near_last = false;
         auto need = [&](const Node32& nd) {
             double s0, s1, t0, t1;
             image_box(nd, s0, s1, t0, t1);
             const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
             const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
+            near_last = gs <= near_s && gt <= near_t;
             const double R = Kv * (double)nd.hmax * icm;
             const double Rp = R + f.sqrt_band + f.pad;
             return gs * gs + gt * gt <= Rp * Rp;
@@ -480,14 +484,10 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         // A node is taken whole (up to T.bucket leaves) when its box itself lies near the
         // tile; a node that reaches the tile only through a large hmax is descended down to
         // one sweep of 64, so that only the subtrees holding the large-h particles are read.
-        auto take = [&](const Node32& nd, int cnt) {
+        auto take = [&](const Node32&, int cnt) {
             if (cnt <= 64) return true;
             if (cnt > T.bucket) return false;
-            double s0, s1, t0, t1;
-            image_box(nd, s0, s1, t0, t1);
-            const double gs = fmax(0.0, fmax(s0 - rs1, rs0 - s1));
-            const double gt = fmax(0.0, fmax(t0 - rt1, rt0 - t1));
-            return gs <= near_s && gt <= near_t;
+            return near_last;
         };
         walk(T, true, need, bucket, take);
         if (npend > 0) flush(npend);
