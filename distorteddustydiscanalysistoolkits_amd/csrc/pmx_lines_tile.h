@@ -440,8 +440,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         int npend = 0;                 // wave-uniform
         // (on the FP32 position alone: its residual moves the projection by less than pad32)
         // (TILT: the image's residual is at most (1 + max |slope|) times a projection's)
-        auto reaches = [&](int64_t p) {
-            const float4 H = T.leaf[p];
+        auto reaches = [&](const float4& H) {
             const double x = (double)H.x, y = (double)H.y, z = (double)H.z;
             const double sb = TILT ? (x * f.u[0] + y * f.u[1]) + z * f.u[2] : dot_axis(x, y, z, f.au, f.u);
             const double tb = TILT ? (x * f.v[0] + y * f.v[1]) + z * f.v[2] : dot_axis(x, y, z, f.av, f.v);
@@ -469,9 +468,14 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
             npend = rest;
             sweep((int64_t)q, lane < cnt);
         };
+        // (the next 64 particles' positions are loaded while the current ones are tested)
         auto bucket = [&](int lo, int cnt) {
+            float4 Hn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (lane < cnt) Hn = T.leaf[(int64_t)lo + lane];
             for (int base = 0; base < cnt; base += 64) {
-                const bool hit = base + lane < cnt && reaches((int64_t)lo + base + lane);
+                const float4 H = Hn;
+                if (base + 64 + lane < cnt) Hn = T.leaf[(int64_t)lo + base + 64 + lane];
+                const bool hit = base + lane < cnt && reaches(H);
                 const uint64_t mk = ballot(hit);
                 if (hit)
                     L.pend[npend + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32),
