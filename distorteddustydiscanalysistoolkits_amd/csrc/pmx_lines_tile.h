@@ -246,8 +246,19 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         auto sweep = [&](int64_t p, bool valid) {
             int nrow = 0, ncolc = 0;
             if (valid) {
+                // every load of the particle issued up front (one latency instead of three); an
+                // FP32 input's position has no low part
                 const float4 H = T.leaf[p];
-                const float4 Lo = T.leaflo[p];
+                float4 Lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (!T.f32in) Lo = T.leaflo[p];
+                double invh_l = 0.0, w64_l = 0.0;
+                float A_l[2] = {0.0f, 0.0f};
+                if (MODE == 0) {
+                    invh_l = T.ih64[p];
+                    w64_l = T.w64[p];
+                    if (cat.n > 0) A_l[0] = T.pay[(int64_t)cat.fcol[0] * n + p];
+                    if (cat.n > 1) A_l[1] = T.pay[(int64_t)cat.fcol[1] * n + p];
+                }
                 const double x = (double)H.x + (double)Lo.x, y = (double)H.y + (double)Lo.y,
                              z = (double)H.z + (double)Lo.z;
                 const double sb = TILT ? (x * f.u[0] + y * f.u[1]) + z * f.u[2] : dot_axis(x, y, z, f.au, f.u);
@@ -290,10 +301,10 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         w_cc[lane] = cx0 | cx1 << 8 | cy0 << 16;
                         if (MODE == 0) {
                             // (m/rho) / h and 1 / (h dq) of line_integrated_kernel_function
-                            const double invh = T.ih64[p];
-                            w_wih[lane] = H2 ? (T.w64[p] * invh) * invh : T.w64[p] * invh;
-                            w_xs[lane] = invh * inv_dq;
-                            for (int c = 0; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
+                            w_wih[lane] = H2 ? (w64_l * invh_l) * invh_l : w64_l * invh_l;
+                            w_xs[lane] = invh_l * inv_dq;
+                            for (int c = 0; c < cat.n && c < 2; ++c) w_A[c * 64 + lane] = A_l[c];
+                            for (int c = 2; c < cat.n; ++c) w_A[c * 64 + lane] = T.pay[(int64_t)cat.fcol[c] * n + p];
                         }
                     }
                 }
