@@ -298,8 +298,9 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
         if (noalloc_init_device(c) != PMX_OK) return PMX_ERR_HIP;
         std::vector<double> h64;
         upload_tables(c, h64);
-        std::vector<float> hf(h64.size());
-        for (size_t i = 0; i < h64.size(); ++i) hf[i] = (float)h64[i];
+        std::vector<float> hf((size_t)6 * kLutF, 0.0f);
+        for (int k = 0; k < 6; ++k)
+            for (int i = 0; i < kLutN; ++i) hf[(size_t)k * kLutF + i] = (float)h64[(size_t)k * kLutN + i];
         if (ensure(c, c->lut64, 8 * h64.size()) != PMX_OK || ensure(c, c->lutf, 4 * hf.size()) != PMX_OK ||
             hipMemcpy(c->lut64.p, h64.data(), 8 * h64.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->lutf.p, hf.data(), 4 * hf.size(), hipMemcpyHostToDevice) != hipSuccess)

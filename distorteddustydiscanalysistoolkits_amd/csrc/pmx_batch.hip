@@ -395,7 +395,7 @@ int make_units(pmx_batch* b) {
             if (U.tilt) b->tilt = true;
             U.o = b->lines_o[j];
             U.nl = nP;
-            U.lut = c->lutf.as<float>() + (size_t)b->kernel * kLutN;
+            U.lut = c->lutf.as<float>() + (size_t)b->kernel * kLutF;
             U.inv_dq = 1.0 / c->lut_dq[b->kernel];
             U.out = outbuf + (int64_t)base * nP;
             U.ntiles = (int64_t)U.f.tx * U.f.ty;

@@ -14,6 +14,9 @@ namespace pmx {
 constexpr int kM4 = 0, kM5 = 1, kM6 = 2, kC2 = 3, kC4 = 4, kC6 = 5;
 constexpr int kGather = 0, kScatter = 1, kSymmetric = 2;
 constexpr int kLutN = 4096;
+// Stride of the FP32 copies of the tables: each kernel's 4096 values (the last is Inorm(K) = 0)
+// are followed by zeros, so a lookup at an index clamped to kLutN - 1 reads lut[i + 1] = 0.
+constexpr int kLutF = kLutN + 64;
 
 // KernelFunctionValid(K, T) (e.g. M4_spline.jl:62, M5_spline.jl:66, M6_spline.jl:66).
 __host__ __device__ constexpr double support(int k) {

@@ -84,7 +84,7 @@ struct pmx_ctx {
     pmx::Tree tree;
     // kernel tables (device): Inorm per kernel, FP64 and FP32, and grid step dq
     pmx::DevBuf lut64;
-    pmx::DevBuf lutf;          // Inorm per kernel rounded to FP32 (16 KB each; the line-map kernel copies it to LDS)
+    pmx::DevBuf lutf;          // Inorm per kernel rounded to FP32, stride kLutF (zeros after each table; the line-map kernel copies it to LDS)
     double lut_dq[6] = {0, 0, 0, 0, 0, 0};
     // Morton-ordered particle data (device)
     pmx::DevBuf codes, order, keys_tmp, vals_tmp;

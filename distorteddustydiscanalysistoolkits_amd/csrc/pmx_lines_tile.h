@@ -126,7 +126,7 @@ __host__ __device__ inline size_t lines_tile_lds(int ncol, int mode, bool wg1 = 
 }
 
 // One 8 x 8-cell tile of a line map (one wavefront).  `lut` is the kernel's Inorm table (in LDS
-// or global memory).  MODE 0: interpolation (FP64 weight sums).  MODE 1: neighbour digest per
+// or global memory) followed by at least one zero.  MODE 0: interpolation (FP64 weight sums).  MODE 1: neighbour digest per
 // line (count, sum of sorted particle indices, sum of their squares; unsigned 64-bit, wrapping).
 // WG1: the workgroup is this one wavefront (the batch kernel, whose table stays in global
 // memory): the chunk's line coordinates are staged in LDS, so the table has the vector L1 to
@@ -162,7 +162,6 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
     unsigned long long* dg = reinterpret_cast<unsigned long long*>(acc);
     float* w_A = reinterpret_cast<float*>(acc + nacc * kCap);     // [ncol][64]
     const double Kv = support(K);
-    const double xmax = Kv * inv_dq;       // support in table steps
     const int64_t n = T.n;
 
     sync();
@@ -195,6 +194,10 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
 
     for (int C0 = L0; C0 < L1; C0 += kCap) {
         const int m = min(kCap, L1 - C0);
+        // the chunk's LDS arrays indexed by sorted line (one address operation per access)
+        double* accC = acc - C0;
+        unsigned long long* dgC = dg - C0;
+        const double2* STc = ST - C0;
         for (int k = lane; k < kCap; k += 64) {
             if (WG1 && k < m) ST[k] = st_sorted[C0 + k];
             for (int a = 0; a < nacc; ++a) {
@@ -372,8 +375,8 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 double4 Snext = make_double4(0.0, 0.0, 0.0, 0.0);
                 if constexpr (TILT) Snext = tv.sl[k];
                 for (int it = 0; it < len; ++it) {
-                    const int kk = k - C0;
-                    const double2 Ln = WG1 ? ST[kk] : Lnext;
+                    const int kc = k;                      // this pair's sorted line
+                    const double2 Ln = WG1 ? STc[kc] : Lnext;
                     double4 Sl = Snext;
                     k = (k + 1 == b) ? a : k + 1;
                     if constexpr (!WG1) Lnext = st_sorted[k];
@@ -393,7 +396,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                     bool ok = d2 < r2lo;
                     if (!ok) {
                         // reference formula on the FP64 columns (_squared_distance_line_coords)
-                        const int64_t li = perm[C0 + kk];
+                        const int64_t li = perm[kc];
                         double px, py, pz;
                         pos64(T, pj, px, py, pz);
                         double dl0 = f.d[0], dl1 = f.d[1], dl2 = f.d[2];
@@ -412,34 +415,34 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                     if (MODE == 0) {
                         // line_integrated_kernel_function: invh * Inorm(r invh).  r from the
                         // hardware FP64 reciprocal square root (relative error ~2^-23) and one
-                        // Newton step (~1e-14; d2 = 0 gives r = 0 through the clamp), q and the
+                        // Newton step (~1e-14; d2 + 1e-300 equals d2 for any d2 above ~1e-284,
+                        // and a particle on the line gets r ~ 1e-150, i.e. Inorm(0)), q and the
                         // table interpolation in FP64 on FP32 table values: FP32 q would carry
-                        // ~1e-7 relative error, amplified by dlnI/dlnq
-                        const double y = __builtin_amdgcn_rsq(fmax(d2, 1e-300));
-                        const double r0 = d2 * y;
-                        const double r = fma(fma(-r0, r0, d2), 0.5 * y, r0);
-                        const double x = r * xscale;             // q / dq
-                        double I = 0.0;
-                        if (x < xmax) {
-                            int i = (int)x;
-                            i = i < 0 ? 0 : (i > kLutN - 2 ? kLutN - 2 : i);
-                            const double ft = x - (double)i;
-                            float2 Lv;                                 // lut[i], lut[i + 1]
-                            __builtin_memcpy(&Lv, lut + i, sizeof(Lv));
-                            const double I0 = (double)Lv.x;
-                            I = fma(ft, (double)Lv.y - I0, I0);
-                        }
+                        // ~1e-7 relative error, amplified by dlnI/dlnq.  The table ends with
+                        // Inorm(K) = 0 followed by zeros (kLutF), so a clamped index needs no
+                        // test of q >= K: there both table values are 0.
+                        const double dd = d2 + 1e-300;
+                        const double y = __builtin_amdgcn_rsq(dd);
+                        const double r0 = dd * y;
+                        const double r = fma(fma(-r0, r0, dd), 0.5 * y, r0);
+                        const double x = fmin(r * xscale, (double)(kLutN - 1));   // q / dq
+                        const int i = (int)x;
+                        const double ft = __builtin_amdgcn_fract(x);
+                        float2 Lv;                                 // lut[i], lut[i + 1]
+                        __builtin_memcpy(&Lv, lut + i, sizeof(Lv));
+                        const double I0 = (double)Lv.x;
+                        const double I = fma(ft, (double)Lv.y - I0, I0);
                         const double w = wih * I;
-                        atomicAdd(&acc[kk], w);
-                        if (cat.n > 0) atomicAdd(&acc[kCap + kk], (double)A0 * w);
-                        if (cat.n > 1) atomicAdd(&acc[2 * kCap + kk], (double)A1 * w);
+                        atomicAdd(&accC[kc], w);
+                        if (cat.n > 0) atomicAdd(&accC[kCap + kc], (double)A0 * w);
+                        if (cat.n > 1) atomicAdd(&accC[2 * kCap + kc], (double)A1 * w);
                         for (int c = 2; c < cat.n; ++c)
-                            atomicAdd(&acc[(1 + c) * kCap + kk], (double)w_A[c * 64 + j] * w);
+                            atomicAdd(&accC[(1 + c) * kCap + kc], (double)w_A[c * 64 + j] * w);
                     } else {
                         const unsigned long long ip = (unsigned long long)pj;
-                        atomicAdd(&dg[kk], 1ull);
-                        atomicAdd(&dg[kCap + kk], ip);
-                        atomicAdd(&dg[2 * kCap + kk], ip * ip);
+                        atomicAdd(&dgC[kc], 1ull);
+                        atomicAdd(&dgC[kCap + kc], ip);
+                        atomicAdd(&dgC[2 * kCap + kc], ip * ip);
                     }
                 }
             }

@@ -256,8 +256,8 @@ __global__ void __launch_bounds__(64 * kTileWaves) k_lines_tiled(T32 T, Frame f,
                                                                  unsigned* __restrict__ next_tile, TiltView tv) {
     extern __shared__ double2 dyn2[];             // per wavefront: accumulators, particle fields
     __shared__ LineTileLDS Ls[kTileWaves];
-    __shared__ float lut_s[kLutN];
-    for (int i = threadIdx.x; i < kLutN; i += 64 * kTileWaves) lut_s[i] = lut[i];
+    __shared__ float lut_s[kLutN + 2];             // the table and a zero after it
+    for (int i = threadIdx.x; i < kLutN + 2; i += 64 * kTileWaves) lut_s[i] = lut[i];
     __syncthreads();
     const int w = threadIdx.x >> 6;
     double* dyn = reinterpret_cast<double*>(dyn2) + (size_t)w * (lines_tile_lds(cat.n, MODE) / sizeof(double));
@@ -300,7 +300,7 @@ int launch_tiled(pmx_ctx* c, const Binned& B, const TileCat& cat, const double* 
     T.bucket = c->tune.tiled_bucket;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, c->stream, B.next_tile, 1);
     hipLaunchKernelGGL((k_lines_tiled<K, MODE, H2, TILT>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat,
-                       B.cells, B.perm, B.sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutN, 1.0 / c->lut_dq[K], out,
+                       B.cells, B.perm, B.sts, o, nl, c->lutf.as<float>() + (size_t)K * kLutF, 1.0 / c->lut_dq[K], out,
                        ldo, digest, B.next_tile, B.tv);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
