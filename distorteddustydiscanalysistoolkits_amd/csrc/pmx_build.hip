@@ -546,7 +546,7 @@ constexpr int kBrtPad = 128;
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
                                                 int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo,
-                                                int32_t* __restrict__ escape) {
+                                                int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc) {
     __shared__ uint64_t win[kBlock + 2 * kBrtPad];
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                                  // internal nodes 1 .. n-1
@@ -617,6 +617,21 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
             e = (delta(next, next + 1) < delta(next - 1, next)) ? (int32_t)(off + next) : (int32_t)next;
         }
         escape[i] = e;
+        auto escape_after = [&](int64_t h) -> int32_t {
+            if (h == n) return 0;
+            if (h == n - 1) return (int32_t)(off + n);
+            const int64_t nx = h + 1;
+            return (delta(nx, nx + 1) < delta(nx - 1, nx)) ? (int32_t)(off + nx) : (int32_t)nx;
+        };
+        auto leaf_out = [&](int64_t li) {
+            const int32_t le = escape_after(li + 1);
+            escape[n + li] = le;
+            leaf_esc[li] = le;
+            lr[2 * (n + li)] = 0;
+            lr[2 * (n + li) + 1] = 0;
+        };
+        leaf_out(i - 1);
+        if (i == n - 1) leaf_out(n - 1);
     }
 }
 
@@ -1216,9 +1231,10 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
         hipLaunchKernelGGL(k_brt, dim3(grid_for(n, kBlock, 1 << 20)), dim3(kBlock), 0, s, codes, n, lr, parent,
-                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape);
-    }
+                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape, c->leaf_esc.as<int32_t>());
+    } else {
     hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
+    }
     if (n >= 2) {
         // the FP32 tree only; the FP64 nodes follow on first use (ensure_leaf64), except the
         // root's box, which is the particle bounding box of the Morton normalisation

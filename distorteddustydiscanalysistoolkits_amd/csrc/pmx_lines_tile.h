@@ -370,16 +370,21 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 // lanes on the same row start at different columns (fewer LDS bank conflicts)
                 const int len = b - a;
                 int k = a + (lane % len);
-                // the next line's (s, t) is read one pair ahead (WG1: from LDS, in the pair)
-                double2 Lnext = WG1 ? make_double2(0.0, 0.0) : st_sorted[k];
+                // the next line's (s, t) is read one pair ahead (WG1: from LDS, in the pair), at a
+                // 32-bit byte offset from the array (line sets on this path hold < 2^28 lines)
+                auto st_at = [&](int kq) {
+                    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(st_sorted) + ((uint32_t)kq << 4));
+                };
+                double2 Lnext = WG1 ? make_double2(0.0, 0.0) : st_at(k);
                 double4 Snext = make_double4(0.0, 0.0, 0.0, 0.0);
                 if constexpr (TILT) Snext = tv.sl[k];
+#pragma unroll 2   // two pairs per trip: no register copies of the prefetched line
                 for (int it = 0; it < len; ++it) {
                     const int kc = k;                      // this pair's sorted line
                     const double2 Ln = WG1 ? STc[kc] : Lnext;
                     double4 Sl = Snext;
                     k = (k + 1 == b) ? a : k + 1;
-                    if constexpr (!WG1) Lnext = st_sorted[k];
+                    if constexpr (!WG1) Lnext = st_at(k);
                     if constexpr (TILT) Snext = tv.sl[k];
                     double d2;
                     if constexpr (TILT) {
@@ -390,7 +395,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         d2 = fmax((es * es + et * et) - ed * ed, 0.0);
                     } else {
                         const double ds = sb - Ln.x, dt = tb - Ln.y;
-                        d2 = ds * ds + dt * dt;
+                        d2 = fma(ds, ds, dt * dt);   // within ~1e-16 d2 of the unfused form, far inside the band
                     }
                     if (d2 > r2hi) continue;
                     bool ok = d2 < r2lo;

@@ -241,11 +241,15 @@ __global__ void k_zero_counter(unsigned* __restrict__ p, int n) {
 // serves them in a few cycles.  Each wavefront takes tiles from a device counter, so the
 // wavefronts of a workgroup stay busy until the map is done.
 constexpr int kTileWaves = 8;
+// Line sets of at least this many lines take the union walk.
+constexpr int64_t kMaxTiledLines = (int64_t)1 << 28;
 
 // H2: weight Inorm / h^2 instead of Inorm / h (a compile-time choice, so the default kernel
-// carries no test of it).  TILT: a nearly parallel set (tv).
+// carries no test of it).  TILT: a nearly parallel set (tv).  Parallel sets run at 4 wavefronts
+// per SIMD (at most 128 VGPRs: two workgroups per CU); the nearly parallel kernels need more
+// registers and run at 2.
 template <int K, int MODE, bool H2 = false, bool TILT = false>
-__global__ void __launch_bounds__(64 * kTileWaves) k_lines_tiled(T32 T, Frame f, TileCat cat,
+__global__ void __launch_bounds__(64 * kTileWaves, TILT ? 2 : 4) k_lines_tiled(T32 T, Frame f, TileCat cat,
                                                                  const int32_t* __restrict__ cstart,
                                                                  const int32_t* __restrict__ perm,
                                                                  const double2* __restrict__ st_sorted,
@@ -365,7 +369,8 @@ constexpr double kTiltMaxSpread = 2.0;
 int bin_lines(pmx_ctx* c, const LineWork* ws, const double* o, const double* d, int64_t nl, bool* ok,
               Binned* out, bool allow_tilt = true) {
     *ok = false;
-    if (nl < 1 || nl > (int64_t)INT32_MAX - kCap || c->tree.n < 1) return PMX_OK;
+    // (the tile kernel addresses the sorted lines with 32-bit byte offsets: fewer than 2^28 lines)
+    if (nl < 1 || nl >= kMaxTiledLines || c->tree.n < 1) return PMX_OK;
     // direction of line 0 and the particle box (root node, or the single leaf)
     double d0[3];
     for (int k = 0; k < 3; ++k)
