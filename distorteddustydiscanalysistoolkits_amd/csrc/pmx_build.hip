@@ -32,14 +32,31 @@ __global__ void __launch_bounds__(kBlock) k_bbox_partial(const T* __restrict__ x
     double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
     int nans = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+    auto take = [&](const double (&v)[3]) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (v[k] != v[k]) { ++nans; continue; }
             mn[k] = v[k] < mn[k] ? v[k] : mn[k];
             mx[k] = v[k] > mx[k] ? v[k] : mx[k];
         }
+    };
+    // four elements per thread and trip: twelve loads in flight
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        double v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u][0] = (double)x[i + u * stride];
+            v[u][1] = (double)y[i + u * stride];
+            v[u][2] = (double)z[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) take(v[u]);
+    }
+    for (; i < n; i += stride) {
+        const double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+        take(v);
     }
     __shared__ double s[6][kBlock];
 #pragma unroll
@@ -114,23 +131,8 @@ __device__ inline uint64_t quantize(double v, double vmin, double vmax) {
         double inv = 1.0 / dv;
         f = (v - vmin) * inv;
     }
-    return (uint64_t)floor(scale * f);
-}
-
-// _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez; value = index.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, const T* __restrict__ y,
-                                                   const T* __restrict__ z, int64_t n,
-                                                   const double* __restrict__ box,
-                                                   uint64_t* __restrict__ codes, int32_t* __restrict__ vals) {
-    const double x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        uint64_t ix = quantize((double)x[i], x0, x1);
-        uint64_t iy = quantize((double)y[i], y0, y1);
-        uint64_t iz = quantize((double)z[i], z0, z1);
-        codes[i] = (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
-        vals[i] = (int32_t)i;
-    }
+    // f lies in [0, 1], so floor(scale * f) < 2^21: the 32-bit conversion is exact
+    return (uint64_t)(uint32_t)floor(scale * f);
 }
 
 // ---------------------------------------------------------------------------------
@@ -149,6 +151,37 @@ constexpr int kBins = 1 << kRadixBits;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;   // 4096
 constexpr int kWaves = kBlock / 64;
+
+// _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez; value = index.
+// One workgroup per 4096-key tile of the sort, which also counts the tile's first digit
+// (bits 0-7) as k_radix_hist would, so the sort's first pass needs no histogram pass.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, const T* __restrict__ y,
+                                                   const T* __restrict__ z, int64_t n,
+                                                   const double* __restrict__ box, int ntiles,
+                                                   uint64_t* __restrict__ codes, int32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kBins];
+    cnt[threadIdx.x] = 0;
+    const double x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t i = base + r * kBlock + threadIdx.x;
+        if (i < n) {
+            const uint64_t ix = quantize((double)x[i], x0, x1);
+            const uint64_t iy = quantize((double)y[i], y0, y1);
+            const uint64_t iz = quantize((double)z[i], z0, z1);
+            const uint64_t code = (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
+            codes[i] = code;
+            vals[i] = (int32_t)i;
+            atomicAdd(&cnt[code & (kBins - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
 
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
                                                        int ntiles, uint32_t* __restrict__ hist) {
@@ -1027,12 +1060,13 @@ __global__ void k_root_box(const double* __restrict__ box, const Node32* __restr
 template <typename T>
 int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* y, const void* z,
                  int64_t n, double* part, int nparts, int* nan_count, double* box, uint64_t* codes,
-                 int32_t* vals) {
+                 int32_t* vals, uint32_t* hist) {
     hipLaunchKernelGGL(k_bbox_partial<T>, dim3(nparts), dim3(kBlock), 0, c->stream, (const T*)x, (const T*)y,
                        (const T*)z, n, part, nan_count);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, part, nparts, box);
-    hipLaunchKernelGGL(k_encode<T>, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, c->stream, (const T*)x,
-                       (const T*)y, (const T*)z, n, box, codes, vals);
+    const int ntiles = (int)((n + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_encode<T>, dim3(ntiles), dim3(kBlock), 0, c->stream, (const T*)x, (const T*)y,
+                       (const T*)z, n, box, ntiles, codes, vals, hist);
     (void)p;
     return PMX_OK;
 }
@@ -1040,7 +1074,8 @@ int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* 
 }  // namespace
 
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
-                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist) {
+                     int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist,
+                     bool first_hist_ready) {
     static_assert(kTile == 4096 && kBins == 256, "radix_hist_bytes assumes 4096-key tiles of 8-bit digits");
     const int ntiles = (int)((n + kTile - 1) / kTile);
     if (!hist) {
@@ -1052,7 +1087,8 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
     int32_t *vin = vals, *vout = vals_alt;
     bool alt = false;
     for (int shift = begin_bit; shift < end_bit; shift += kRadixBits) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, n, shift, ntiles, hist);
+        if (!(first_hist_ready && shift == begin_bit))
+            hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, n, shift, ntiles, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(kBins), dim3(kBlock), 0, c->stream, hist, ntiles, totals);
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, vin, n, shift, ntiles,
                            hist, totals, kout, vout);
@@ -1071,7 +1107,7 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
 // fp64: also write the FP64 nodes (otherwise only the FP32 tree of the compensated path).
 int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     hipStream_t s = c->stream;
-    constexpr int kLaunchRounds = 10;
+    constexpr int kLaunchRounds = 12;
     PMX_TRY(ensure(c, c->scratch_c, 4 * (size_t)kMaxRounds + 64));
     int32_t* listA = c->keys_tmp.as<int32_t>();              // free after the sort (8n bytes)
     int32_t* listB = listA + (n / 2 + 64);
@@ -1161,15 +1197,17 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->vals_tmp, 4 * (size_t)n));
     uint64_t* codes = c->codes.as<uint64_t>();
     int32_t* order = c->order.as<int32_t>();
+    PMX_TRY(ensure(c, c->hist, radix_hist_bytes(n)));
+    uint32_t* hist = c->hist.as<uint32_t>();
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 1, 0u);
     if (p->dtype == PMX_F64)
-        launch_front<double>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order);
+        launch_front<double>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order, hist);
     else
-        launch_front<float>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order);
-    // ---- stable sort + de-dup ----
+        launch_front<float>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order, hist);
+    // ---- stable sort + de-dup (the encode counted the first digit) ----
     bool in_alt = false;
     PMX_TRY(radix_sort_pairs(c, codes, order, n, c->keys_tmp.as<uint64_t>(), c->vals_tmp.as<int32_t>(), 0, 64,
-                             &in_alt));
+                             &in_alt, hist, true));
     if (in_alt) {
         std::swap(c->codes, c->keys_tmp);
         std::swap(c->order, c->vals_tmp);
