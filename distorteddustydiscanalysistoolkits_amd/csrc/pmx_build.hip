@@ -486,10 +486,14 @@ struct Leaf32Out {
     double* ih64;
 };
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__ rec, int ncols,
+// NC > 0: records of NC columns known at compile time, read with the widest aligned vector loads
+// the record size allows (a random record then costs one or two load instructions instead of one
+// per column); NC = 0: ncols at run time, one load per column.
+template <typename T, int NC>
+__global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__ rec, int ncols_rt,
                                                            const int32_t* __restrict__ order, int64_t n,
                                                            double* __restrict__ cols64, Leaf32Out o32) {
+    const int ncols = NC > 0 ? NC : ncols_rt;
     const int ncol = ncols - 6;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         int32_t o = order[i];
@@ -497,15 +501,40 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
         const T* r = rec + (int64_t)o * ncols;
         constexpr bool kCols = sizeof(T) == 8;
         double v[6];
+        if constexpr (NC > 0) {
+            // records start at multiples of NC * sizeof(T) bytes from a 256-byte aligned array
+            constexpr int kBytes = NC * (int)sizeof(T);
+            constexpr int kVec = kBytes % 16 == 0 ? 16 : (kBytes % 8 == 0 ? 8 : 4);
+            using V = typename std::conditional<kVec == 16, int4, typename std::conditional<kVec == 8, int2, int>::type>::type;
+            T rv[NC];
+            const V* rvp = reinterpret_cast<const V*>(r);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            v[c] = (double)r[c];
-            if (kCols) cols64[(int64_t)c * n + i] = v[c];
-        }
-        for (int c = 6; c < ncols; ++c) {
-            const double q = (double)r[c];
-            if (kCols) cols64[(int64_t)c * n + i] = q;
-            o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
+            for (int k = 0; k < kBytes / kVec; ++k) {
+                const V x = rvp[k];
+                __builtin_memcpy(reinterpret_cast<char*>(rv) + k * kVec, &x, kVec);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                v[c] = (double)rv[c];
+                if (kCols) cols64[(int64_t)c * n + i] = v[c];
+            }
+#pragma unroll
+            for (int c = 6; c < NC; ++c) {
+                const double q = (double)rv[c];
+                if (kCols) cols64[(int64_t)c * n + i] = q;
+                o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                v[c] = (double)r[c];
+                if (kCols) cols64[(int64_t)c * n + i] = v[c];
+            }
+            for (int c = 6; c < ncols; ++c) {
+                const double q = (double)r[c];
+                if (kCols) cols64[(int64_t)c * n + i] = q;
+                o32.pay[(int64_t)(c - 5) * n + i] = (float)q;
+            }
         }
         const float xh = (float)v[0], yh = (float)v[1], zh = (float)v[2];
         o32.leaf[i] = make_float4(xh, yh, zh, (float)v[4]);
@@ -516,6 +545,21 @@ __global__ void __launch_bounds__(kBlock) k_gather_records(const T* __restrict__
         o32.pay[(int64_t)(1 + ncol) * n + i] = (float)v[5];
         o32.w64[i] = w;
         o32.ih64[i] = 1.0 / v[4];
+    }
+}
+
+// Record gather with the record width known at compile time for 0 to 4 quant columns.
+template <typename T>
+void launch_gather(pmx_ctx* c, const T* rec, int ncols, const int32_t* order, int64_t n, double* cols64,
+                   const Leaf32Out& o32, unsigned g) {
+    hipStream_t s = c->stream;
+    switch (ncols) {
+    case 6: hipLaunchKernelGGL((k_gather_records<T, 6>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
+    case 7: hipLaunchKernelGGL((k_gather_records<T, 7>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
+    case 8: hipLaunchKernelGGL((k_gather_records<T, 8>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
+    case 9: hipLaunchKernelGGL((k_gather_records<T, 9>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
+    case 10: hipLaunchKernelGGL((k_gather_records<T, 10>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
+    default: hipLaunchKernelGGL((k_gather_records<T, 0>), dim3(g), dim3(kBlock), 0, s, rec, ncols, order, n, cols64, o32); break;
     }
 }
 
@@ -1241,12 +1285,10 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     o32.ih64 = o32.w64 + n;
     if (p->dtype == PMX_F64) {
         hipLaunchKernelGGL(k_pack_records<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<double>());
-        hipLaunchKernelGGL(k_gather_records<double>, dim3(g), dim3(kBlock), 0, s, c->records.as<double>(), ncols,
-                           order, n, cols64, o32);
+        launch_gather<double>(c, c->records.as<double>(), ncols, order, n, cols64, o32, g);
     } else {
         hipLaunchKernelGGL(k_pack_records<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<float>());
-        hipLaunchKernelGGL(k_gather_records<float>, dim3(g), dim3(kBlock), 0, s, c->records.as<float>(), ncols,
-                           order, n, cols64, o32);
+        launch_gather<float>(c, c->records.as<float>(), ncols, order, n, cols64, o32, g);
     }
     c->tree.f32in = f32in;
     c->tree.cols64_ready = !f32in;
