@@ -183,19 +183,36 @@ __global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, cons
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Each wavefront counts into its own LDS histogram (fewer lanes of one atomic instruction meet on
+// one counter), reading two keys per lane and load.
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
                                                        int ntiles, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t cnt[kBins];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kTile;
+    __shared__ uint32_t cnt[kWaves][kBins];
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        int64_t i = base + r * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kBins - 1)], 1u);
+    for (int w = 0; w < kWaves; ++w) cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t* mine = cnt[threadIdx.x >> 6];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    if (base + kTile <= n) {
+        const ulonglong2* k2 = reinterpret_cast<const ulonglong2*>(keys + base);
+#pragma unroll
+        for (int r = 0; r < kItems / 2; ++r) {
+            const ulonglong2 v = k2[r * kBlock + threadIdx.x];
+            atomicAdd(&mine[(v.x >> shift) & (kBins - 1)], 1u);
+            atomicAdd(&mine[(v.y >> shift) & (kBins - 1)], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const int64_t i = base + r * kBlock + threadIdx.x;
+            if (i < n) atomicAdd(&mine[(keys[i] >> shift) & (kBins - 1)], 1u);
+        }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) tot += cnt[w][threadIdx.x];
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = tot;
 }
 
 // Block-wide exclusive scan of one uint32 per thread; returns the block total.

@@ -183,7 +183,7 @@ int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
                int64_t* order_out);
 // Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), 8 bits per pass; the
-// result is in keys_alt / vals_alt when *result_in_alt.  hist: the histogram buffer of
+// result is in keys_alt / vals_alt when *result_in_alt.  Both key arrays 16-byte aligned.  hist: the histogram buffer of
 // radix_hist_bytes(n) bytes (null: the context's); first_hist_ready: it already holds the first
 // pass's per-tile digit counts (k_encode writes them).
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
