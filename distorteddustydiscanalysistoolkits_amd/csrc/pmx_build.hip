@@ -152,27 +152,65 @@ constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;   // 4096
 constexpr int kWaves = kBlock / 64;
 
+// Input columns of the build (x y z m h rho quant...), each of the input type.
+struct ColPtrs {
+    const void* p[24];
+};
+
 // _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez; value = index.
 // One workgroup per 4096-key tile of the sort, which also counts the tile's first digit
-// (bits 0-7) as k_radix_hist would, so the sort's first pass needs no histogram pass.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, const T* __restrict__ y,
-                                                   const T* __restrict__ z, int64_t n,
+// (bits 0-7) as k_radix_hist would, so the sort's first pass needs no histogram pass, and writes
+// each particle's interleaved input record (x y z m h rho quant...; the columns are then read
+// once, and the record gather after the sort takes one random record per particle).  NC > 0:
+// NC columns known at compile time, records stored with the widest aligned vector stores their
+// size allows; NC = 0: ncols at run time.
+template <typename T, int NC>
+__global__ void __launch_bounds__(kBlock) k_encode(ColPtrs src, int ncols_rt, int64_t n,
                                                    const double* __restrict__ box, int ntiles,
                                                    uint64_t* __restrict__ codes, int32_t* __restrict__ vals,
-                                                   uint32_t* __restrict__ hist) {
+                                                   uint32_t* __restrict__ hist, T* __restrict__ rec) {
     __shared__ uint32_t cnt[kBins];
     cnt[threadIdx.x] = 0;
+    const int ncols = NC > 0 ? NC : ncols_rt;
+    const T* __restrict__ x = static_cast<const T*>(src.p[0]);
+    const T* __restrict__ y = static_cast<const T*>(src.p[1]);
+    const T* __restrict__ z = static_cast<const T*>(src.p[2]);
     const double x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kTile;
-#pragma unroll 4
+#pragma unroll 2
     for (int r = 0; r < kItems; ++r) {
         const int64_t i = base + r * kBlock + threadIdx.x;
         if (i < n) {
-            const uint64_t ix = quantize((double)x[i], x0, x1);
-            const uint64_t iy = quantize((double)y[i], y0, y1);
-            const uint64_t iz = quantize((double)z[i], z0, z1);
+            const T xv = x[i], yv = y[i], zv = z[i];
+            if constexpr (NC > 0) {
+                // records start at multiples of NC * sizeof(T) bytes from a 256-byte aligned array
+                constexpr int kBytes = NC * (int)sizeof(T);
+                constexpr int kVec = kBytes % 16 == 0 ? 16 : (kBytes % 8 == 0 ? 8 : 4);
+                using V = typename std::conditional<kVec == 16, int4, typename std::conditional<kVec == 8, int2, int>::type>::type;
+                T rv[NC];
+                rv[0] = xv;
+                rv[1] = yv;
+                rv[2] = zv;
+#pragma unroll
+                for (int c = 3; c < NC; ++c) rv[c] = static_cast<const T*>(src.p[c])[i];
+                V* out = reinterpret_cast<V*>(rec + i * NC);
+#pragma unroll
+                for (int k = 0; k < kBytes / kVec; ++k) {
+                    V v;
+                    __builtin_memcpy(&v, reinterpret_cast<const char*>(rv) + k * kVec, kVec);
+                    out[k] = v;
+                }
+            } else {
+                T* out = rec + i * ncols;
+                out[0] = xv;
+                out[1] = yv;
+                out[2] = zv;
+                for (int c = 3; c < ncols; ++c) out[c] = static_cast<const T*>(src.p[c])[i];
+            }
+            const uint64_t ix = quantize((double)xv, x0, x1);
+            const uint64_t iy = quantize((double)yv, y0, y1);
+            const uint64_t iz = quantize((double)zv, z0, z1);
             const uint64_t code = (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
             codes[i] = code;
             vals[i] = (int32_t)i;
@@ -465,30 +503,9 @@ __global__ void __launch_bounds__(kBlock) k_dedup_apply(uint64_t* __restrict__ c
 // Column gather into Morton order (the permute! calls of InterpolationInput.jl:211-219),
 // converting to FP64.  Column c of the output lives at cols64 + c * n.
 // ---------------------------------------------------------------------------------
-struct ColPtrs {
-    const void* p[24];
-};
-
-// The input columns are first interleaved into one record per particle (coalesced reads
-// and writes), so the permutation reads one record per particle instead of one element per
-// column from unrelated places.
-// A workgroup transposes kBlock particles through LDS: column reads and record writes are
-// both contiguous runs.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pack_records(ColPtrs src, int ncols, int64_t n, T* __restrict__ rec) {
-    __shared__ T tile[kBlock * 24];
-    const int t = threadIdx.x;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-        const int cnt = (int)((n - base) < kBlock ? (n - base) : kBlock);
-        __syncthreads();
-        if (t < cnt)
-            for (int c = 0; c < ncols; ++c) tile[t * ncols + c] = static_cast<const T*>(src.p[c])[base + t];
-        __syncthreads();
-        T* out = rec + base * ncols;
-        const int total = cnt * ncols;
-        for (int e = t; e < total; e += kBlock) out[e] = tile[e];
-    }
-}
+// The input columns were interleaved into one record per particle by k_encode (coalesced
+// reads and writes), so the permutation reads one record per particle instead of one element
+// per column from unrelated places.
 
 // Also writes the FP32 leaf data of the compensated path: {x_hi, y_hi, z_hi, h} and
 // {x_lo, y_lo, z_lo, 1/h} (hi + lo = the FP64 coordinate to 2^-48), and the payload columns
@@ -1119,16 +1136,22 @@ __global__ void k_root_box(const double* __restrict__ box, const Node32* __restr
 }
 
 template <typename T>
-int launch_front(pmx_ctx* c, const pmx_particles* p, const void* x, const void* y, const void* z,
-                 int64_t n, double* part, int nparts, int* nan_count, double* box, uint64_t* codes,
-                 int32_t* vals, uint32_t* hist) {
-    hipLaunchKernelGGL(k_bbox_partial<T>, dim3(nparts), dim3(kBlock), 0, c->stream, (const T*)x, (const T*)y,
-                       (const T*)z, n, part, nan_count);
+int launch_front(pmx_ctx* c, const ColPtrs& cp, int ncols, int64_t n, double* part, int nparts, int* nan_count,
+                 double* box, uint64_t* codes, int32_t* vals, uint32_t* hist, T* rec) {
+    hipLaunchKernelGGL(k_bbox_partial<T>, dim3(nparts), dim3(kBlock), 0, c->stream, (const T*)cp.p[0],
+                       (const T*)cp.p[1], (const T*)cp.p[2], n, part, nan_count);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, part, nparts, box);
     const int ntiles = (int)((n + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_encode<T>, dim3(ntiles), dim3(kBlock), 0, c->stream, (const T*)x, (const T*)y,
-                       (const T*)z, n, box, ntiles, codes, vals, hist);
-    (void)p;
+    const dim3 g(ntiles), b(kBlock);
+    hipStream_t s = c->stream;
+    switch (ncols) {
+    case 6: hipLaunchKernelGGL((k_encode<T, 6>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    case 7: hipLaunchKernelGGL((k_encode<T, 7>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    case 8: hipLaunchKernelGGL((k_encode<T, 8>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    case 9: hipLaunchKernelGGL((k_encode<T, 9>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    case 10: hipLaunchKernelGGL((k_encode<T, 10>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    default: hipLaunchKernelGGL((k_encode<T, 0>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    }
     return PMX_OK;
 }
 
@@ -1260,11 +1283,14 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     int32_t* order = c->order.as<int32_t>();
     PMX_TRY(ensure(c, c->hist, radix_hist_bytes(n)));
     uint32_t* hist = c->hist.as<uint32_t>();
+    PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
+    ColPtrs cp;
+    for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 1, 0u);
     if (p->dtype == PMX_F64)
-        launch_front<double>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order, hist);
+        launch_front<double>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<double>());
     else
-        launch_front<float>(c, p, src[0], src[1], src[2], n, part, nparts, nan_count, box, codes, order, hist);
+        launch_front<float>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<float>());
     // ---- stable sort + de-dup (the encode counted the first digit) ----
     bool in_alt = false;
     PMX_TRY(radix_sort_pairs(c, codes, order, n, c->keys_tmp.as<uint64_t>(), c->vals_tmp.as<int32_t>(), 0, 64,
@@ -1285,10 +1311,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     const bool f32in = p->dtype == PMX_F32;
     if (!f32in) PMX_TRY(ensure(c, c->cols64, 8 * (size_t)n * ncols));
     double* cols64 = f32in ? nullptr : c->cols64.as<double>();
-    ColPtrs cp;
-    for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
     const unsigned g = grid_for(n, kBlock, 16384);
-    PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
     // FP32 data of the compensated path (layout shared with tree32 / prepare_f32)
     const size_t payf = (size_t)n * (2 + p->ncol);
     PMX_TRY(ensure(c, c->nodes32, sizeof(Node32) * (size_t)(n + 1)));
@@ -1300,13 +1323,10 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     o32.pay = c->pay32.as<float>();
     o32.w64 = reinterpret_cast<double*>(c->pay32.as<float>() + payf + (payf & 1));
     o32.ih64 = o32.w64 + n;
-    if (p->dtype == PMX_F64) {
-        hipLaunchKernelGGL(k_pack_records<double>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<double>());
+    if (p->dtype == PMX_F64)
         launch_gather<double>(c, c->records.as<double>(), ncols, order, n, cols64, o32, g);
-    } else {
-        hipLaunchKernelGGL(k_pack_records<float>, dim3(g), dim3(kBlock), 0, s, cp, ncols, n, c->records.as<float>());
+    else
         launch_gather<float>(c, c->records.as<float>(), ncols, order, n, cols64, o32, g);
-    }
     c->tree.f32in = f32in;
     c->tree.cols64_ready = !f32in;
     if (!async) PMX_HIP(c, hipEventRecord(c->ev[2], s));
