@@ -321,16 +321,10 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
             uint32_t iw[3] = {0u, 0u, 0u};
             if (nrow > 0) iw[gw] = (uint32_t)nrow << (10 * gs);
 #pragma unroll
-            for (int o2 = 1; o2 < 64; o2 <<= 1) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const uint32_t u = __shfl_up(iw[q], o2);
-                    if (lane >= o2) iw[q] += u;
-                }
-            }
+            for (int q = 0; q < 3; ++q) iw[q] = wave_inclusive_add(iw[q]);
             uint32_t tw[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) tw[q] = (uint32_t)uniform((int)__shfl(iw[q], 63));
+            for (int q = 0; q < 3; ++q) tw[q] = (uint32_t)__builtin_amdgcn_readlane((int)iw[q], 63);
             int total = 0, excl = 0;
 #pragma unroll
             for (int c = 8; c >= 1; --c) {           // groups in decreasing column count

@@ -60,6 +60,19 @@ __device__ inline void wave_sync() {
 }
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Inclusive prefix sum over the 64 lanes with DPP moves (no LDS): shifts of 1, 2, 4 and 8 lanes
+// inside each row of 16 (lanes whose source lies before the row add 0), then lane 15 of each row
+// into the next row and lane 31 into rows 2 and 3.
+__device__ inline uint32_t wave_inclusive_add(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return v;
+}
+
 // Union walk of the tile over left / escape links, from node `start` until the walk leaves
 // its subtree (escape link `stop`; 0 from the root).  need(nd) is each lane's conservative
 // node test; bucket(lo, cnt) consumes a run of leaves (lo .. lo+cnt-1) in Morton order.
