@@ -241,6 +241,13 @@ struct pmx_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4] = {};
     double info[4] = {0, 0, 0, 0};
+    // The snapshots' builds are independent: they run on kBuildStreams streams (default priority)
+    // forked from and joined back into `stream`, so small builds overlap (parallel branches of
+    // the captured graph).  build_order: jobs by decreasing particle count, dealt round-robin.
+    static constexpr int kBuildStreams = 4;
+    hipStream_t bstream[kBuildStreams] = {};
+    hipEvent_t fork = nullptr, join[kBuildStreams] = {};
+    std::vector<int> build_order;
 };
 
 namespace pmx {
@@ -465,15 +472,24 @@ int enqueue_interpolation(pmx_batch* b) {
     return PMX_OK;
 }
 
-// Every job's capture-safe build on the batch stream.
+// Every job's capture-safe build: forked from the batch stream onto the build streams, joined
+// back before anything that follows on the batch stream.
 int enqueue_builds(pmx_batch* b) {
-    for (size_t j = 0; j < b->jobs.size(); ++j) {
+    constexpr int K = pmx_batch::kBuildStreams;
+    PMXB_HIP(b, hipEventRecord(b->fork, b->stream));
+    for (int k = 0; k < K; ++k) PMXB_HIP(b, hipStreamWaitEvent(b->bstream[k], b->fork, 0));
+    for (size_t r = 0; r < b->build_order.size(); ++r) {
+        const int j = b->build_order[r];
         pmx_ctx* c = b->ctx[j];
         hipStream_t own = c->stream;
-        c->stream = b->stream;
+        c->stream = b->bstream[r % K];
         const int rc = build_lbvh(c, &b->jobs[j].particles, kBuildAsync, nullptr, nullptr);
         c->stream = own;
-        if (rc != PMX_OK) return ctx_fail(b, c, rc, (int)j);
+        if (rc != PMX_OK) return ctx_fail(b, c, rc, j);
+    }
+    for (int k = 0; k < K; ++k) {
+        PMXB_HIP(b, hipEventRecord(b->join[k], b->bstream[k]));
+        PMXB_HIP(b, hipStreamWaitEvent(b->stream, b->join[k], 0));
     }
     return PMX_OK;
 }
@@ -523,6 +539,9 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
         PMXB_HIP(b, hipSetDevice(device_ordinal));
         PMXB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         for (auto& e : b->ev) PMXB_HIP(b, hipEventCreate(&e));
+        for (auto& bs : b->bstream) PMXB_HIP(b, hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+        PMXB_HIP(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        for (auto& e : b->join) PMXB_HIP(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         hipDeviceProp_t prop;
         PMXB_HIP(b, hipGetDeviceProperties(&prop, device_ordinal));
         b->ncu = std::max(1, prop.multiProcessorCount);
@@ -568,6 +587,10 @@ int pmx_batch_create(pmx_batch** out, int32_t device_ordinal, int32_t njobs, con
                 return bfail(b, PMX_ERR_ARGUMENT, tag + "lines must be device memory, n >= 1");
         }
         b->jobs.assign(jobs, jobs + njobs);
+        b->build_order.resize((size_t)njobs);
+        for (int j = 0; j < njobs; ++j) b->build_order[(size_t)j] = j;
+        std::stable_sort(b->build_order.begin(), b->build_order.end(),
+                         [&](int a, int c2) { return jobs[a].particles.n > jobs[c2].particles.n; });
         b->qperm.assign(njobs, nullptr);
         b->plan.resize(njobs);
         b->lines_o.assign(njobs, nullptr);
@@ -729,12 +752,19 @@ void pmx_batch_destroy(pmx_batch* b) {
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    for (auto& bs : b->bstream)
+        if (bs) hipStreamSynchronize(bs);
     drop_graph(b);
     if (b->units_dev.p) hipFree(b->units_dev.p);
     if (b->counter.p) hipFree(b->counter.p);
     if (b->split_buf.p) hipFree(b->split_buf.p);
     for (pmx_ctx* c : b->ctx) pmx_destroy(c);
     for (auto& e : b->ev)
+        if (e) hipEventDestroy(e);
+    for (auto& bs : b->bstream)
+        if (bs) hipStreamDestroy(bs);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (auto& e : b->join)
         if (e) hipEventDestroy(e);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
