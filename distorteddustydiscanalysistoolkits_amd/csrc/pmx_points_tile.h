@@ -486,6 +486,17 @@ struct PointWalk;
 // the accepted pairs to an LDS queue; the wavefront then evaluates 64 queued pairs at a time,
 // one pair per lane, adding FP32 weights into the samples' FP64 LDS sums.  Only accepted pairs
 // cost kernel evaluations, however unevenly they fall on the lanes.
+// Separation |r| of a queued pair from its square: the hardware FP64 reciprocal square root
+// (relative error ~2^-23) and one Newton step (~1e-14), instead of the IEEE square root's longer
+// sequence; d2 + 1e-300 equals d2 for any d2 above ~1e-284, and a particle on the sample gets
+// r ~ 1e-150 (q = 0 to the kernel).
+__device__ inline double pair_dist(double d2) {
+    const double dd = d2 + 1e-300;
+    const double y = __builtin_amdgcn_rsq(dd);
+    const double r0 = dd * y;
+    return fma(fma(-r0, r0, dd), 0.5 * y, r0);
+}
+
 template <int K, int S>
 struct PointWalk<K, S, false> {
     static constexpr int kAcc = 3;
@@ -541,7 +552,7 @@ struct PointWalk<K, S, false> {
             const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
             const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
             const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
-            const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+            const double rd = pair_dist(ex * ex + ey * ey + ez * ez);
             const float invh = P.invh[j];
             float W;
             {
@@ -665,7 +676,7 @@ struct PointWalk<K, S, true> {
             const double ex = ((double)qh0 - (double)P.x[j]) + ((double)ql0 - (double)P.xl[j]);
             const double ey = ((double)qh1 - (double)P.y[j]) + ((double)ql1 - (double)P.yl[j]);
             const double ez = ((double)qh2 - (double)P.z[j]) + ((double)ql2 - (double)P.zl[j]);
-            const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+            const double rd = pair_dist(ex * ex + ey * ey + ez * ez);
             const double sig64 = norm3<double>(K);
             double W, G;
             {
