@@ -71,7 +71,7 @@ int ensure(pmx_ctx* c, DevBuf& b, size_t bytes) {
     if (b.p) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
-        hipFree(b.p);
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -312,18 +312,18 @@ int pmx_create(pmx_ctx** out, int32_t device_ordinal) {
 
 void pmx_destroy(pmx_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->lut64, &c->lutf, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
                       &c->lr, &c->parent, &c->escape, &c->visited, &c->adj, &c->range_hi, &c->range_lo, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
                       &c->tl_st, &c->tl_part, &c->tl_cells, &c->tl_sl, &c->tl_tsl, &c->records, &c->split, &c->gridpts};
     for (DevBuf* b : bufs)
-        if (b->p) hipFree(b->p);
+        if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -506,8 +506,8 @@ int check_nans(pmx_batch* b) {
 }
 
 void drop_graph(pmx_batch* b) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -662,7 +662,7 @@ int pmx_batch_run(pmx_batch* b, int32_t use_graph) {
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(s, &g);
                 if (rc != PMX_OK) {
-                    if (g) hipGraphDestroy(g);
+                    if (g) (void)hipGraphDestroy(g);
                     return rc;
                 }
                 if (ec != hipSuccess) return bfail(b, PMX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
@@ -750,23 +750,23 @@ const char* pmx_batch_last_error(const pmx_batch* b) { return b ? b->err.c_str()
 
 void pmx_batch_destroy(pmx_batch* b) {
     if (!b) return;
-    hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto& bs : b->bstream)
-        if (bs) hipStreamSynchronize(bs);
+        if (bs) (void)hipStreamSynchronize(bs);
     drop_graph(b);
-    if (b->units_dev.p) hipFree(b->units_dev.p);
-    if (b->counter.p) hipFree(b->counter.p);
-    if (b->split_buf.p) hipFree(b->split_buf.p);
+    if (b->units_dev.p) (void)hipFree(b->units_dev.p);
+    if (b->counter.p) (void)hipFree(b->counter.p);
+    if (b->split_buf.p) (void)hipFree(b->split_buf.p);
     for (pmx_ctx* c : b->ctx) pmx_destroy(c);
     for (auto& e : b->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     for (auto& bs : b->bstream)
-        if (bs) hipStreamDestroy(bs);
-    if (b->fork) hipEventDestroy(b->fork);
+        if (bs) (void)hipStreamDestroy(bs);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto& e : b->join)
-        if (e) hipEventDestroy(e);
-    if (b->stream) hipStreamDestroy(b->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
