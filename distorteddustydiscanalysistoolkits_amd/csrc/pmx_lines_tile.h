@@ -112,10 +112,9 @@ __device__ inline double dot_axis(double x, double y, double z, int ax, const do
 struct alignas(16) LineTileLDS {
     double w_sb[64], w_tb[64], w_r2[64], w_wih[64], w_xs[64], w_dw[64];
     int32_t cs[kCells + 1];
-    int32_t w_p[64], w_cc[64];          // w_cc: cx0 | cx1 << 8 | cy0 << 16 (tile-local)
+    int32_t w_p[64];
     int32_t pend[128];                  // particles that reach the tile, not yet swept
-    uint16_t w_start[64];
-    unsigned char w_own[64 * kTS];
+    uint16_t w_item[64 * kTS];          // items: lane | row << 6 | first col << 9 | last col << 12 (tile-local)
 };
 
 // Dynamic LDS of one tile: accumulators (and, for one-wavefront workgroups, the chunk's line
@@ -148,9 +147,8 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                                   int64_t tile, LineTileLDS& L, double* dyn, const TiltView& tv = TiltView{}) {
     int32_t* cs = L.cs;
     double *w_sb = L.w_sb, *w_tb = L.w_tb, *w_r2 = L.w_r2, *w_wih = L.w_wih, *w_xs = L.w_xs, *w_dw = L.w_dw;
-    int32_t *w_p = L.w_p, *w_cc = L.w_cc;
-    uint16_t* w_start = L.w_start;
-    unsigned char* w_own = L.w_own;
+    int32_t* w_p = L.w_p;
+    uint16_t* w_item = L.w_item;
     const int lane = WG1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
     auto sync = [] {
         if constexpr (WG1) __syncthreads();
@@ -247,7 +245,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
         // sweep are dealt round-robin to the lanes (particles with large h do not hold the
         // other lanes idle), and a row's lines are one contiguous run of sorted lines.
         auto sweep = [&](int64_t p, bool valid) {
-            int nrow = 0, ncolc = 0;
+            int nrow = 0, ncolc = 0, cell0 = 0;   // cell0: the item code of the first row
             if (valid) {
                 // every load of the particle issued up front (one latency instead of three); an
                 // FP32 input's position has no low part
@@ -301,7 +299,7 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                         w_r2[lane] = r2;
                         if constexpr (TILT) w_dw[lane] = dw;
                         w_p[lane] = (int32_t)p;
-                        w_cc[lane] = cx0 | cx1 << 8 | cy0 << 16;
+                        cell0 = lane | cy0 << 6 | cx0 << 9 | cx1 << 12;
                         if (MODE == 0) {
                             // (m/rho) / h and 1 / (h dq) of line_integrated_kernel_function
                             w_wih[lane] = H2 ? (w64_l * invh_l) * invh_l : w64_l * invh_l;
@@ -335,15 +333,13 @@ __device__ __attribute__((always_inline)) inline void lines_tile(const T32& T, c
                 total += tot_c;
             }
             if (total == 0) return;
-            w_start[lane] = (uint16_t)excl;
-            for (int i = 0; i < nrow; ++i) w_own[excl + i] = (unsigned char)lane;
+            for (int i = 0; i < nrow; ++i) w_item[excl + i] = (uint16_t)(cell0 + (i << 6));
             sync();
             for (int t = lane; t < total; t += 64) {
-                const int j = w_own[t];
-                const int cc = w_cc[j];
-                const int row = (cc >> 16) + (t - w_start[j]);
-                const int a = max(cs[row * kTS + (cc & 0xff)], C0);
-                const int b = min(cs[row * kTS + ((cc >> 8) & 0xff) + 1], C0 + m);
+                const int itm = w_item[t];
+                const int j = itm & 63, row = (itm >> 6) & 7;
+                const int a = max(cs[row * kTS + ((itm >> 9) & 7)], C0);
+                const int b = min(cs[row * kTS + (itm >> 12) + 1], C0 + m);
                 if (a >= b) continue;
                 const double sb = w_sb[j], tb = w_tb[j], r2 = w_r2[j];
                 const double dwj = TILT ? w_dw[j] : 0.0;
