@@ -728,6 +728,9 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
             e = (delta(next, next + 1) < delta(next - 1, next)) ? (int32_t)(off + next) : (int32_t)next;
         }
         escape[i] = e;
+        // leaf escapes (BinaryRadixTree.jl:262-281) by the rule of the internal ones: the
+        // top-most node starting right after the leaf (a left child's right sibling and a right
+        // child's parent's escape both start there); leaves have no children
         auto escape_after = [&](int64_t h) -> int32_t {
             if (h == n) return 0;
             if (h == n - 1) return (int32_t)(off + n);
@@ -747,8 +750,8 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
 }
 
 // Leaf escapes (BinaryRadixTree.jl:262-281): left child -> right sibling, right child ->
-// escape of the parent.  Independent per leaf once the internal escapes exist.  Also writes
-// the leaves' (zero) child links, which k_brt does not touch.
+// escape of the parent, from the parent links; also the leaves' (zero) child links.  Used for
+// the one-leaf tree only (k_brt computes every other tree's leaf escapes).
 __global__ void __launch_bounds__(kBlock) k_escape_leaf(int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ parent, int64_t n,
                                                         int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc) {
@@ -1342,15 +1345,14 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     int32_t* lr = c->lr.as<int32_t>();
     int32_t* parent = c->parent.as<int32_t>();
     int32_t* escape = c->escape.as<int32_t>();
-    // No fills: k_brt writes the links, parents and ranges of every internal node, the escape
-    // kernels every escape and the leaves' (zero) links, k_refit_small the arrival counter of
-    // every node the bottom-up rounds count at
+    // No fills: k_brt writes the links, parents, ranges and escapes of every node and the leaves'
+    // (zero) links, k_refit_small the arrival counter of every node the bottom-up rounds count at
     hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
         hipLaunchKernelGGL(k_brt, dim3(grid_for(n, kBlock, 1 << 20)), dim3(kBlock), 0, s, codes, n, lr, parent,
                            c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape, c->leaf_esc.as<int32_t>());
     } else {
-    hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
+        hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
     }
     if (n >= 2) {
         // the FP32 tree only; the FP64 nodes follow on first use (ensure_leaf64), except the
