@@ -77,10 +77,12 @@ __global__ void k_frame_scale(BatchUnit* __restrict__ units, int nunits) {
 }
 
 // Sample tiles of derivative sub-catalogs: the tile layout is sized by the unit's catalog.
+// (P: the scalar tiles' staging area, at least as large as the derivative tiles' runs need)
 template <int K, int S>
 __device__ void batch_deriv_tile(const BatchUnit& U, int64_t t, pts::BPos& P, void* dyn, pts::PointSplit sp) {
     pts::BTile<true> Y(dyn, U.pcat.nder, U.pcat.nacc);
-    pts::points_tile<K, S, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, P, Y, sp);
+    auto& Pd = *reinterpret_cast<pts::BPosT<kRunDeriv>*>(&P);
+    pts::points_tile<K, S, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, U.out, U.ldo, t, Pd, Y, sp);
 }
 
 // DERIV: the batch holds derivative units (an instantiation of its own, launched only then, so
@@ -175,10 +177,11 @@ __global__ void __launch_bounds__(64) k_batch_items(const BatchUnit* __restrict_
             const BatchUnit& U = U_of(units, sp, item);
             if (U.deriv) {
                 pts::BTile<true> Yd(dyn2, U.pcat.nder, U.pcat.nacc);
+                auto& Pd = *reinterpret_cast<pts::BPosT<kRunDeriv>*>(&P);
                 if (U.strategy == kGather)
-                    pts::points_item<K, kGather, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P, Yd);
+                    pts::points_item<K, kGather, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, Pd, Yd);
                 else
-                    pts::points_item<K, kSymmetric, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, P,
+                    pts::points_item<K, kSymmetric, true>(U.T, U.pcat, U.qx, U.qy, U.qz, U.qperm, U.npts, sp, item, Pd,
                                                          Yd);
             } else {
                 batch_item<K>(U, sp, item, P, *reinterpret_cast<pts::BTile<false>*>(dyn2));
@@ -359,9 +362,9 @@ int make_units(pmx_batch* b) {
             for (const CatChunk& ch : make_chunks(full_catalog(J.catalog)))
                 for (const pts::PointCat& cat : point_cats(ch)) {
                     BatchUnit U = unit();
-                    U.T = tree32(c, b->bucket_points);
                     U.pcat = cat;
                     U.deriv = cat.ng + cat.nd + cat.nc > 0;
+                    U.T = tree32(c, b->bucket_points, run_cap(U.deriv));
                     U.qx = J.points.x;
                     U.qy = J.points.y;
                     U.qz = J.points.z;
@@ -383,7 +386,7 @@ int make_units(pmx_batch* b) {
         for (int base = 0; base < std::max(ns, 1); base += per) {
             BatchUnit U = unit();
             const int cnt = std::min(per, ns - base);
-            U.T = tree32(c, 64);
+            U.T = tree32(c, 64, kMaxRunLines);
             U.T.bucket = b->bucket_lines;
             U.f = b->plan[j].f;
             U.f.near = (double)c->tune.tiled_near;

@@ -42,7 +42,8 @@ struct __align__(16) Node32 {
 
 // Launch-shape knobs of a context, read from the environment when it is created:
 // PMX_LINES_TILED=0 sends every line set through the union walk; PMX_BUCKET_POINTS /
-// PMX_BUCKET_LINES set the leaf-run size the sample-tile walks hand over at once (kMinRun..64);
+// PMX_BUCKET_LINES set the leaf-run size the sample-tile walks hand over at once (kMinRun..64;
+// derivative point tiles clamp it to 32, their staging capacity);
 // PMX_TILED_BUCKET the leaf-run size of the line-map tile walk (1..65536); PMX_TILED_NEAR how
 // close (in cells) a node's box must be to the tile for the walk to read it whole;
 // PMX_REFIT_SMALL the leaf-run length up to which the refit computes a node directly from its

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kWave) k_points32(T32 T, PointCat cat, const d
                                                     const int32_t* __restrict__ perm,
                                                     int64_t npts,
                                                     float* __restrict__ out, int64_t ldo, PointSplit sp) {
-    __shared__ BPos P;
+    __shared__ BPosT<run_cap(DERIV)> P;
     const int64_t ntiles = (npts + kWave - 1) / kWave;
     if constexpr (DERIV) {
         extern __shared__ double dyn_tile[];        // deriv_tile_lds(cat.nder, cat.nacc) bytes
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(kWave) k_points32_items(T32 T, PointCat cat, c
                                                           const double* __restrict__ qz,
                                                           const int32_t* __restrict__ perm, int64_t npts,
                                                           PointSplit sp) {
-    __shared__ BPos P;
+    __shared__ BPosT<run_cap(DERIV)> P;
     const unsigned int nitems = min(sp.counters[0], (unsigned int)sp.cap_items);
     if constexpr (DERIV) {
         extern __shared__ double dyn_tile[];
@@ -305,9 +305,9 @@ struct LineCat32 {
 };
 
 struct BLine {
-    double w64[kMaxBucket];
-    double invh64[kMaxBucket];
-    float f[8][kMaxBucket];
+    double w64[kMaxRunLines];
+    double invh64[kMaxRunLines];
+    float f[8][kMaxRunLines];
 };
 
 // H2: weight Inorm / h^2 instead of Inorm / h (compile-time, as in the tiled line kernel).
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(kWave) k_lines32(T32 T, LineCat32 cat, const d
                                                    const double* __restrict__ dvec, const int32_t* __restrict__ perm,
                                                    int64_t nl, const double* __restrict__ lut, double dq,
                                                    float* __restrict__ out, int64_t ldo) {
-    __shared__ BPos P;
+    __shared__ BPosT<kMaxRunLines> P;
     __shared__ BLine Y;
     const int lane = threadIdx.x;
     const double Kv = support(K);
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(kWave) k_lines32_nb(T32 T, double Kv, const do
                                                       const double* __restrict__ dvec, int64_t nl,
                                                       int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
                                                       int32_t* __restrict__ idx) {
-    __shared__ BPos P;
+    __shared__ BPosT<kMaxRunLines> P;
     const float Kvf = (float)Kv;
     const int64_t ntiles = (nl + kWave - 1) / kWave;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -426,7 +426,7 @@ void launch_points32_s(pmx_ctx* c, const PointCat& cat, const double* qx, const 
                        const int32_t* perm, int64_t npts, float* out, int64_t ldo,
                        const PointSplit& sp) {
     const dim3 g(tile_grid(npts)), b(kWave);
-    T32 T = tree32(c, c->tune.bucket_points);
+    T32 T = tree32(c, c->tune.bucket_points, run_cap(DERIV));
     const bool split = S != kScatter && sp.budget > 0;
     const size_t lds = DERIV ? deriv_tile_lds(cat.nder, cat.nacc) : 0;
     if (split) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, sp.counters, 4);
@@ -483,13 +483,13 @@ template <int K>
 void launch_lines32(pmx_ctx* c, const LineCat32& cat, const double* o, const double* d, const int32_t* perm,
                     int64_t nl, float* out, int64_t ldo) {
     auto kern = c->line_inv_h2 ? k_lines32<K, true> : k_lines32<K, false>;
-    hipLaunchKernelGGL(kern, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines), cat, o, d,
+    hipLaunchKernelGGL(kern, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines, kMaxRunLines), cat, o, d,
                        perm, nl, c->lut64.as<double>() + (size_t)K * kLutN, c->lut_dq[K], out, ldo);
 }
 
 }  // namespace
 
-T32 tree32(pmx_ctx* c, int bucket) {
+T32 tree32(pmx_ctx* c, int bucket, int cap) {
     T32 T;
     const size_t payf = (size_t)c->tree.n * (2 + c->tree.ncol);
     T.nodes = c->nodes32.as<Node32>();
@@ -504,7 +504,7 @@ T32 tree32(pmx_ctx* c, int bucket) {
     T.ih64 = T.w64 + c->tree.n;
     T.n = c->tree.n;
     T.root = c->tree.root;
-    T.bucket = std::max(kMinRun, std::min(bucket, kMaxBucket));
+    T.bucket = std::max(kMinRun, std::min(bucket, cap));
     return T;
 }
 
@@ -634,7 +634,7 @@ int interp_lines_f32c(pmx_ctx* c, const int* slots, const int* shep, int ncols, 
 
 int nearest_f32c(pmx_ctx* c, const double* qx, const double* qy, const double* qz, int64_t npts, int64_t* idx) {
     PMX_TRY(prepare_f32(c));
-    hipLaunchKernelGGL(k_nearest32, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_points), qx, qy,
+    hipLaunchKernelGGL(k_nearest32, dim3(tile_grid(npts)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_points, kRunScalar), qx, qy,
                        qz, npts, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;
@@ -645,7 +645,7 @@ int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const doub
     PMX_TRY(prepare_f32(c));
     const dim3 g(tile_grid(npts)), b(kWave);
     const double Kv = support(c->tree.kernel);
-    T32 T = tree32(c, c->tune.bucket_points);
+    T32 T = tree32(c, c->tune.bucket_points, kRunScalar);
     if (strategy == kGather)
         hipLaunchKernelGGL(k_points32_nb<kGather>, g, b, 0, c->stream, T, Kv, qx, qy, qz, ha, npts, counts, offsets, idx);
     else if (strategy == kScatter)
@@ -660,7 +660,7 @@ int neighbors_points_f32c(pmx_ctx* c, int strategy, const double* qx, const doub
 int neighbors_lines_f32c(pmx_ctx* c, const double* o, const double* d, int64_t nl, int64_t* counts,
                          const int64_t* offsets, int32_t* idx) {
     PMX_TRY(prepare_f32(c));
-    hipLaunchKernelGGL(k_lines32_nb, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines),
+    hipLaunchKernelGGL(k_lines32_nb, dim3(tile_grid(nl)), dim3(kWave), 0, c->stream, tree32(c, c->tune.bucket_lines, kMaxRunLines),
                        support(c->tree.kernel), o, d, nl, counts, offsets, idx);
     PMX_HIP(c, hipGetLastError());
     return PMX_OK;

@@ -300,7 +300,7 @@ int launch_tiled(pmx_ctx* c, const Binned& B, const TileCat& cat, const double* 
                                                             lds));
     const int64_t resident = (int64_t)std::max(1, per_cu) * c->ncu;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(resident, (ntiles + kTileWaves - 1) / kTileWaves));
-    T32 T = tree32(c, 64);
+    T32 T = tree32(c, 64, kMaxRunLines);
     T.bucket = c->tune.tiled_bucket;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, c->stream, B.next_tile, 1);
     hipLaunchKernelGGL((k_lines_tiled<K, MODE, H2, TILT>), dim3(grid), dim3(64 * kTileWaves), lds, c->stream, T, f, cat,

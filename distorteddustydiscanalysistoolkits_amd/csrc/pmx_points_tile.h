@@ -14,7 +14,6 @@ namespace pmx {
 namespace pts {
 
 constexpr int kWave = 64;
-constexpr int kMaxBucket = 64;
 constexpr float kEps = 4e-6f;     // guard band of FP32 distance / radius comparisons
 constexpr int kMaxF = 9;          // distinct field columns staged per leaf (points)
 constexpr int kMaxD = 7;          // distinct derivative columns staged per leaf (1 + 3 + 3)
@@ -169,13 +168,16 @@ __device__ inline double d2_exact64(const double p[3], const T32& T, int64_t li)
     return s;
 }
 
-// Positions of a bucket staged in LDS.
-struct BPos {
-    float x[kMaxBucket], y[kMaxBucket], z[kMaxBucket], h[kMaxBucket];
-    float xl[kMaxBucket], yl[kMaxBucket], zl[kMaxBucket], invh[kMaxBucket];
+// Positions of a leaf run staged in LDS (CAP leaves at most).
+template <int CAP>
+struct BPosT {
+    float x[CAP], y[CAP], z[CAP], h[CAP];
+    float xl[CAP], yl[CAP], zl[CAP], invh[CAP];
 };
+using BPos = BPosT<kRunScalar>;
 
-__device__ inline void stage_pos(const T32& T, BPos& P, int lo, int cnt) {
+template <int CAP>
+__device__ inline void stage_pos(const T32& T, BPosT<CAP>& P, int lo, int cnt) {
     const int lane = threadIdx.x;
     if (lane < cnt) {
         const float4 L = T.leaf[lo + lane];
@@ -185,7 +187,8 @@ __device__ inline void stage_pos(const T32& T, BPos& P, int lo, int cnt) {
     }
 }
 
-__device__ inline float d2_pos32(const PointLane& q, const BPos& P, int j, float& dx, float& dy, float& dz) {
+template <int CAP>
+__device__ inline float d2_pos32(const PointLane& q, const BPosT<CAP>& P, int j, float& dx, float& dy, float& dz) {
     dx = (q.ph[0] - P.x[j]) + (q.pl[0] - P.xl[j]);
     dy = (q.ph[1] - P.y[j]) + (q.pl[1] - P.yl[j]);
     dz = (q.ph[2] - P.z[j]) + (q.pl[2] - P.zl[j]);
@@ -255,7 +258,8 @@ __device__ inline void seed_run(const T32& T, const float c[3], int& lo, int& cn
 // replaces the best when its FP64 distance is smaller, or equal with a smaller index, so the
 // result is the lexicographic minimum of (d2, index) whatever the visiting order; nodes at the
 // best distance are never pruned (guard band), so ties are all seen.
-__device__ inline int64_t nearest32(const T32& T, const PointLane& q, BPos& P) {
+template <int CAP>
+__device__ inline int64_t nearest32(const T32& T, const PointLane& q, BPosT<CAP>& P) {
     int64_t best = -1;
     double best64 = __builtin_inf();
     float best32 = __builtin_inff();
@@ -365,8 +369,8 @@ __device__ inline TileBox tile_box(const PointLane& q, float radius2_32) {
 // Can staged particle j be accepted by any sample of the tile?  Its distance from the tile's box
 // against the largest radius a pair with it can have, conservative as d2_box32 (a rejected
 // particle is further than its radius rule from every sample).
-template <int S>
-__device__ inline bool reaches_tile(const TileBox& B, const BPos& P, int j, float Kvf) {
+template <int S, int CAP>
+__device__ inline bool reaches_tile(const TileBox& B, const BPosT<CAP>& P, int j, float Kvf) {
     float s = 0.0f;
     const float xh[3] = {P.x[j], P.y[j], P.z[j]}, xl[3] = {P.xl[j], P.yl[j], P.zl[j]};
 #pragma unroll
@@ -410,9 +414,9 @@ struct BTile;
 
 template <>
 struct BTile<false> {
-    float w[kMaxBucket];
-    double ih64[kMaxBucket];   // 1 / h in FP64
-    float s[2][kMaxBucket];
+    float w[kRunScalar];
+    double ih64[kRunScalar];   // 1 / h in FP64
+    float s[2][kRunScalar];
     float ph[3][kWave], pl[3][kWave];
     double invha[kWave];
     double acc[3][kWave];
@@ -435,12 +439,12 @@ struct BTile<true> {
     uint16_t* pair;   // [kQueue]
     __device__ BTile(void* base, int nder, int nacc) {
         char* p = static_cast<char*>(base);
-        w64 = reinterpret_cast<double*>(p); p += 8 * kMaxBucket;
-        ih64 = reinterpret_cast<double*>(p); p += 8 * kMaxBucket;
-        d = reinterpret_cast<double*>(p); p += 8 * kMaxBucket * (size_t)nder;
+        w64 = reinterpret_cast<double*>(p); p += 8 * kRunDeriv;
+        ih64 = reinterpret_cast<double*>(p); p += 8 * kRunDeriv;
+        d = reinterpret_cast<double*>(p); p += 8 * kRunDeriv * (size_t)nder;
         invha = reinterpret_cast<double*>(p); p += 8 * kWave;
         acc = reinterpret_cast<double*>(p); p += 8 * kWave * (size_t)nacc;
-        s = reinterpret_cast<float*>(p); p += 4 * 2 * kMaxBucket;
+        s = reinterpret_cast<float*>(p); p += 4 * 2 * kRunDeriv;
         ph = reinterpret_cast<float*>(p); p += 4 * 3 * kWave;
         pl = reinterpret_cast<float*>(p); p += 4 * 3 * kWave;
         pair = reinterpret_cast<uint16_t*>(p);
@@ -448,7 +452,7 @@ struct BTile<true> {
 };
 
 __host__ __device__ inline size_t deriv_tile_lds(int nder, int nacc) {
-    return 8 * (size_t)(2 * kMaxBucket + kMaxBucket * nder + kWave + kWave * nacc) + 4 * (size_t)(2 * kMaxBucket + 6 * kWave) +
+    return 8 * (size_t)(2 * kRunDeriv + kRunDeriv * nder + kWave + kWave * nacc) + 4 * (size_t)(2 * kRunDeriv + 6 * kWave) +
            2 * (size_t)kQueue;
 }
 
@@ -627,7 +631,7 @@ struct PointWalk<K, S, true> {
     static constexpr int kAcc = kAccMax;
     const T32& T;
     const PointCat& cat;
-    BPos& P;
+    BPosT<kRunDeriv>& P;
     BTile<true>& Y;
     PointLane q;
     int lane;
@@ -637,7 +641,7 @@ struct PointWalk<K, S, true> {
     int spent;                  // leaves tested so far (wave-uniform)
     TileBox box;                // the tile's samples and their largest radius (wave-uniform)
 
-    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPos& P_, BTile<true>& Y_, const PointLane& q_)
+    __device__ PointWalk(const T32& T_, const PointCat& cat_, BPosT<kRunDeriv>& P_, BTile<true>& Y_, const PointLane& q_)
         : T(T_), cat(cat_), P(P_), Y(Y_), q(q_), lane(threadIdx.x), Kv(support(K)), radius2_64(0.0),
           Kvf((float)support(K)), radius2_32(0.0f), qhead(0), qtail(0), spent(0) {}
 
@@ -705,7 +709,7 @@ struct PointWalk<K, S, true> {
             const double w = wb * W;
             add(kA_S1, sa, w);
             if (cat.ns > 0) add(kA_SC, sa, (double)Y.s[j] * w);
-            if (cat.ns > 1) add(kA_SC + 1, sa, (double)Y.s[kMaxBucket + j] * w);
+            if (cat.ns > 1) add(kA_SC + 1, sa, (double)Y.s[kRunDeriv + j] * w);
             // (m/rho) grad W = (m/rho) sigma h^-5 (f'(q)/q) Delta
             const double cg = wb * G;
             const double gx = cg * ex, gy = cg * ey, gz = cg * ez;
@@ -713,23 +717,23 @@ struct PointWalk<K, S, true> {
             add(kA_B + 1, sa, gy);
             add(kA_B + 2, sa, gz);
             if (cat.ng) {
-                const double A = Y.d[cat.gf * kMaxBucket + j];
+                const double A = Y.d[cat.gf * kRunDeriv + j];
                 add(kA_GF, sa, A * gx);
                 add(kA_GF + 1, sa, A * gy);
                 add(kA_GF + 2, sa, A * gz);
                 add(kA_GS, sa, A * w);
             }
             if (cat.nd) {
-                const double Ax = Y.d[cat.df[0] * kMaxBucket + j], Ay = Y.d[cat.df[1] * kMaxBucket + j],
-                             Az = Y.d[cat.df[2] * kMaxBucket + j];
+                const double Ax = Y.d[cat.df[0] * kRunDeriv + j], Ay = Y.d[cat.df[1] * kRunDeriv + j],
+                             Az = Y.d[cat.df[2] * kRunDeriv + j];
                 add(kA_DF, sa, (Ax * gx + Ay * gy) + Az * gz);
                 add(kA_DS, sa, Ax * w);
                 add(kA_DS + 1, sa, Ay * w);
                 add(kA_DS + 2, sa, Az * w);
             }
             if (cat.nc) {
-                const double Ax = Y.d[cat.cf[0] * kMaxBucket + j], Ay = Y.d[cat.cf[1] * kMaxBucket + j],
-                             Az = Y.d[cat.cf[2] * kMaxBucket + j];
+                const double Ax = Y.d[cat.cf[0] * kRunDeriv + j], Ay = Y.d[cat.cf[1] * kRunDeriv + j],
+                             Az = Y.d[cat.cf[2] * kRunDeriv + j];
                 add(kA_CF, sa, Ay * gz - Az * gy);
                 add(kA_CF + 1, sa, Az * gx - Ax * gz);
                 add(kA_CF + 2, sa, Ax * gy - Ay * gx);
@@ -751,9 +755,9 @@ struct PointWalk<K, S, true> {
             const int64_t li = lo + lane;
             Y.w64[lane] = T.w64[li];
             Y.ih64[lane] = T.ih64[li];
-            for (int f = 0; f < cat.ns; ++f) Y.s[f * kMaxBucket + lane] = T.pay[(int64_t)cat.scol[f] * n + li];
+            for (int f = 0; f < cat.ns; ++f) Y.s[f * kRunDeriv + lane] = T.pay[(int64_t)cat.scol[f] * n + li];
             for (int f = 0; f < cat.nder; ++f)
-                Y.d[f * kMaxBucket + lane] = col64(T, cat.dcol[f], li);
+                Y.d[f * kRunDeriv + lane] = col64(T, cat.dcol[f], li);
         }
         __syncthreads();
         const uint64_t cand = ballot(lane < cnt && reaches_tile<S>(box, P, lane, Kvf));
@@ -836,7 +840,7 @@ template <int K, int S, bool DERIV>
 __device__ inline void points_tile(const T32& T, const PointCat& cat, const double* __restrict__ qx,
                                    const double* __restrict__ qy, const double* __restrict__ qz,
                                    const int32_t* __restrict__ perm, int64_t npts, float* __restrict__ out,
-                                   int64_t ldo, int64_t tile, BPos& P,
+                                   int64_t ldo, int64_t tile, BPosT<run_cap(DERIV)>& P,
                                    BTile<DERIV>& Y, const PointSplit& sp) {
     const int lane = threadIdx.x;
     const int64_t slot = tile * kWave + lane;
@@ -892,7 +896,7 @@ template <int K, int S, bool DERIV>
 __device__ inline void points_item(const T32& T, const PointCat& cat, const double* __restrict__ qx,
                                    const double* __restrict__ qy, const double* __restrict__ qz,
                                    const int32_t* __restrict__ perm, int64_t npts, const PointSplit& sp,
-                                   int64_t item, BPos& P, BTile<DERIV>& Y) {
+                                   int64_t item, BPosT<run_cap(DERIV)>& P, BTile<DERIV>& Y) {
     const int lane = threadIdx.x;
     const int2 it = sp.items[item];
     const int split = uniform(it.x), node = uniform(it.y);

@@ -126,8 +126,19 @@ __device__ inline void walk(const T32& T, bool any_valid, Need&& need, Bucket&& 
     walk(T, any_valid, need, bucket, [&](const Node32&, int cnt) { return cnt <= T.bucket; });
 }
 
+// Leaf-run capacities of the walks (their staged particle data in LDS): derivative point tiles
+// stage runs of up to 32 (13 KB of LDS per tile, 3 wavefronts per SIMD: C2 63.5 -> 60.2 ms against
+// runs of 64 at 2.5 wavefronts per SIMD); scalar point tiles and the line union walk, whose
+// occupancy LDS does not limit, runs of up to 64 (the batch interpolation: 143 ms with runs of 32
+// against 137 ms).
+constexpr int kRunDeriv = 32;
+constexpr int kRunScalar = 64;
+constexpr int kMaxRunLines = 64;
+constexpr int run_cap(bool deriv) { return deriv ? kRunDeriv : kRunScalar; }
+
 // FP32 tree view of the context's current LBVH (prepare_f32 must have run); `bucket` is the
-// largest leaf run a walk hands over in one piece (clamped to kMinRun..64).
-T32 tree32(pmx_ctx* c, int bucket);
+// largest leaf run a walk hands over in one piece, clamped to kMinRun..cap (the walk's staging
+// capacity: run_cap(deriv) for point tiles, kMaxRunLines for the line union walk).
+T32 tree32(pmx_ctx* c, int bucket, int cap);
 
 }  // namespace pmx
