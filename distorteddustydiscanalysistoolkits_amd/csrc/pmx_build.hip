@@ -654,10 +654,16 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(const double* __restrict__
 // outside it go to global memory.
 constexpr int kBrtPad = 128;
 
+// unread (min_run > 0): per internal node, 1 when its parent covers at most min_run leaves, so
+// that no FP32 walk reads it (k_refit_small writes no record for it).  The parent's prefix
+// length is the larger of the code deltas at the two ends of the node's range and the parent is
+// every key sharing that prefix, so it covers at most min_run leaves exactly when the key min_run
+// places beyond the range's far end (towards the sibling) no longer shares it.
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
                                                 int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo,
-                                                int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc) {
+                                                int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc,
+                                                uint8_t* __restrict__ unread, int min_run) {
     __shared__ uint64_t win[kBlock + 2 * kBrtPad];
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t m = n - 1;                                  // internal nodes 1 .. n-1
@@ -699,6 +705,16 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         int64_t lo = i < j ? i : j, hi = i < j ? j : i;
         range_hi[i] = (int32_t)hi;
         range_lo[i] = (int32_t)lo;
+        if (min_run > 0) {
+            uint8_t u = 0;
+            if (hi - lo + 1 <= min_run && !(lo == 1 && hi == n)) {
+                const int ldl = lo > 1 ? delta(lo - 1, lo) : -1;
+                const int ldr = hi < n ? delta(hi, hi + 1) : -1;
+                if (ldr > ldl) u = (lo + min_run > n || delta(lo, lo + min_run) < ldr) ? 1 : 0;   // left child
+                else u = (hi - min_run < 1 || delta(hi - min_run, hi) < ldl) ? 1 : 0;            // right child
+            }
+            unread[i] = u;
+        }
         int64_t split;
         if (code(lo) == code(hi)) {
             split = (lo + hi) >> 1;
@@ -958,7 +974,7 @@ template <bool F32>
 __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict__ cols64,
                                                         const float4* __restrict__ leaf32, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
-                                                        const int32_t* __restrict__ parent,
+                                                        const uint8_t* __restrict__ unread_in,
                                                         const int32_t* __restrict__ range_lo,
                                                         const int32_t* __restrict__ range_hi, int64_t n,
                                                         uint32_t* __restrict__ visited, Node64* nodes,
@@ -978,14 +994,15 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
         const int64_t i0 = 1 + it * stride + (int64_t)blockIdx.x * kBlock;
         const int64_t i = i0 + threadIdx.x;
         // the node's own entries first (independent loads, in flight during the staging)
-        int32_t lo = 0, hi = -1, pa = 0, lch = 0, rch = 0, esc = 0;
+        int32_t lo = 0, hi = -1, lch = 0, rch = 0, esc = 0;
+        uint8_t un = 0;
         if (i < n) {
             lo = range_lo[i];
             hi = range_hi[i];
             lch = lr[2 * i];
             rch = lr[2 * i + 1];
             esc = escape[i];
-            if (min_run > 0) pa = parent[i];
+            if (min_run > 0) un = unread_in[i];
         }
         const int64_t w0 = i0 - 1 - kPad > 0 ? i0 - 1 - kPad : 0;
         const int64_t w1 = i0 + kBlock + kPad < n ? i0 + kBlock + kPad : n;
@@ -997,10 +1014,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
         __syncthreads();
         bool take = false;
         if (i < n) {
-            bool unread = false;
-            if (min_run > 0 && hi - lo + 1 <= min_run && pa > 0 && pa < n)
-                unread = range_hi[pa] - range_lo[pa] + 1 <= min_run;
-            if (unread) {
+            if (min_run > 0 && un) {
                 // no record: no FP32 walk reaches this node
             } else if (lo >= 1 && hi <= n && hi - lo + 1 <= small_range) {
                 // small_range <= kPad: the leaf range lies inside the staged window
@@ -1216,11 +1230,12 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     const int small = std::max(kMinRun, std::min(c->tune.refit_small, kRefitPad));
     if (!fp64 && R.leaf32)
         hipLaunchKernelGGL(k_refit_small<true>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
-                           R.parent, R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, kMinRun, listA, &counts[0]);
+                           c->unread.as<uint8_t>(), R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, kMinRun, listA,
+                           &counts[0]);
     else
         hipLaunchKernelGGL(k_refit_small<false>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
-                           R.parent, R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, fp64 ? 0 : kMinRun, listA,
-                           &counts[0]);
+                           c->unread.as<uint8_t>(), R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small,
+                           fp64 ? 0 : kMinRun, listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
         int32_t* cur = (round & 1) ? listB : listA;
@@ -1340,6 +1355,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->range_hi, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->range_lo, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->visited, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->unread, (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->leaf_esc, 4 * (size_t)n));
     PMX_TRY(ensure(c, c->nodes64, sizeof(Node64) * (size_t)(n + 1)));
     int32_t* lr = c->lr.as<int32_t>();
@@ -1350,7 +1366,8 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
         hipLaunchKernelGGL(k_brt, dim3(grid_for(n, kBlock, 1 << 20)), dim3(kBlock), 0, s, codes, n, lr, parent,
-                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape, c->leaf_esc.as<int32_t>());
+                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape, c->leaf_esc.as<int32_t>(),
+                           c->unread.as<uint8_t>(), kMinRun);
     } else {
         hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());
     }
