@@ -655,10 +655,8 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(const double* __restrict__
 constexpr int kBrtPad = 128;
 
 // unread (min_run > 0): per internal node, 1 when its parent covers at most min_run leaves, so
-// that no FP32 walk reads it (k_refit_small writes no record for it).  The parent's prefix
-// length is the larger of the code deltas at the two ends of the node's range and the parent is
-// every key sharing that prefix, so it covers at most min_run leaves exactly when the key min_run
-// places beyond the range's far end (towards the sibling) no longer shares it.
+// that no FP32 walk reads it (k_refit_small writes no record for it).  Each node writes the flag
+// of its internal children (every internal node but the root has one parent; the root's is 0).
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
                                                 int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo,
@@ -705,16 +703,6 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         int64_t lo = i < j ? i : j, hi = i < j ? j : i;
         range_hi[i] = (int32_t)hi;
         range_lo[i] = (int32_t)lo;
-        if (min_run > 0) {
-            uint8_t u = 0;
-            if (hi - lo + 1 <= min_run && !(lo == 1 && hi == n)) {
-                const int ldl = lo > 1 ? delta(lo - 1, lo) : -1;
-                const int ldr = hi < n ? delta(hi, hi + 1) : -1;
-                if (ldr > ldl) u = (lo + min_run > n || delta(lo, lo + min_run) < ldr) ? 1 : 0;   // left child
-                else u = (hi - min_run < 1 || delta(hi - min_run, hi) < ldl) ? 1 : 0;            // right child
-            }
-            unread[i] = u;
-        }
         int64_t split;
         if (code(lo) == code(hi)) {
             split = (lo + hi) >> 1;
@@ -734,6 +722,12 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         int64_t right = (split + 1 == hi) ? hi + off : split + 1;
         lr[2 * i] = (int32_t)left;
         lr[2 * i + 1] = (int32_t)right;
+        if (min_run > 0) {
+            const uint8_t u = hi - lo + 1 <= min_run ? 1 : 0;
+            if (split != lo) unread[split] = u;
+            if (split + 1 != hi) unread[split + 1] = u;
+            if (i == 1) unread[1] = 0;
+        }
         parent[left] = (int32_t)i;
         parent[right] = (int32_t)i;
         int32_t e;
