@@ -364,6 +364,18 @@ def run_batch(a, torch, dev):
     log(f"[bench] graph replay {ms:.1f} ms/step")
     b.close()
     compare = None if a.no_compare else per_call_compare(jobs, torch, dev)
+    roof = None
+    if compare is not None:
+        # pair-stream model of the batch's interpolation (persistent kernel + split items + finish):
+        # one (6 + k)-column FP32 record per accepted (sample, particle) pair, over the eager run's
+        # interpolation phase
+        t_i = eager["interp_ms"] * 1e-3
+        pb = compare["f32c"]["pair_bytes"]
+        roof = {"bound": "hbm", "achieved": pb / t_i / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": pb / t_i / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_batch + k_batch_items + k_batch_finish (eager interpolation phase)",
+                "algorithmic_bytes_per_launch": pb, "bytes_per_unit": 28,
+                "units": compare["f32c"]["accepted_pairs"], "unit_name": "accepted (sample, particle) pairs"}
     line = {
         "metric": "interp grid points/s + LBVH build particles/s at 16M parts, FP32c vs FP64",
         "value": P_total / (ms * 1e-3),
@@ -383,7 +395,7 @@ def run_batch(a, torch, dev):
                                         "split_budget")},
         "per_call": compare,
         "speedup_vs_f64_per_call": None if compare is None else (P_total / (ms * 1e-3)) / compare["f64"]["grid_points_per_s"],
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roof, "cpu_baseline": None,
     }
     print(json.dumps(line))
     return 0
@@ -524,14 +536,17 @@ def run_dusty(a, torch, dev):
                 for _ in species]
         ratio = torch.empty(P, dtype=ft, device=dev)
 
+        kpts = [0.0] * len(species)     # point-interpolation kernel time of the last step, per species
+
         def step():
             tb = ti = 0.0
-            for sp, o in zip(species, outs):
+            for j, (sp, o) in enumerate(zip(species, outs)):
                 ctx = sp["ctx"]
                 ctx.build(*sp["cols"], quant=sp["quant"], kernel=KERNEL_M4)
                 tb += ctx.timings()[0]
                 ctx.interpolate_points(_lib.PMX_SYMMETRIC, prec, cat_p, *dgrid, o["p"])
                 ti += ctx.timings()[5]
+                kpts[j] = ctx.timings()[7]
                 ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat_l, dlo, dld, o["l"])
                 ti += ctx.timings()[5]
             torch.div(outs[1]["p"][0], outs[0]["p"][0], out=ratio)
@@ -549,13 +564,24 @@ def run_dusty(a, torch, dev):
         torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - t) / steps
         return dict(ms_per_step=ms, build_ms=float(np.median(tbs)), interp_ms=float(np.median(tis)), outs=outs,
-                    ratio=ratio)
+                    ratio=ratio, point_kernel_ms=list(kpts))
 
     log(f"[bench] dusty: FP32c steps={a.steps} warmup={a.warmup}")
     r32 = run(_lib.PMX_PREC_F32C, a.steps, a.warmup)
     log(f"[bench] dusty FP32c: {r32['ms_per_step']:.1f} ms/step (builds {r32['build_ms']:.1f}, interpolation "
         f"{r32['interp_ms']:.1f})")
     points_per_step = len(species) * (P + M)
+    # pair-stream model of the dominant kernels (the point tiles of rho, div v, curl v): one
+    # 10-column FP32 record per accepted (sample, particle) pair; each context still holds its
+    # species' tree
+    n_acc = [int(sp["ctx"].point_stats(_lib.PMX_SYMMETRIC, *dgrid)[4]) for sp in species]
+    pb = 4 * (6 + 4) * sum(n_acc)
+    t_k = sum(r32["point_kernel_ms"]) * 1e-3
+    roof = {"bound": "hbm", "achieved": pb / t_k / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": pb / t_k / 1e9 / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "point interpolation kernels (f32c), both species",
+            "algorithmic_bytes_per_launch": pb, "bytes_per_unit": 40, "units": sum(n_acc),
+            "unit_name": "accepted (sample, particle) pairs", "kernel_ms": r32["point_kernel_ms"]}
     res = {"f32c": {k: r32[k] for k in ("ms_per_step", "build_ms", "interp_ms")}}
     res["f32c"]["grid_points_per_s"] = points_per_step / (r32["ms_per_step"] * 1e-3)
     errors = None
@@ -591,7 +617,7 @@ def run_dusty(a, torch, dev):
         "paths": res,
         "speedup_vs_f64": (res["f64"]["ms_per_step"] / res["f32c"]["ms_per_step"]) if "f64" in res else None,
         "field_error_f32c_vs_f64": errors,
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roof, "cpu_baseline": None,
     }
     print(json.dumps(line))
     for sp in species:
@@ -610,6 +636,7 @@ def per_call_compare(jobs, torch, dev):
     for name, prec in (("f32c", _lib.PMX_PREC_F32C), ("f64", _lib.PMX_PREC_F64)):
         ctx = _lib.Context(0)
         tb = ti = 0.0
+        n_acc = pair_bytes = 0
         t0 = time.time()
         for k, j in enumerate([order[0]] + order):
             job = jobs[j]
@@ -625,9 +652,16 @@ def per_call_compare(jobs, torch, dev):
             if k > 0:                      # the first call is the warm-up
                 tb += b_ms
                 ti += ctx.timings()[6]
+                if name == "f32c":         # accepted pairs of the same call (pair-stream model)
+                    st = (ctx.point_stats(job["strategy"], *job["samples"]) if job["kind"] == "points"
+                          else ctx.line_stats(*job["samples"]))
+                    n_acc += int(st[4])
+                    pair_bytes += 4 * (6 + len(job["quant"])) * int(st[4])
         ctx.close()
         log(f"[bench] per-call {name}: builds {tb:.1f} ms, interpolation {ti:.1f} ms ({time.time() - t0:.1f} s)")
         res[name] = {"build_ms": tb, "interp_ms": ti, "grid_points_per_s": P_total / ((tb + ti) * 1e-3)}
+        if name == "f32c":
+            res[name].update(accepted_pairs=n_acc, pair_bytes=pair_bytes)
     return res
 
 
