@@ -956,7 +956,7 @@ __device__ inline void appender_push(BlockAppender& A, bool take, int32_t p, int
     __syncthreads();                                          \
     BlockAppender A{A##_buf, &A##_n, A##_wc, &A##_base}
 
-constexpr int kRefitPad = 64;   // largest leaf range of a node refit directly (staged window margin)
+constexpr int kRefitPad = 128;   // largest leaf range of a node refit directly (staged window margin)
 
 // min_run > 0 (FP32 tree only): a node whose parent covers at most min_run leaves gets no
 // record.  Every FP32 walk hands over a node of at most min_run leaves as one leaf run, so it

@@ -59,7 +59,7 @@ struct Tuning {
     int bucket_lines = 64;
     int tiled_bucket = 1024;
     int tiled_near = 16;
-    int refit_small = 32;
+    int refit_small = 128;
 };
 
 struct Tree {
