@@ -257,6 +257,32 @@ def test_f32c_points_split_tiles(built, strategy, monkeypatch):
         assert max(rel_norms(g, r)) <= F32C_TOL, f"field {j}: {rel_norms(g, r)}"
 
 
+@pytest.mark.parametrize("refit_small", ["16", "40", "64"])
+def test_f32c_refit_range_knob(built, refit_small, monkeypatch):
+    """The leaf range refit directly from the staged leaf window (PMX_REFIT_SMALL, default 128)
+    changes only how the FP32 boxes are computed; the boxes, the records no walk reads and so
+    the accepted pairs of the FP32 walks stay those of the oracle."""
+    monkeypatch.setenv("PMX_REFIT_SMALL", refit_small)
+    c2 = _lib.Context(0)
+    try:
+        cols, quant = disc_columns(N, seed=3, warped=True, dtype=built["dtype"])
+        print(f"[refit knob] PMX_REFIT_SMALL={refit_small} dtype={built['dtype'].__name__}", flush=True)
+        c2.build(*cols, quant=quant, kernel=0)
+        qx, qy, qz = cyl_points(16, 16, 8)
+        off, idx = c2.point_neighbors(O.STRATEGIES["symmetric"], qx, qy, qz, precision=_lib.PMX_PREC_F32C)
+        o, d = disc.faceon_lines(40)
+        dg = c2.line_digest(o, d)
+    finally:
+        c2.close()
+    roff, ridx = built["tree"].point_neighbors("M4", "symmetric", qx, qy, qz)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(idx, ridx)
+    assert dg is not None
+    lroff, lridx = built["tree"].line_neighbors("M4", "scatter", o, d)
+    for g, r in zip(dg, _digest_from_csr(lroff, lridx)):
+        assert np.array_equal(g, r)
+
+
 def test_f32c_lines_fields(built):
     o, d = disc.faceon_lines(32)
     cat = _lib.catalog_struct(scalar_slots=[0, 3], scalar_shepard=[0, 1])
