@@ -175,11 +175,12 @@ def main():
                 for _ in range(w.L)]
 
         def step():
+            # the build returns once its inputs are consumed; its phase times are read after the
+            # interpolation, which is ordered after it on the context's stream
             ctx.build(*w.dcols, quant=w.dquant, kernel=w.kernel)
-            tb = ctx.timings()[0]
             w.interpolate(ctx, prec, outs)
             t = ctx.timings()
-            return tb, t[5], t[7]
+            return t[0], t[5], t[7]
 
         for _ in range(warmup):
             step()
@@ -543,10 +544,11 @@ def run_dusty(a, torch, dev):
             for j, (sp, o) in enumerate(zip(species, outs)):
                 ctx = sp["ctx"]
                 ctx.build(*sp["cols"], quant=sp["quant"], kernel=KERNEL_M4)
-                tb += ctx.timings()[0]
                 ctx.interpolate_points(_lib.PMX_SYMMETRIC, prec, cat_p, *dgrid, o["p"])
-                ti += ctx.timings()[5]
-                kpts[j] = ctx.timings()[7]
+                t = ctx.timings()
+                tb += t[0]
+                ti += t[5]
+                kpts[j] = t[7]
                 ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat_l, dlo, dld, o["l"])
                 ti += ctx.timings()[5]
             torch.div(outs[1]["p"][0], outs[0]["p"][0], out=ratio)
@@ -641,7 +643,6 @@ def per_call_compare(jobs, torch, dev):
         for k, j in enumerate([order[0]] + order):
             job = jobs[j]
             ctx.build(*job["cols"], quant=job["quant"], kernel=KERNEL_M4)
-            b_ms = ctx.timings()[0]
             cat = _lib.catalog_struct(scalar_slots=job["scalar_slots"], scalar_shepard=job["scalar_shepard"])
             P = job["out"][0].shape[0]
             out = [torch.empty(P, dtype=torch.float32 if name == "f32c" else torch.float64, device=dev)]
@@ -650,8 +651,9 @@ def per_call_compare(jobs, torch, dev):
             else:
                 ctx.interpolate_lines(_lib.PMX_SCATTER, prec, cat, job["samples"][0], job["samples"][1], out)
             if k > 0:                      # the first call is the warm-up
-                tb += b_ms
-                ti += ctx.timings()[6]
+                t = ctx.timings()
+                tb += t[0]
+                ti += t[6]
                 if name == "f32c":         # accepted pairs of the same call (pair-stream model)
                     st = (ctx.point_stats(job["strategy"], *job["samples"]) if job["kind"] == "points"
                           else ctx.line_stats(*job["samples"]))

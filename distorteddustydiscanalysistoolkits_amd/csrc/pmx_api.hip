@@ -765,6 +765,8 @@ int pmx_line_kernel_table(int32_t kernel, double* Q, double* Inorm) {
 int pmx_last_timings(const pmx_ctx* c, double* ms, int32_t n) {
     return guard(c, [&]() -> int {
         if (!c || !ms) return PMX_ERR_ARGUMENT;
+        pmx_ctx* m = const_cast<pmx_ctx*>(c);
+        PMX_TRY(settle_build_timings(m));
         for (int i = 0; i < n && i < 10; ++i) ms[i] = c->timings[i];
         return PMX_OK;
     });

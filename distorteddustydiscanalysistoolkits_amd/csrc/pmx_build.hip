@@ -1303,6 +1303,16 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         launch_front<double>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<double>());
     else
         launch_front<float>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<float>());
+    // The encode is the last kernel that reads the caller's columns: the NaN check waits for it,
+    // and the rest of the build runs on after this call returns (every later call on the
+    // context is ordered after it on the context's stream; host outputs wait for it below).
+    if (!async) {
+        int nan_host = 0;
+        PMX_HIP(c, hipMemcpyAsync(&nan_host, nan_count, sizeof(int), hipMemcpyDeviceToHost, s));
+        PMX_HIP(c, hipStreamSynchronize(s));
+        if (nan_host > 0)
+            return set_error(c, PMX_ERR_ARGUMENT, "InexactError: NaN particle coordinate (UInt64(floor(NaN)))");
+    }
     // ---- stable sort + de-dup (the encode counted the first digit) ----
     bool in_alt = false;
     PMX_TRY(radix_sort_pairs(c, codes, order, n, c->keys_tmp.as<uint64_t>(), c->vals_tmp.as<int32_t>(), 0, 64,
@@ -1383,13 +1393,14 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
         return PMX_OK;
     }
     PMX_HIP(c, hipEventRecord(c->ev[3], s));
-    // ---- outputs ----
-    int nan_host = 0;
-    PMX_HIP(c, hipMemcpyAsync(&nan_host, nan_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (codes_out) PMX_HIP(c, hipMemcpyAsync(codes_out, codes, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
-    PMX_HIP(c, hipStreamSynchronize(s));
-    if (nan_host > 0) return set_error(c, PMX_ERR_ARGUMENT, "InexactError: NaN particle coordinate (UInt64(floor(NaN)))");
+    c->build_timings_pending = true;     // read from the events by the first pmx_last_timings
+    // ---- host outputs (each waits for the build) ----
+    if (codes_out) {
+        PMX_HIP(c, hipMemcpyAsync(codes_out, codes, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+        PMX_HIP(c, hipStreamSynchronize(s));
+    }
     if (order_out) {
+        PMX_HIP(c, hipStreamSynchronize(s));
         std::vector<int32_t> tmp((size_t)n);
         PMX_HIP(c, hipMemcpy(tmp.data(), order, 4 * (size_t)n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) order_out[i] = tmp[(size_t)i];
@@ -1424,6 +1435,20 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
             }
         }
     }
+    c->tree.n = n;
+    c->tree.leaf64_ready = false;
+    c->tree.root = n >= 2 ? 1 : 0;
+    c->tree.ncol = p->ncol;
+    c->tree.kernel = p->kernel;
+    c->tree.built = true;
+    c->tree.f32_ready = true;    // the refit and the record gather also wrote the FP32 tree
+    return PMX_OK;
+}
+
+// Phase times of the last synchronous build, from its events (waits for the build to finish).
+int settle_build_timings(pmx_ctx* c) {
+    if (!c->build_timings_pending) return PMX_OK;
+    PMX_HIP(c, hipEventSynchronize(c->ev[3]));
     float ms[3];
     PMX_HIP(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
     PMX_HIP(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
@@ -1432,13 +1457,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     c->timings[1] = ms[0];
     c->timings[2] = ms[2];
     c->timings[3] = ms[1];
-    c->tree.n = n;
-    c->tree.leaf64_ready = false;
-    c->tree.root = n >= 2 ? 1 : 0;
-    c->tree.ncol = p->ncol;
-    c->tree.kernel = p->kernel;
-    c->tree.built = true;
-    c->tree.f32_ready = true;    // the refit and the record gather also wrote the FP32 tree
+    c->build_timings_pending = false;
     return PMX_OK;
 }
 

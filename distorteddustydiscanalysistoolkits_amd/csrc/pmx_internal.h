@@ -113,6 +113,7 @@ struct pmx_ctx {
     pmx::DevBuf tl_sl, tl_tsl;             // nearly parallel sets: per-line slopes (unsorted, sorted), per-tile bounds
     hipEvent_t ev[12] = {};
     double timings[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool build_timings_pending = false;   // timings[0..3] not yet read from the last build's events
 };
 
 namespace pmx {
@@ -184,6 +185,7 @@ constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
                int64_t* order_out);
+int settle_build_timings(pmx_ctx* c);
 // Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), 8 bits per pass; the
 // result is in keys_alt / vals_alt when *result_in_alt.  Both key arrays 16-byte aligned.  hist: the histogram buffer of
 // radix_hist_bytes(n) bytes (null: the context's); first_hist_ready: it already holds the first

@@ -141,7 +141,11 @@ const char *pmx_version(void);
  *   codes_out (n, nullable): de-duplicated sorted codes (enc.codes)
  *   order_out (n, nullable): 0-based original index of each sorted slot (enc.order - 1)
  *   flags & PMX_BUILD_WRITEBACK: the caller's x, y, z, m, h, rho and quant arrays (host or
- *   device memory) are overwritten in Morton order, as the reference's permute! does. */
+ *   device memory) are overwritten in Morton order, as the reference's permute! does.
+ * The call returns once the caller's columns are consumed and checked for NaN (a NaN
+ * coordinate is reported here); without host outputs or write-back the rest of the build
+ * runs on the context's stream after the call returns, ordered before every later call on
+ * the context (pmx_last_timings waits for it). */
 int pmx_build_lbvh(struct pmx_ctx *ctx, const pmx_particles *p, int32_t flags,
                    uint64_t *codes_out, int64_t *order_out);
 
