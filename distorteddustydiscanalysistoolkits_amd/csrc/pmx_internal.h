@@ -105,7 +105,7 @@ struct pmx_ctx {
     void* noalloc_scratch = nullptr;   // this device's address of the no-allocation scratch array
     bool line_inv_h2 = false;  // the current line call weights by Inorm / h^2 (PMX_LINE_INV_H2)
     int split_budget = 4096;   // heavy-tile splitting of FP32c gather / symmetric points (0 = off)
-    int split_item = 4096;
+    int split_item = 2048;     // largest subtree of a split tile queued as one item
     pmx::DevBuf split;         // split-tile counters, item queue and slots (FP32c points)
     pmx::DevBuf gridpts;       // samples of a Cartesian structured grid (x | y | z), then its axes
     pmx::DevBuf records;       // particle records (AoS) for the Morton-order permutation
@@ -179,8 +179,9 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20) {
 
 // build (pmx_build.hip)
 // build_lbvh flag (internal): capture-safe build, see pmx_build.hip
-// capacities of the split-tile storage (slots of FP64 sums, queued subtree items)
-constexpr int kSplitSlots = 65536, kSplitItems = 1 << 21;
+// capacities of the split-tile storage (slots of FP64 sums, queued subtree items: 8 B each; an
+// item that finds the queue full is walked by its tile's wavefront alone, which is far slower)
+constexpr int kSplitSlots = 65536, kSplitItems = 1 << 23;
 constexpr int kBuildAsync = 1 << 30;
 int* nan_count_device(pmx_ctx* c);
 int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_out,
