@@ -231,7 +231,7 @@ struct pmx_batch {
     std::vector<BatchUnit> units;
     DevBuf units_dev, counter, split_buf;
     pts::PointSplit split{};                      // heavy point tiles (gather / symmetric)
-    int split_budget = 4096, split_item = 4096;
+    int split_budget = 4096, split_item = 2048;
     int64_t total_tiles = 0;
     size_t lds_bytes = 0;
     size_t lds_items = 0;                         // dynamic LDS of the split-item kernel
