@@ -524,16 +524,19 @@ int pmx_export_tree(pmx_ctx* c, int32_t* left, int32_t* right, int32_t* parent, 
         PMX_TRY(require_tree(c));
         PMX_HIP(c, hipSetDevice(c->device));
         const int64_t n = c->tree.n, tot = 2 * n - 1;
-        std::vector<int32_t> lr((size_t)(4 * n + 4)), pa((size_t)(2 * n + 2)), es((size_t)(2 * n + 2));
+        std::vector<int32_t> lr((size_t)(4 * n + 4)), pa((size_t)(2 * n + 2)), es((size_t)(2 * n + 2)), le((size_t)n);
         PMX_HIP(c, hipStreamSynchronize(c->stream));
         PMX_HIP(c, hipMemcpy(lr.data(), c->lr.p, 4 * lr.size(), hipMemcpyDeviceToHost));
         PMX_HIP(c, hipMemcpy(pa.data(), c->parent.p, 4 * pa.size(), hipMemcpyDeviceToHost));
         PMX_HIP(c, hipMemcpy(es.data(), c->escape.p, 4 * es.size(), hipMemcpyDeviceToHost));
+        PMX_HIP(c, hipMemcpy(le.data(), c->leaf_esc.p, 4 * le.size(), hipMemcpyDeviceToHost));
+        // ids 1 .. n-1 are internal nodes, n .. 2n-1 the leaves (no children; escapes in leaf_esc)
         for (int64_t id = 1; id <= tot; ++id) {
-            if (left) left[id - 1] = lr[(size_t)(2 * id)];
-            if (right) right[id - 1] = lr[(size_t)(2 * id + 1)];
+            const bool leaf = id >= n;
+            if (left) left[id - 1] = leaf ? 0 : lr[(size_t)(2 * id)];
+            if (right) right[id - 1] = leaf ? 0 : lr[(size_t)(2 * id + 1)];
             if (parent) parent[id - 1] = pa[(size_t)id];
-            if (escape) escape[id - 1] = es[(size_t)id];
+            if (escape) escape[id - 1] = leaf ? le[(size_t)(id - n)] : es[(size_t)id];
         }
         return PMX_OK;
     });

@@ -740,20 +740,15 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         escape[i] = e;
         // leaf escapes (BinaryRadixTree.jl:262-281) by the rule of the internal ones: the
         // top-most node starting right after the leaf (a left child's right sibling and a right
-        // child's parent's escape both start there); leaves have no children
+        // child's parent's escape both start there).  Only leaf_esc holds them: no kernel reads
+        // the leaves' entries of escape / lr (the tree export fills them from leaf_esc and 0).
         auto escape_after = [&](int64_t h) -> int32_t {
             if (h == n) return 0;
             if (h == n - 1) return (int32_t)(off + n);
             const int64_t nx = h + 1;
             return (delta(nx, nx + 1) < delta(nx - 1, nx)) ? (int32_t)(off + nx) : (int32_t)nx;
         };
-        auto leaf_out = [&](int64_t li) {
-            const int32_t le = escape_after(li + 1);
-            escape[n + li] = le;
-            leaf_esc[li] = le;
-            lr[2 * (n + li)] = 0;
-            lr[2 * (n + li) + 1] = 0;
-        };
+        auto leaf_out = [&](int64_t li) { leaf_esc[li] = escape_after(li + 1); };
         leaf_out(i - 1);
         if (i == n - 1) leaf_out(n - 1);
     }
