@@ -410,18 +410,28 @@ __global__ void __launch_bounds__(kBlock, 4) k_radix_scatter(const uint64_t* __r
 
 // ---------------------------------------------------------------------------------
 // De-duplication (sort_by_morton!, MortonCode.jl:189-193): c'_i = max(c_i, c'_{i-1} + 1)
-// has the closed form c'_i = i + max_{j <= i}(c_j - j), computed as a max-scan.
+// has the closed form c'_i = i + max_{j <= i}(c_j - j), computed as a max-scan.  Sorted codes
+// without two equal neighbours increase by at least 1 per index, so c_j - j never decreases and
+// c' = c: the tile pass flags equal neighbours (dup, zeroed with the NaN counter) and the scan
+// and apply passes return at once when there are none.
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_dedup_tilemax(const uint64_t* __restrict__ c, int64_t n,
-                                                          int64_t* __restrict__ tmax) {
+                                                          int64_t* __restrict__ tmax, int* __restrict__ dup) {
     __shared__ int64_t s[kBlock];
     const int64_t base = (int64_t)blockIdx.x * kTile;
     int64_t m = INT64_MIN;
+    int eq = 0;
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         int64_t i = base + r * kBlock + threadIdx.x;
-        if (i < n) { int64_t d = (int64_t)c[i] - i; m = d > m ? d : m; }
+        if (i < n) {
+            const uint64_t ci = c[i];
+            int64_t d = (int64_t)ci - i;
+            m = d > m ? d : m;
+            if (i > 0 && c[i - 1] == ci) eq = 1;
+        }
     }
+    if (__syncthreads_or(eq) && threadIdx.x == 0) *dup = 1;
     s[threadIdx.x] = m;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -432,7 +442,9 @@ __global__ void __launch_bounds__(kBlock) k_dedup_tilemax(const uint64_t* __rest
 }
 
 // exclusive max-scan of tile maxima (single workgroup)
-__global__ void __launch_bounds__(kBlock) k_dedup_scan(int64_t* __restrict__ tmax, int ntiles) {
+__global__ void __launch_bounds__(kBlock) k_dedup_scan(int64_t* __restrict__ tmax, int ntiles,
+                                                       const int* __restrict__ dup) {
+    if (*dup == 0) return;
     __shared__ int64_t s[kBlock];
     int64_t carry = INT64_MIN;
     for (int base = 0; base < ntiles; base += kBlock) {
@@ -456,7 +468,8 @@ __global__ void __launch_bounds__(kBlock) k_dedup_scan(int64_t* __restrict__ tma
 }
 
 __global__ void __launch_bounds__(kBlock) k_dedup_apply(uint64_t* __restrict__ c, int64_t n,
-                                                        const int64_t* __restrict__ tpre) {
+                                                        const int64_t* __restrict__ tpre, const int* __restrict__ dup) {
+    if (*dup == 0) return;
     // Four rounds of kBlock x 4 consecutive codes (coalesced): per-thread running max, a
     // wavefront max-scan, then the wavefronts combined with the carry of earlier rounds.
     constexpr int kVec = 4;
@@ -1293,7 +1306,8 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->records, esz * (size_t)n * ncols));
     ColPtrs cp;
     for (int j = 0; j < ncols; ++j) cp.p[j] = src[j];
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 1, 0u);
+    int* dup = nan_count + 1;     // equal neighbouring codes after the sort (de-dup needed)
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 2, 0u);
     if (p->dtype == PMX_F64)
         launch_front<double>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<double>());
     else
@@ -1320,9 +1334,9 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     }
     const int ntiles = (int)((n + kTile - 1) / kTile);
     int64_t* tmax = reinterpret_cast<int64_t*>(c->keys_tmp.p);   // free after the sort
-    hipLaunchKernelGGL(k_dedup_tilemax, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
-    hipLaunchKernelGGL(k_dedup_scan, dim3(1), dim3(kBlock), 0, s, tmax, ntiles);
-    hipLaunchKernelGGL(k_dedup_apply, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax);
+    hipLaunchKernelGGL(k_dedup_tilemax, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax, dup);
+    hipLaunchKernelGGL(k_dedup_scan, dim3(1), dim3(kBlock), 0, s, tmax, ntiles, dup);
+    hipLaunchKernelGGL(k_dedup_apply, dim3(ntiles), dim3(kBlock), 0, s, codes, n, tmax, dup);
     if (!async) PMX_HIP(c, hipEventRecord(c->ev[1], s));
     // ---- columns into Morton order ----
     const bool f32in = p->dtype == PMX_F32;
