@@ -316,7 +316,7 @@ void pmx_destroy(pmx_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->lut64, &c->lutf, &c->codes, &c->order, &c->keys_tmp, &c->vals_tmp, &c->cols64,
                       &c->leaf64, &c->leaf_esc, &c->nodes64, &c->nodes32, &c->leaf32, &c->pay32,
-                      &c->lr, &c->parent, &c->escape, &c->visited, &c->unread, &c->range_hi, &c->range_lo, &c->scratch_a,
+                      &c->lr, &c->parent, &c->escape, &c->visited, &c->unread, &c->range_far, &c->scratch_a,
                       &c->scratch_b, &c->scratch_c, &c->hist, &c->staging, &c->outbuf, &c->qbuf, &c->misc,
                       &c->tl_st, &c->tl_part, &c->tl_cells, &c->tl_sl, &c->tl_tsl, &c->records, &c->split, &c->gridpts};
     for (DevBuf* b : bufs)

@@ -672,7 +672,7 @@ constexpr int kBrtPad = 128;
 // of its internal children (every internal node but the root has one parent; the root's is 0).
 __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, int64_t n,
                                                 int32_t* __restrict__ lr, int32_t* __restrict__ parent,
-                                                int32_t* __restrict__ range_hi, int32_t* __restrict__ range_lo,
+                                                int32_t* __restrict__ range_far,
                                                 int32_t* __restrict__ escape, int32_t* __restrict__ leaf_esc,
                                                 uint8_t* __restrict__ unread, int min_run) {
     __shared__ uint64_t win[kBlock + 2 * kBrtPad];
@@ -714,8 +714,7 @@ __global__ void __launch_bounds__(kBlock) k_brt(const uint64_t* __restrict__ c, 
         }
         int64_t j = i + l * d;
         int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-        range_hi[i] = (int32_t)hi;
-        range_lo[i] = (int32_t)lo;
+        range_far[i] = (int32_t)j;     // the range's other end (one end is i)
         int64_t split;
         if (code(lo) == code(hi)) {
             split = (lo + hi) >> 1;
@@ -812,8 +811,17 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, i
 // The FP32 copy of a node used by the FP32-compensated traversals: box rounded outward,
 // hmax rounded up (every FP32 node test stays conservative), plus its 0-based leaf run.
 // `nodes` is null when only the FP32 tree is built (the FP64 nodes are refitted on first use).
-__device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __restrict__ rlo,
-                                  const int32_t* __restrict__ rhi, Node64* nodes, Node32* __restrict__ n32) {
+// Leaf range [lo, hi] (1-based) of internal node i: one end is i itself, the other range_far[i].
+__device__ inline void node_range(const int32_t* __restrict__ far, int64_t i, int32_t& lo, int32_t& hi) {
+    const int32_t j = far[i], k = (int32_t)i;
+    lo = j < k ? j : k;
+    hi = j < k ? k : j;
+}
+
+__device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __restrict__ far, Node64* nodes,
+                                  Node32* __restrict__ n32) {
+    int32_t lo, hi;
+    node_range(far, i, lo, hi);
     if (nodes) nodes[i] = a;
     Node32 b;
     b.mnx = f32_down(a.mn[0]); b.mny = f32_down(a.mn[1]); b.mnz = f32_down(a.mn[2]);
@@ -821,8 +829,8 @@ __device__ inline void store_node(const Node64& a, int64_t i, const int32_t* __r
     b.hmax = f32_up(a.hmax);
     b.left = a.left;
     b.escape = a.escape;
-    b.lo = rlo[i] - 1;
-    b.cnt = rhi[i] - rlo[i] + 1;
+    b.lo = lo - 1;
+    b.cnt = hi - lo + 1;
     b.pad = 0;
     n32[i] = b;
 }
@@ -849,8 +857,7 @@ __device__ inline Leaf64 read_leaf(const double* __restrict__ cols64, const floa
 __device__ inline void combine_node(const double* __restrict__ cols64, const float4* __restrict__ leaf32,
                                     const int32_t* __restrict__ lr,
                                     const int32_t* __restrict__ escape, int64_t n, Node64* nodes, int32_t p,
-                                    const int32_t* __restrict__ rlo, const int32_t* __restrict__ rhi,
-                                    Node32* __restrict__ n32) {
+                                    const int32_t* __restrict__ far, Node32* __restrict__ n32) {
     const int32_t l = lr[2 * p], r = lr[2 * p + 1];
     double lmn[3], lmx[3], rmn[3], rmx[3], hl, hr;
     if (l >= n) {
@@ -891,7 +898,7 @@ __device__ inline void combine_node(const double* __restrict__ cols64, const flo
     out.hmax = hl > hr ? hl : hr;
     out.left = l;
     out.escape = escape[p];
-    store_node(out, p, rlo, rhi, nodes, n32);
+    store_node(out, p, far, nodes, n32);
 }
 
 // Append `p` (when `take`) to the next round's list with one atomic per wavefront.
@@ -977,8 +984,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                                                         const float4* __restrict__ leaf32, const int32_t* __restrict__ lr,
                                                         const int32_t* __restrict__ escape,
                                                         const uint8_t* __restrict__ unread_in,
-                                                        const int32_t* __restrict__ range_lo,
-                                                        const int32_t* __restrict__ range_hi, int64_t n,
+                                                        const int32_t* __restrict__ range_far, int64_t n,
                                                         uint32_t* __restrict__ visited, Node64* nodes,
                                                         Node32* __restrict__ n32, int small_range, int min_run,
                                                         int32_t* __restrict__ next, uint32_t* __restrict__ next_count) {
@@ -999,8 +1005,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
         int32_t lo = 0, hi = -1, lch = 0, rch = 0, esc = 0;
         uint8_t un = 0;
         if (i < n) {
-            lo = range_lo[i];
-            hi = range_hi[i];
+            node_range(range_far, i, lo, hi);
             lch = lr[2 * i];
             rch = lr[2 * i + 1];
             esc = escape[i];
@@ -1055,7 +1060,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                     out.hmax = hm;
                     out.left = lch;
                     out.escape = esc;
-                    store_node(out, i, range_lo, range_hi, nodes, n32);
+                    store_node(out, i, range_far, nodes, n32);
                 }
             } else {
                 uint32_t done = 0;
@@ -1063,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock) k_refit_small(const double* __restrict
                 for (int side = 0; side < 2; ++side) {
                     const int32_t ch = chs[side];
                     if (ch >= n) done += 1;
-                    else if (ch > 0 && range_hi[ch] - range_lo[ch] + 1 <= small_range) done += 1;
+                    else if (ch > 0 && abs(range_far[ch] - ch) + 1 <= small_range) done += 1;
                 }
                 visited[i] = done;
                 take = done == 2;
@@ -1082,8 +1087,7 @@ struct RefitTree {
     const int32_t* lr;
     const int32_t* parent;
     const int32_t* escape;
-    const int32_t* rlo;
-    const int32_t* rhi;
+    const int32_t* far;         // other end of each internal node's leaf range
     int64_t n;
     uint32_t* visited;
     Node64* nodes;
@@ -1097,7 +1101,7 @@ __device__ inline void refit_step(const RefitTree& R, int32_t x, bool valid, int
     bool take = false;
     int32_t p = 0;
     if (valid && x > 0 && x < n) {
-        combine_node(R.cols64, R.leaf32, R.lr, R.escape, n, R.nodes, x, R.rlo, R.rhi, R.n32);
+        combine_node(R.cols64, R.leaf32, R.lr, R.escape, n, R.nodes, x, R.far, R.n32);
         const int32_t* parent = R.parent;
         uint32_t* visited = R.visited;
         p = parent[x];
@@ -1222,8 +1226,7 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     R.lr = c->lr.as<int32_t>();
     R.parent = c->parent.as<int32_t>();
     R.escape = c->escape.as<int32_t>();
-    R.rlo = c->range_lo.as<int32_t>();
-    R.rhi = c->range_hi.as<int32_t>();
+    R.far = c->range_far.as<int32_t>();
     R.n = n;
     R.visited = c->visited.as<uint32_t>();
     R.nodes = fp64 ? c->nodes64.as<Node64>() : nullptr;
@@ -1232,11 +1235,11 @@ int refit_rounds(pmx_ctx* c, int64_t n, unsigned g, bool fp64) {
     const int small = std::max(kMinRun, std::min(c->tune.refit_small, kRefitPad));
     if (!fp64 && R.leaf32)
         hipLaunchKernelGGL(k_refit_small<true>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
-                           c->unread.as<uint8_t>(), R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small, kMinRun, listA,
+                           c->unread.as<uint8_t>(), R.far, n, R.visited, R.nodes, R.n32, small, kMinRun, listA,
                            &counts[0]);
     else
         hipLaunchKernelGGL(k_refit_small<false>, dim3(gs), dim3(kBlock), 0, s, R.cols64, R.leaf32, R.lr, R.escape,
-                           c->unread.as<uint8_t>(), R.rlo, R.rhi, n, R.visited, R.nodes, R.n32, small,
+                           c->unread.as<uint8_t>(), R.far, n, R.visited, R.nodes, R.n32, small,
                            fp64 ? 0 : kMinRun, listA, &counts[0]);
     const unsigned grr = std::max(1u, std::min(g / 4, 1024u));
     for (int round = 0; round < kLaunchRounds; ++round) {
@@ -1365,8 +1368,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     PMX_TRY(ensure(c, c->lr, 8 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->parent, 4 * (size_t)(2 * n + 2)));
     PMX_TRY(ensure(c, c->escape, 4 * (size_t)(2 * n + 2)));
-    PMX_TRY(ensure(c, c->range_hi, 4 * (size_t)(n + 2)));
-    PMX_TRY(ensure(c, c->range_lo, 4 * (size_t)(n + 2)));
+    PMX_TRY(ensure(c, c->range_far, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->visited, 4 * (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->unread, (size_t)(n + 2)));
     PMX_TRY(ensure(c, c->leaf_esc, 4 * (size_t)n));
@@ -1379,7 +1381,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     hipLaunchKernelGGL(k_tree_heads, dim3(1), dim3(64), 0, s, lr, parent, escape);
     if (n >= 2) {
         hipLaunchKernelGGL(k_brt, dim3(grid_for(n, kBlock, 1 << 20)), dim3(kBlock), 0, s, codes, n, lr, parent,
-                           c->range_hi.as<int32_t>(), c->range_lo.as<int32_t>(), escape, c->leaf_esc.as<int32_t>(),
+                           c->range_far.as<int32_t>(), escape, c->leaf_esc.as<int32_t>(),
                            c->unread.as<uint8_t>(), kMinRun);
     } else {
         hipLaunchKernelGGL(k_escape_leaf, dim3(g), dim3(kBlock), 0, s, lr, parent, n, escape, c->leaf_esc.as<int32_t>());

@@ -97,7 +97,7 @@ struct pmx_ctx {
     pmx::DevBuf leaf32;        // float4[n] x y z h
     pmx::DevBuf pay32;         // float payload for the compensated path
     pmx::DevBuf lr, parent, escape;  // int32 [2n] by unified id (left/right interleaved in lr)
-    pmx::DevBuf visited, range_hi, range_lo;
+    pmx::DevBuf visited, range_far;   // range_far: other end of each internal node's leaf range
     pmx::DevBuf unread;        // u8 by internal id: 1 when no FP32 walk reads the node (its parent covers <= kMinRun leaves)
     pmx::DevBuf scratch_a, scratch_b, scratch_c, hist, staging, outbuf, qbuf, misc;
     int ncu = 256;             // compute units of the device
