@@ -157,7 +157,8 @@ struct ColPtrs {
     const void* p[24];
 };
 
-// _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez; value = index.
+// _encode_morton_code3D (MortonCode.jl:275-305): code = ex<<2 | ey<<1 | ez (the sort generates the
+// index values itself in its first pass).
 // One workgroup per 4096-key tile of the sort, which also counts the tile's first digit
 // (bits 0-7) as k_radix_hist would, so the sort's first pass needs no histogram pass, and writes
 // each particle's interleaved input record (x y z m h rho quant...; the columns are then read
@@ -167,7 +168,7 @@ struct ColPtrs {
 template <typename T, int NC>
 __global__ void __launch_bounds__(kBlock) k_encode(ColPtrs src, int ncols_rt, int64_t n,
                                                    const double* __restrict__ box, int ntiles,
-                                                   uint64_t* __restrict__ codes, int32_t* __restrict__ vals,
+                                                   uint64_t* __restrict__ codes,
                                                    uint32_t* __restrict__ hist, T* __restrict__ rec) {
     __shared__ uint32_t cnt[kBins];
     cnt[threadIdx.x] = 0;
@@ -213,7 +214,6 @@ __global__ void __launch_bounds__(kBlock) k_encode(ColPtrs src, int ncols_rt, in
             const uint64_t iz = quantize((double)zv, z0, z1);
             const uint64_t code = (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
             codes[i] = code;
-            vals[i] = (int32_t)i;
             atomicAdd(&cnt[code & (kBins - 1)], 1u);
         }
     }
@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* __restrict__ hi
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+template <bool IDX>
 __global__ void __launch_bounds__(kBlock, 4) k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                           const int32_t* __restrict__ vals_in, int64_t n,
                                                           int shift, int ntiles,
@@ -399,7 +400,10 @@ __global__ void __launch_bounds__(kBlock, 4) k_radix_scatter(const uint64_t* __r
     int32_t* s_v = reinterpret_cast<int32_t*>(s_stage);
 #pragma unroll
     for (int r = 0; r < kItems; ++r)
-        if (dr[r] != 0xffffffffu) s_v[dr[r]] = vals_in[wbase + r * 64 + lane];
+        if (dr[r] != 0xffffffffu) {
+            if constexpr (IDX) s_v[dr[r]] = (int32_t)(wbase + r * 64 + lane);   // the values are the indices
+            else s_v[dr[r]] = vals_in[wbase + r * 64 + lane];
+        }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kItems; ++q) {
@@ -1160,7 +1164,7 @@ __global__ void k_root_box(const double* __restrict__ box, const Node32* __restr
 
 template <typename T>
 int launch_front(pmx_ctx* c, const ColPtrs& cp, int ncols, int64_t n, double* part, int nparts, int* nan_count,
-                 double* box, uint64_t* codes, int32_t* vals, uint32_t* hist, T* rec) {
+                 double* box, uint64_t* codes, uint32_t* hist, T* rec) {
     hipLaunchKernelGGL(k_bbox_partial<T>, dim3(nparts), dim3(kBlock), 0, c->stream, (const T*)cp.p[0],
                        (const T*)cp.p[1], (const T*)cp.p[2], n, part, nan_count);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, part, nparts, box);
@@ -1168,12 +1172,12 @@ int launch_front(pmx_ctx* c, const ColPtrs& cp, int ncols, int64_t n, double* pa
     const dim3 g(ntiles), b(kBlock);
     hipStream_t s = c->stream;
     switch (ncols) {
-    case 6: hipLaunchKernelGGL((k_encode<T, 6>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
-    case 7: hipLaunchKernelGGL((k_encode<T, 7>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
-    case 8: hipLaunchKernelGGL((k_encode<T, 8>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
-    case 9: hipLaunchKernelGGL((k_encode<T, 9>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
-    case 10: hipLaunchKernelGGL((k_encode<T, 10>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
-    default: hipLaunchKernelGGL((k_encode<T, 0>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, vals, hist, rec); break;
+    case 6: hipLaunchKernelGGL((k_encode<T, 6>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
+    case 7: hipLaunchKernelGGL((k_encode<T, 7>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
+    case 8: hipLaunchKernelGGL((k_encode<T, 8>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
+    case 9: hipLaunchKernelGGL((k_encode<T, 9>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
+    case 10: hipLaunchKernelGGL((k_encode<T, 10>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
+    default: hipLaunchKernelGGL((k_encode<T, 0>), g, b, 0, s, cp, ncols, n, box, ntiles, codes, hist, rec); break;
     }
     return PMX_OK;
 }
@@ -1182,7 +1186,7 @@ int launch_front(pmx_ctx* c, const ColPtrs& cp, int ncols, int64_t n, double* pa
 
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
                      int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist,
-                     bool first_hist_ready) {
+                     bool first_hist_ready, bool vals_are_indices) {
     static_assert(kTile == 4096 && kBins == 256, "radix_hist_bytes assumes 4096-key tiles of 8-bit digits");
     const int ntiles = (int)((n + kTile - 1) / kTile);
     if (!hist) {
@@ -1197,8 +1201,12 @@ int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint6
         if (!(first_hist_ready && shift == begin_bit))
             hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, n, shift, ntiles, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(kBins), dim3(kBlock), 0, c->stream, hist, ntiles, totals);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, vin, n, shift, ntiles,
-                           hist, totals, kout, vout);
+        if (vals_are_indices && shift == begin_bit)
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, vin, n, shift,
+                               ntiles, hist, totals, kout, vout);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(ntiles), dim3(kBlock), 0, c->stream, kin, vin, n, shift,
+                               ntiles, hist, totals, kout, vout);
         std::swap(kin, kout);
         std::swap(vin, vout);
         alt = !alt;
@@ -1312,9 +1320,9 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     int* dup = nan_count + 1;     // equal neighbouring codes after the sort (de-dup needed)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, (uint32_t*)nan_count, 2, 0u);
     if (p->dtype == PMX_F64)
-        launch_front<double>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<double>());
+        launch_front<double>(c, cp, ncols, n, part, nparts, nan_count, box, codes, hist, c->records.as<double>());
     else
-        launch_front<float>(c, cp, ncols, n, part, nparts, nan_count, box, codes, order, hist, c->records.as<float>());
+        launch_front<float>(c, cp, ncols, n, part, nparts, nan_count, box, codes, hist, c->records.as<float>());
     // The encode is the last kernel that reads the caller's columns: the NaN check waits for it,
     // and the rest of the build runs on after this call returns (every later call on the
     // context is ordered after it on the context's stream; host outputs wait for it below).
@@ -1328,7 +1336,7 @@ int build_lbvh(pmx_ctx* c, const pmx_particles* p, int flags, uint64_t* codes_ou
     // ---- stable sort + de-dup (the encode counted the first digit) ----
     bool in_alt = false;
     PMX_TRY(radix_sort_pairs(c, codes, order, n, c->keys_tmp.as<uint64_t>(), c->vals_tmp.as<int32_t>(), 0, 64,
-                             &in_alt, hist, true));
+                             &in_alt, hist, true, true));
     if (in_alt) {
         std::swap(c->codes, c->keys_tmp);
         std::swap(c->order, c->vals_tmp);

@@ -190,10 +190,11 @@ int settle_build_timings(pmx_ctx* c);
 // Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), 8 bits per pass; the
 // result is in keys_alt / vals_alt when *result_in_alt.  Both key arrays 16-byte aligned.  hist: the histogram buffer of
 // radix_hist_bytes(n) bytes (null: the context's); first_hist_ready: it already holds the first
-// pass's per-tile digit counts (k_encode writes them).
+// pass's per-tile digit counts (k_encode writes them); vals_are_indices: the input values are
+// 0 .. n-1 and are not read (vals is then only written, by the second pass).
 int radix_sort_pairs(pmx_ctx* c, uint64_t* keys, int32_t* vals, int64_t n, uint64_t* keys_alt,
                      int32_t* vals_alt, int begin_bit, int end_bit, bool* result_in_alt, uint32_t* hist = nullptr,
-                     bool first_hist_ready = false);
+                     bool first_hist_ready = false, bool vals_are_indices = false);
 constexpr size_t radix_hist_bytes(int64_t n) { return sizeof(uint32_t) * 256 * (size_t)((n + 4095) / 4096 + 1); }
 
 // Device work memory of one line-map call.  The per-call entry points take it from the
